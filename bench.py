@@ -1,0 +1,218 @@
+"""Benchmark: fused batched LOITS forward + VJP on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-plain] [--no-cpu]
+
+Workload (one "step"): B = 256 float32 densities of N = 2048 nodes on
+x in [0.001, 0.999] (the density family x^a (1-x)^b + Gaussian bump, seeded),
+2^18 events per density (67,108,864 events), Philox seed 1234, sequence 0:
+the batched forward (curve tables, sampling, compaction) followed by the
+reverse VJP w.r.t. the densities for an N(0,1) float32 cotangent.  Inputs are
+resident in HBM before the timed region; nothing is skipped inside it.
+
+Printed JSON line: events/s for the whole step; the roofline of the dominant
+kernel (algorithmic bytes / its HIP-event-timed average duration); the
+per-instance-launch comparator ("plain": the same kernels launched once per
+density); and the CPU baseline (the reference's own OpenMP core, compiled
+from its sources into oracle/_ref, on a bounded sample of the same workload).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+HBM_PEAK_GBS = 8000.0
+B, N, E = 256, 2048, 1 << 18
+SEED, SEQUENCE = 1234, 0
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=20)
+    p.add_argument("--warmup", type=int, default=3)
+    p.add_argument("--no-plain", action="store_true")
+    p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--threads", type=int, default=1024)
+    p.add_argument("--chunk", type=int, default=1 << 18)
+    p.add_argument("--guide-shift", type=int, default=1)
+    p.add_argument("--scan-mode", type=int, default=0)
+    return p.parse_args()
+
+
+def algorithmic_bytes(kept, nodes, s=4):
+    """Minimum traffic the API forces (float32 sizes): forward reads x and the
+    density and writes the kept events; the VJP reads the cotangents, x and the
+    density and writes the density gradient."""
+    fwd = nodes * (s + s) + kept * s
+    bwd = kept * s + nodes * (s + s + s)
+    return fwd, bwd
+
+
+def cpu_baseline():
+    """The reference's OpenMP core (oracle/_ref) on a bounded sample: forward
+    and backward of a 1-D-mapped reference call with N = 2048 nodes per curve."""
+    sys.path.insert(0, str(ROOT))
+    from oracle.oracle import Oracle, available
+    from quantom_loits_amd.workloads import reference_1d_inputs, uniform_family
+
+    if available("openmp", portable=True):
+        lib, kind = Oracle("openmp", portable=True), "reference"
+    else:
+        lib, kind = Oracle("port"), "port"
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    b = max(threads, 16)
+    x, d = uniform_family(b, N, seed=0, dtype=np.float32)
+    args = reference_1d_inputs(x, d)
+
+    def run(e):
+        t0 = time.perf_counter()
+        st = lib.forward(*args, e, SEED, SEQUENCE)
+        g = np.zeros_like(st["events"])
+        g[:, 0] = 1.0
+        lib.backward(g, args[0], args[1], args[2], args[3], args[5], st)
+        return time.perf_counter() - t0, st["events"].shape[0]
+
+    dt, _ = run(1024)  # calibrate
+    per_event = max(dt / (b * 1024), 1e-9)
+    e = int(min(1 << 16, max(1024, 10.0 / per_event / b)))
+    dt, n = run(e)
+    return dict(value=n / dt, unit="events/s", cores=threads, kind=kind,
+                sample=f"{b} curves x N={N} x {e} events fwd+VJP through the reference "
+                       f"{'openmp core' if kind == 'reference' else 'oracle port'} "
+                       f"(1-D mapping), {dt:.1f} s wall")
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 "
+                                   "(the batched path shards by instances, see DESIGN.md)"}))
+        raise SystemExit(2)
+    import torch
+
+    sys.path.insert(0, str(ROOT))
+    from quantom_loits_amd.batched import BatchedLOITS, Tuning
+    from quantom_loits_amd.workloads import uniform_family
+
+    dev = torch.device("cuda", 0)
+    x_np, d_np = uniform_family(B, N, seed=0, dtype=np.float32)
+    x = torch.from_numpy(x_np.reshape(-1)).to(dev)
+    d = torch.from_numpy(d_np.reshape(-1)).to(dev)
+    tun = Tuning(threads=args.threads, guide_shift=args.guide_shift, chunk_events=args.chunk,
+                 scan_mode=args.scan_mode)
+    s = BatchedLOITS.uniform(B, N, E, SEQUENCE, dtype=torch.float32, device=dev, tuning=tun)
+    st = s.new_state()
+    gen = torch.Generator(device=dev).manual_seed(1)
+    grad = torch.randn(B * E, generator=gen, device=dev, dtype=torch.float32)
+    gd = torch.empty_like(d)
+    stream = torch.cuda.current_stream()
+
+    def step(ev=None):
+        if ev is not None:
+            ev[0].record(stream)
+        s.forward_raw(x, d, SEED, st)
+        if ev is not None:
+            ev[1].record(stream)
+        s.backward_raw(x, d, SEED, st, grad, gd)
+        if ev is not None:
+            ev[2].record(stream)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    total = int(st["total"].item())
+    dense = s.dense_flag(st)
+    events = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        step(events[k])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    fwd_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in events]))
+    bwd_ms = float(np.mean([e[1].elapsed_time(e[2]) for e in events]))
+    ms_per_step = wall * 1e3 / args.steps
+    fwd_b, bwd_b = algorithmic_bytes(total, B * N)
+    step_gbs = (fwd_b + bwd_b) / (ms_per_step * 1e-3) / 1e9
+
+    if bwd_ms >= fwd_ms:
+        dom = dict(name="vjp_kernel (batched VJP)", ms=bwd_ms, bytes=bwd_b)
+    else:
+        dom = dict(name="tables+sample+finalize (batched forward)", ms=fwd_ms, bytes=fwd_b)
+    achieved = dom["bytes"] / (dom["ms"] * 1e-3) / 1e9
+    traffic = None
+    tfile = ROOT / "profiles" / "traffic.json"
+    if tfile.is_file():
+        try:
+            traffic = json.loads(tfile.read_text()).get(dom["name"].split()[0])
+        except (ValueError, OSError):
+            traffic = None
+
+    plain = None
+    if not args.no_plain:
+        singles = [BatchedLOITS(np.array([N]), np.array([E]), np.array([i * E], np.uint64),
+                                np.array([2 * SEQUENCE], np.uint64), None, np.array([i * N]),
+                                dtype=torch.float32, device=dev,
+                                tuning=Tuning(threads=1024, chunk_events=1 << 14))
+                   for i in range(B)]
+        sts = [p.new_state() for p in singles]
+        gds = torch.empty_like(d)
+
+        def plain_step():
+            for i, (p, sti) in enumerate(zip(singles, sts)):
+                p.forward_raw(x, d, SEED, sti)
+                p.backward_raw(x, d, SEED, sti, grad[i * E:(i + 1) * E], gds)
+
+        plain_step()
+        torch.cuda.synchronize()
+        ps = max(2, args.steps // 5)
+        t1 = time.perf_counter()
+        for _ in range(ps):
+            plain_step()
+        torch.cuda.synchronize()
+        plain_ms = (time.perf_counter() - t1) * 1e3 / ps
+        plain = dict(ms_per_step=plain_ms, events_per_s=B * E / (plain_ms * 1e-3),
+                     speedup=plain_ms / ms_per_step)
+
+    cpu = None if args.no_cpu else cpu_baseline()
+    out = {
+        "metric": "events/s fwd+VJP, B=256 N=2048, fused batched vs plain; % HBM peak",
+        "value": B * E / (ms_per_step * 1e-3),
+        "unit": "events/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded density family, N(0,1) cotangent)",
+        "config": {"workload": "batched LOITS forward + VJP, B=256 densities, N=2048 nodes, "
+                               "2^18 events each (67,108,864 events), float32",
+                   "instances": B, "nodes": N, "events_per_instance": E,
+                   "kept_events": total, "all_kept": dense,
+                   "tuning": {"threads": tun.threads, "chunk_events": tun.chunk_events,
+                              "guide_shift": tun.guide_shift, "scan_mode": tun.scan_mode}},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": dom["name"], "kernel_ms": dom["ms"],
+                     "algorithmic_bytes": dom["bytes"]},
+        "step": {"forward_ms": fwd_ms, "backward_ms": bwd_ms,
+                 "algorithmic_bytes": fwd_b + bwd_b, "achieved_gbs": step_gbs,
+                 "frac": step_gbs / HBM_PEAK_GBS},
+        "plain": plain,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,189 @@
+"""Batched 1-D LOITS: many x-curves of uneven sizes in single launches.
+
+``BatchedLOITS`` samples ``events[i]`` values from each instance's density
+(the reference's x-axis sampler under the 1-D mapping nx = ny = 1, see
+cpp/loits_core.cpp:55-338) and returns them packed, instance after instance,
+in slot order (the compaction order of cpp/loits_core.cpp:239-266).  Its
+autograd backward is the hand-written HIP VJP w.r.t. the densities, which
+recomputes the Philox stream instead of storing uniforms or bin indices.
+
+Counter mapping: instance i, slot s uses Philox counter ``counter_base[i] + s``
+on stream ``streams[i]``.  ``BatchedLOITS.uniform(B, N, E, sequence)`` uses
+``counter_base = i * E`` and stream ``2 * sequence``: the x column of a
+reference call on the same B curves with weights 1 and ``n_events = E``.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from ._native import load_extension
+
+DTYPE_CODE = {torch.float64: 0, torch.float32: 1}
+
+
+@dataclass
+class Tuning:
+    threads: int = 1024
+    guide_shift: int = 1
+    chunk_events: int = 1 << 18
+    scan_mode: int = 0        # 0 parallel CDF scan, 1 the reference's sequential chain
+    persistent: int = 0
+
+
+class BatchedLOITS:
+    """Plan and launch the batched forward / pack / VJP for fixed instance sizes."""
+
+    def __init__(self, n_nodes, events, counter_base=None, streams=None, accept=None,
+                 node_off=None, *, dtype=torch.float32, device="cuda", tuning=None, window=None):
+        n_nodes = np.asarray(n_nodes, dtype=np.int64).reshape(-1)
+        events = np.asarray(events, dtype=np.int64).reshape(-1)
+        n = n_nodes.shape[0]
+        if events.shape[0] != n:
+            raise ValueError("n_nodes and events must have one entry per instance")
+        if node_off is None:
+            node_off = np.concatenate([[0], np.cumsum(n_nodes)[:-1]]).astype(np.int64)
+        counter_base = (np.zeros(n, np.uint64) if counter_base is None
+                        else np.asarray(counter_base, dtype=np.uint64).reshape(-1))
+        streams = (np.zeros(n, np.uint64) if streams is None
+                   else np.asarray(streams, dtype=np.uint64).reshape(-1))
+        accept = np.ones(n, np.int64) if accept is None else np.asarray(accept).astype(np.int64)
+        out_off = np.concatenate([[0], np.cumsum(events)[:-1]]).astype(np.int64)
+        table = np.zeros((n, 8), dtype=np.int64)
+        table[:, 0] = node_off
+        table[:, 1] = n_nodes
+        table[:, 2] = events
+        table[:, 3] = out_off
+        table[:, 4] = counter_base.view(np.int64)
+        table[:, 5] = streams.view(np.int64)
+        table[:, 6] = accept
+        if dtype not in DTYPE_CODE:
+            raise TypeError("dtype must be torch.float32 or torch.float64")
+        self.dtype = dtype
+        self.device = torch.device(device)
+        self.tuning = tuning or Tuning()
+        self.window = window
+        ext = load_extension()
+        t = self.tuning
+        self.plan = ext.BatchedPlan(torch.from_numpy(table), DTYPE_CODE[dtype], t.threads,
+                                    t.guide_shift, t.chunk_events, t.scan_mode, t.persistent)
+        self.layout = self.plan.layout()
+        self.desc = self.plan.descriptors().to(self.device)
+        self.n_instances = n
+        self.capacity = int(self.layout["capacity"])
+        self.total_nodes = int(self.layout["total_nodes"])
+        self.events = events
+        self.n_nodes = n_nodes
+        self.node_off = node_off
+
+    @classmethod
+    def uniform(cls, B, N, E, sequence=0, **kw):
+        """B curves of N nodes, E events each, as one reference call would draw them."""
+        counter_base = np.arange(B, dtype=np.uint64) * np.uint64(E)
+        streams = np.full(B, 2 * sequence, dtype=np.uint64)
+        return cls(np.full(B, N), np.full(B, E), counter_base, streams, **kw)
+
+    # -- buffers ----------------------------------------------------------
+    def new_state(self):
+        """Device buffers for one forward/backward pair."""
+        d = self.device
+        L = self.layout
+        return dict(
+            tables=torch.empty(int(L["table_bytes"]), dtype=torch.uint8, device=d),
+            scratch=torch.empty(int(L["scratch_bytes"]), dtype=torch.uint8, device=d),
+            events=torch.empty(max(self.capacity, 1), dtype=self.dtype, device=d),
+            kept=torch.empty(self.n_instances, dtype=torch.int64, device=d),
+            total=torch.empty(1, dtype=torch.int64, device=d),
+        )
+
+    def _window(self):
+        if self.window is None:
+            return False, 0.0, 0.0
+        return True, float(self.window[0]), float(self.window[1])
+
+    # -- raw launches (no host synchronisation) ---------------------------
+    def forward_raw(self, x, density, seed, st, norm=None, cdf=None, bins=None, stream=None):
+        use, lo, hi = self._window()
+        self.plan.forward(self.desc, x, density, int(seed) & (2 ** 64 - 1), use, lo, hi,
+                          st["tables"], st["scratch"], st["events"], st["kept"], st["total"],
+                          norm, cdf, bins, stream)
+
+    def pack_raw(self, x, density, st, packed, bins=None, packed_bins=None, stream=None):
+        self.plan.pack(self.desc, x, density, st["tables"], st["scratch"], st["events"],
+                       st["kept"], st["total"], packed, bins, packed_bins, stream)
+
+    def backward_raw(self, x, density, seed, st, grad, grad_density, recompute_tables=False,
+                     stream=None):
+        use, lo, hi = self._window()
+        self.plan.backward(self.desc, x, density, int(seed) & (2 ** 64 - 1), use, lo, hi,
+                           st["tables"], st["scratch"], st["events"], st["kept"], st["total"],
+                           grad, grad_density, bool(recompute_tables), stream)
+
+    def dense_flag(self, st) -> bool:
+        return bool(st["scratch"][:4].view(torch.int32).item())
+
+    # -- convenience -------------------------------------------------------
+    def run_forward(self, x, density, seed, want_bins=False, want_tables=False):
+        """Forward with one host sync; returns (values, kept, state, extras)."""
+        self._check_inputs(x, density)
+        st = self.new_state()
+        extras = {}
+        bins = torch.empty(max(self.capacity, 1), dtype=torch.int32, device=self.device) \
+            if want_bins else None
+        norm = cdf = None
+        if want_tables:
+            norm = torch.empty(self.n_instances, dtype=torch.float64, device=self.device)
+            cdf = torch.empty(max(self.total_nodes, 1), dtype=torch.float64, device=self.device)
+            extras.update(norm=norm, cdf=cdf)
+        self.forward_raw(x, density, seed, st, norm, cdf, bins)
+        total = int(st["total"].item())
+        if self.dense_flag(st):
+            values = st["events"][:total]
+            if bins is not None:
+                extras["bins"] = bins[:total]
+        else:
+            packed = torch.empty(max(total, 1), dtype=self.dtype, device=self.device)
+            pb = torch.empty(max(total, 1), dtype=torch.int32, device=self.device) \
+                if bins is not None else None
+            self.pack_raw(x, density, st, packed, bins, pb)
+            values = packed[:total]
+            if pb is not None:
+                extras["bins"] = pb[:total]
+        return values, st["kept"], st, extras
+
+    def _check_inputs(self, x, density):
+        for name, t in (("x", x), ("density", density)):
+            if not t.is_cuda or t.dtype != self.dtype or not t.is_contiguous():
+                raise ValueError(f"{name} must be a contiguous {self.dtype} GPU tensor")
+            if t.numel() < self.total_nodes:
+                raise ValueError(f"{name} has {t.numel()} nodes; the plan needs {self.total_nodes}")
+
+    def __call__(self, x, density, seed):
+        """Differentiable w.r.t. density: returns the packed kept values."""
+        return _BatchedFunction.apply(density, x, self, int(seed))
+
+
+class _BatchedFunction(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, density, x, sampler, seed):
+        x = x.contiguous()
+        density = density.contiguous()
+        values, kept, st, _ = sampler.run_forward(x, density, seed)
+        ctx.sampler = sampler
+        ctx.seed = seed
+        ctx.st = st
+        ctx.save_for_backward(x, density)
+        return values
+
+    @staticmethod
+    def backward(ctx, grad):
+        x, density = ctx.saved_tensors
+        sampler = ctx.sampler
+        grad = grad.contiguous().to(sampler.dtype)
+        if grad.numel() == 0:
+            grad = torch.zeros(1, dtype=sampler.dtype, device=sampler.device)
+        gd = torch.empty_like(density)
+        sampler.backward_raw(x, density, ctx.seed, ctx.st, grad, gd)
+        return gd, None, None, None

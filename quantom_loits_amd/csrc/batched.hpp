@@ -1,0 +1,50 @@
+// batched.hpp -- parameters and launchers of the batched 1-D kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "loits_device.hpp"
+
+namespace qloits {
+
+struct BatchedParams {
+  const InstanceDesc* inst;
+  const ChunkDesc* chunks;
+  int64_t n_instances, n_chunks;
+  const void* x;
+  const void* density;
+  uint8_t* tables;
+  uint64_t seed;
+  int32_t use_window;
+  int32_t pad;
+  double win_lo, win_hi;
+  void* events;          // capacity layout
+  int32_t* bins;         // optional, capacity layout
+  int64_t* kept;         // [n_instances]
+  int64_t* total;        // [1]
+  double* norm_out;      // optional [n_instances]
+  double* cdf_out;       // optional [total_nodes]
+  // scratch
+  int32_t* flags;        // [0] = dense
+  int64_t* chunk_kept;   // [n_chunks]
+  int64_t* chunk_row;    // [n_chunks]
+  uint32_t* done;        // [n_instances]
+  uint64_t* keepmask;    // per-tile keep bits
+  double* partials;      // per-tile partial VJP sums
+};
+
+template <typename Real>
+hipError_t launch_tables(const BatchedParams& P, int threads, bool sequential, hipStream_t stream);
+template <typename Real>
+hipError_t launch_sample(const BatchedParams& P, int threads, int64_t lds, hipStream_t stream);
+hipError_t launch_finalize(const BatchedParams& P, hipStream_t stream);
+template <typename Real>
+hipError_t launch_pack(const BatchedParams& P, void* packed, int32_t* packed_bins,
+                       hipStream_t stream);
+template <typename Real>
+hipError_t launch_vjp(const BatchedParams& P, int threads, int64_t lds, int maxper,
+                      const void* grad, void* grad_density, hipStream_t stream);
+
+}  // namespace qloits

@@ -1,0 +1,416 @@
+// bindings.cpp -- pybind11 front end over the C ABI (include/loits_hip.h).
+//
+// `forward` / `backward` keep the signatures and return values of the
+// reference bindings (cpp/bindings.cpp:24-33, 132-147; state tuple
+// cpp/bindings.cpp:126-129) with one extra trailing argument, the HIP stream
+// handle (None = torch's current stream).  PyTorch allocates every tensor; the
+// C ABI receives raw device pointers.  BatchedPlan wraps the batched 1-D path.
+#include <torch/extension.h>
+#include <ATen/record_function.h>
+
+#include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "loits_hip.h"
+
+namespace py = pybind11;
+
+namespace {
+
+class RegionGuard {
+ public:
+  RegionGuard(bool enabled, const char* name) {
+    if (!enabled) return;
+    guard_ = std::make_unique<at::RecordFunction>(at::RecordScope::USER_SCOPE);
+    if (guard_->isActive()) guard_->before(name);
+  }
+
+ private:
+  std::unique_ptr<at::RecordFunction> guard_;
+};
+
+void check(int status, const char* what) {
+  if (status != LOITS_HIP_OK)
+    throw std::runtime_error(std::string(what) + ": " + loits_hip_last_error());
+}
+
+void* stream_of(const py::object& stream, const at::Tensor& like) {
+  if (!stream.is_none()) return reinterpret_cast<void*>(stream.cast<uintptr_t>());
+  py::object cuda = py::module_::import("torch").attr("cuda");
+  py::object dev = py::cast(like.device().index());
+  return reinterpret_cast<void*>(cuda.attr("current_stream")(dev).attr("cuda_stream").cast<uintptr_t>());
+}
+
+void require_device(const at::Tensor& t, const char* name) {
+  if (!t.is_cuda()) throw std::invalid_argument(std::string(name) + " must be a GPU tensor");
+  if (!t.is_contiguous()) throw std::invalid_argument(std::string(name) + " must be contiguous");
+}
+
+int dtype_code(const at::Tensor& t, const char* name) {
+  if (t.scalar_type() == at::kDouble) return LOITS_HIP_F64;
+  if (t.scalar_type() == at::kFloat) return LOITS_HIP_F32;
+  throw std::invalid_argument(std::string(name) + " must be float64 or float32");
+}
+
+loits_hip_shape shape_of(const at::Tensor& x_bins, const at::Tensor& q_bins) {
+  if (x_bins.dim() != 3 || q_bins.dim() != 3)
+    throw std::invalid_argument("x_bins and q_bins must be [B, n, k]");
+  return loits_hip_shape{x_bins.size(0), x_bins.size(1), q_bins.size(1),
+                         x_bins.size(2), q_bins.size(2), x_bins.size(1) * q_bins.size(1)};
+}
+
+// cpp/bindings.cpp:24-130
+std::vector<at::Tensor> forward(at::Tensor x_bins, at::Tensor xsec_x, at::Tensor q_bins,
+                                at::Tensor xsec_q, at::Tensor weights, at::Tensor acceptance,
+                                int64_t n_events, uint64_t seed, uint64_t sequence,
+                                bool profile_regions, py::object stream_obj) {
+  RegionGuard total(profile_regions, "loits::forward");
+  for (auto* p : {&x_bins, &xsec_x, &q_bins, &xsec_q, &weights, &acceptance})
+    require_device(*p, "theory output");
+  const int dt = dtype_code(xsec_x, "xsec_x");
+  for (auto* p : {&x_bins, &q_bins, &xsec_q, &weights})
+    if (p->scalar_type() != xsec_x.scalar_type())
+      throw std::invalid_argument("all floating theory outputs must share xsec_x's dtype");
+  if (acceptance.scalar_type() != at::kBool)
+    throw std::invalid_argument("acceptance must be a bool tensor");
+  const loits_hip_shape s = shape_of(x_bins, q_bins);
+  if (xsec_x.numel() != s.batch * s.ny * s.nx * s.kx || xsec_q.numel() != s.batch * s.nx * s.ny * s.kq ||
+      weights.numel() != s.batch * s.cells || acceptance.numel() != s.batch * s.cells)
+    throw std::invalid_argument("theory output shapes do not match x_bins / q_bins");
+  void* stream = stream_of(stream_obj, xsec_x);
+  auto i64 = xsec_x.options().dtype(at::kLong);
+  auto f64 = xsec_x.options().dtype(at::kDouble);
+
+  at::Tensor counts = at::empty({s.batch, s.cells}, i64);
+  int64_t nmax = 0, active = 0;
+  {
+    RegionGuard region(profile_regions, "loits::forward::allocation");
+    at::Tensor na = at::empty({2}, i64);
+    check(loits_hip_allocate(weights.data_ptr(), dt, s.batch * s.cells, n_events,
+                             counts.data_ptr<int64_t>(), na.data_ptr<int64_t>(), stream),
+          "loits_hip_allocate");
+    at::Tensor host = na.cpu();  // the reference reads nmax/active on the host here too
+    nmax = host[0].item<int64_t>();
+    active = host[1].item<int64_t>();
+  }
+  const int64_t total_slots = s.batch * s.cells * nmax;
+  at::Tensor norm_x = at::empty({s.batch, s.ny, s.nx}, f64);
+  at::Tensor norm_q = at::empty({s.batch, s.nx, s.ny}, f64);
+  at::Tensor cdf_x = at::empty(xsec_x.sizes(), f64);
+  at::Tensor cdf_q = at::empty(xsec_q.sizes(), f64);
+  at::Tensor u_x = at::empty({s.batch, s.cells, nmax}, xsec_x.options().dtype(at::kFloat));
+  at::Tensor u_q = at::empty({s.batch, s.cells, nmax}, xsec_x.options().dtype(at::kFloat));
+  at::Tensor interval_x = at::empty({total_slots}, xsec_x.options().dtype(at::kShort));
+  at::Tensor interval_q = at::empty({total_slots}, xsec_x.options().dtype(at::kShort));
+  at::Tensor events_storage = at::empty({active, 2}, xsec_x.options());
+  at::Tensor packed_storage = at::empty({active}, i64);
+  at::Tensor row_offsets = at::empty({s.batch * s.cells + 1}, i64);
+  const int64_t ws_bytes = loits_hip_forward_workspace_size(&s, nmax);
+  if (ws_bytes < 0) throw std::invalid_argument(loits_hip_last_error());
+  at::Tensor workspace = at::empty({ws_bytes}, xsec_x.options().dtype(at::kByte));
+  at::Tensor valid = at::empty({2}, i64);
+  at::Tensor acc_u8 = acceptance.view(at::kByte);
+  {
+    // The HIP forward fuses the reference's stages into three kernels:
+    // curves (rho + CDF, both axes), sampling (both Philox streams,
+    // interpolation, compaction) and row offsets.  The reference's stage
+    // names are kept; stages fused into one kernel are nested markers.
+    RegionGuard r1(profile_regions, "loits::forward::rho_x");
+    RegionGuard r2(profile_regions, "loits::forward::rho_q2");
+    RegionGuard r3(profile_regions, "loits::forward::cdf_x");
+    RegionGuard r4(profile_regions, "loits::forward::cdf_q2");
+  }
+  {
+    RegionGuard region(profile_regions, "loits::forward::stream_compaction");
+    {
+      RegionGuard a(profile_regions, "loits::forward::random_x");
+      RegionGuard b(profile_regions, "loits::forward::random_q2");
+      RegionGuard c(profile_regions, "loits::forward::interpolation_x");
+      RegionGuard d(profile_regions, "loits::forward::interpolation_q2");
+    }
+    loits_hip_forward_args a{};
+    a.shape = s;
+    a.dtype = dt;
+    a.nmax = nmax;
+    a.active = active;
+    a.seed = seed;
+    a.sequence = sequence;
+    a.x_bins = x_bins.data_ptr();
+    a.xsec_x = xsec_x.data_ptr();
+    a.q_bins = q_bins.data_ptr();
+    a.xsec_q = xsec_q.data_ptr();
+    a.acceptance = acc_u8.data_ptr<uint8_t>();
+    a.counts = counts.data_ptr<int64_t>();
+    a.norm_x = norm_x.data_ptr<double>();
+    a.norm_q = norm_q.data_ptr<double>();
+    a.cdf_x = cdf_x.data_ptr<double>();
+    a.cdf_q = cdf_q.data_ptr<double>();
+    a.u_x = u_x.data_ptr<float>();
+    a.u_q = u_q.data_ptr<float>();
+    a.interval_x = interval_x.data_ptr<int16_t>();
+    a.interval_q = interval_q.data_ptr<int16_t>();
+    a.events = events_storage.data_ptr();
+    a.packed = packed_storage.data_ptr<int64_t>();
+    a.row_offsets = row_offsets.data_ptr<int64_t>();
+    a.workspace = workspace.data_ptr();
+    a.valid = valid.data_ptr<int64_t>();
+    check(loits_hip_forward(&a, stream), "loits_hip_forward");
+  }
+  at::Tensor host_valid = valid.cpu();
+  if (host_valid[1].item<int64_t>() != 0)
+    throw std::runtime_error("loits_hip_forward: compaction look-back did not complete");
+  const int64_t nvalid = host_valid[0].item<int64_t>();
+  RegionGuard region(profile_regions, "loits::forward::state_pack");
+  return {events_storage.narrow(0, 0, nvalid), norm_x, norm_q, cdf_x, cdf_q, u_x, u_q,
+          interval_x, interval_q, packed_storage.narrow(0, 0, nvalid), row_offsets};
+}
+
+// cpp/bindings.cpp:132-208 (row_offsets accepted for the openmp-style state)
+std::vector<at::Tensor> backward(at::Tensor grad_events, at::Tensor x_bins, at::Tensor xsec_x,
+                                 at::Tensor q_bins, at::Tensor xsec_q, at::Tensor acceptance,
+                                 at::Tensor norm_x, at::Tensor norm_q, at::Tensor cdf_x,
+                                 at::Tensor cdf_q, at::Tensor u_x, at::Tensor u_q,
+                                 at::Tensor interval_x, at::Tensor interval_q, at::Tensor packed,
+                                 py::args rest) {
+  // rest = ([row_offsets,] profile_regions [, stream])
+  std::vector<py::object> tail;
+  for (auto item : rest) tail.push_back(py::reinterpret_borrow<py::object>(item));
+  if (!tail.empty() && py::hasattr(tail.front(), "dtype")) tail.erase(tail.begin());  // row_offsets
+  const bool profile_regions = !tail.empty() && tail[0].cast<bool>();
+  py::object stream_obj = tail.size() > 1 ? tail[1] : py::none();
+  RegionGuard total(profile_regions, "loits::backward");
+  grad_events = grad_events.contiguous();
+  require_device(grad_events, "grad_events");
+  const int dt = dtype_code(xsec_x, "xsec_x");
+  if (grad_events.scalar_type() != xsec_x.scalar_type()) grad_events = grad_events.to(xsec_x.scalar_type());
+  const loits_hip_shape s = shape_of(x_bins, q_bins);
+  void* stream = stream_of(stream_obj, xsec_x);
+  at::Tensor grad_x = at::empty(xsec_x.sizes(), xsec_x.options());
+  at::Tensor grad_q = at::empty(xsec_q.sizes(), xsec_q.options());
+  const int64_t ws_bytes = loits_hip_backward_workspace_size(&s, u_x.size(2));
+  if (ws_bytes < 0) throw std::invalid_argument(loits_hip_last_error());
+  at::Tensor workspace = at::empty({ws_bytes}, xsec_x.options().dtype(at::kByte));
+  at::Tensor acc_u8 = acceptance.contiguous().view(at::kByte);
+  {
+    RegionGuard a(profile_regions, "loits::backward::interpolation_x");
+    RegionGuard b(profile_regions, "loits::backward::interpolation_q2");
+  }
+  {
+    RegionGuard r1(profile_regions, "loits::backward::cdf_x");
+    RegionGuard r2(profile_regions, "loits::backward::cdf_q2");
+    RegionGuard r3(profile_regions, "loits::backward::rho_x");
+    RegionGuard r4(profile_regions, "loits::backward::rho_q2");
+    loits_hip_backward_args a{};
+    a.shape = s;
+    a.dtype = dt;
+    a.nmax = u_x.size(2);
+    a.nvalid = packed.size(0);
+    a.grad_events = grad_events.data_ptr();
+    a.x_bins = x_bins.data_ptr();
+    a.xsec_x = xsec_x.data_ptr();
+    a.q_bins = q_bins.data_ptr();
+    a.xsec_q = xsec_q.data_ptr();
+    a.acceptance = acc_u8.data_ptr<uint8_t>();
+    a.norm_x = norm_x.data_ptr<double>();
+    a.norm_q = norm_q.data_ptr<double>();
+    a.cdf_x = cdf_x.data_ptr<double>();
+    a.cdf_q = cdf_q.data_ptr<double>();
+    a.u_x = u_x.data_ptr<float>();
+    a.u_q = u_q.data_ptr<float>();
+    a.interval_x = interval_x.data_ptr<int16_t>();
+    a.interval_q = interval_q.data_ptr<int16_t>();
+    a.packed = packed.data_ptr<int64_t>();
+    a.grad_xsec_x = grad_x.data_ptr();
+    a.grad_xsec_q = grad_q.data_ptr();
+    a.workspace = workspace.data_ptr();
+    check(loits_hip_backward(&a, stream), "loits_hip_backward");
+  }
+  RegionGuard region(profile_regions, "loits::backward::state_pack");
+  return {grad_x, grad_q};
+}
+
+at::Tensor philox_fill(int64_t count, uint64_t seed, uint64_t stream_id, int64_t device,
+                       py::object stream_obj) {
+  at::Tensor out = at::empty({count}, at::TensorOptions().dtype(at::kFloat).device(at::kCUDA, device));
+  check(loits_hip_philox_fill(out.data_ptr<float>(), count, seed, stream_id,
+                              stream_of(stream_obj, out)),
+        "loits_hip_philox_fill");
+  return out;
+}
+
+at::Tensor philox_words(int64_t first_block, int64_t blocks, uint64_t stream_id, uint64_t seed,
+                        int64_t device, py::object stream_obj) {
+  at::Tensor out = at::empty({blocks * 4}, at::TensorOptions().dtype(at::kInt).device(at::kCUDA, device));
+  check(loits_hip_philox_words(reinterpret_cast<uint32_t*>(out.data_ptr<int32_t>()), first_block,
+                               blocks, stream_id, seed, stream_of(stream_obj, out)),
+        "loits_hip_philox_words");
+  return out;
+}
+
+// ---------------------------------------------------------------------------
+// Batched 1-D path.
+class BatchedPlan {
+ public:
+  // instances: CPU int64 [n, 8] = node_off, n_nodes, events, out_off,
+  // counter_base, stream, accept, 0 (counter/stream as two's-complement int64)
+  BatchedPlan(at::Tensor instances, int64_t dtype, int64_t threads, int64_t guide_shift,
+              int64_t chunk_events, int64_t scan_mode, int64_t persistent) {
+    instances = instances.contiguous();
+    if (instances.device().is_cuda() || instances.scalar_type() != at::kLong ||
+        instances.dim() != 2 || instances.size(1) != 8)
+      throw std::invalid_argument("instances must be a CPU int64 tensor [n, 8]");
+    const int64_t n = instances.size(0);
+    inst_.resize(size_t(n));
+    const int64_t* src = instances.data_ptr<int64_t>();
+    for (int64_t i = 0; i < n; ++i) {
+      loits_hip_instance& d = inst_[size_t(i)];
+      d.node_off = src[8 * i + 0];
+      d.n_nodes = src[8 * i + 1];
+      d.events = src[8 * i + 2];
+      d.out_off = src[8 * i + 3];
+      d.counter_base = uint64_t(src[8 * i + 4]);
+      d.stream = uint64_t(src[8 * i + 5]);
+      d.accept = int32_t(src[8 * i + 6]);
+      d.reserved = 0;
+    }
+    tuning_.threads = int32_t(threads);
+    tuning_.guide_shift = int32_t(guide_shift);
+    tuning_.chunk_events = chunk_events;
+    tuning_.scan_mode = int32_t(scan_mode);
+    tuning_.persistent = int32_t(persistent);
+    check(loits_hip_batched_layout(inst_.data(), n, int(dtype), &tuning_, &layout_),
+          "loits_hip_batched_layout");
+    desc_host_ = at::empty({layout_.desc_bytes}, at::TensorOptions().dtype(at::kByte));
+    check(loits_hip_batched_encode(inst_.data(), n, &tuning_, &layout_, desc_host_.data_ptr()),
+          "loits_hip_batched_encode");
+  }
+
+  py::dict layout() const {
+    py::dict d;
+    d["n_instances"] = layout_.n_instances;
+    d["n_chunks"] = layout_.n_chunks;
+    d["capacity"] = layout_.capacity;
+    d["total_nodes"] = layout_.total_nodes;
+    d["max_nodes"] = layout_.max_nodes;
+    d["desc_bytes"] = layout_.desc_bytes;
+    d["table_bytes"] = layout_.table_bytes;
+    d["scratch_bytes"] = layout_.scratch_bytes;
+    d["lds_forward"] = layout_.lds_forward;
+    d["lds_backward"] = layout_.lds_backward;
+    d["mask_words"] = layout_.mask_words;
+    d["partial_doubles"] = layout_.partial_doubles;
+    d["dtype"] = layout_.dtype;
+    d["threads"] = layout_.threads;
+    d["scan_mode"] = layout_.scan_mode;
+    return d;
+  }
+
+  at::Tensor descriptors() const { return desc_host_; }
+
+  // All tensors are device tensors the caller allocated with the layout's sizes.
+  void forward(at::Tensor desc, at::Tensor x, at::Tensor density, uint64_t seed, bool use_window,
+               double lo, double hi, at::Tensor tables, at::Tensor scratch, at::Tensor events,
+               at::Tensor kept, at::Tensor total, py::object norm, py::object cdf, py::object bins,
+               py::object stream_obj) {
+    loits_hip_batched_args a = args(desc, x, density, seed, use_window, lo, hi, tables, scratch,
+                                    events, kept, total, norm, cdf, bins);
+    check(loits_hip_batched_forward(&a, stream_of(stream_obj, x)), "loits_hip_batched_forward");
+  }
+
+  void pack(at::Tensor desc, at::Tensor x, at::Tensor density, at::Tensor tables,
+            at::Tensor scratch, at::Tensor events, at::Tensor kept, at::Tensor total,
+            at::Tensor packed, py::object bins, py::object packed_bins, py::object stream_obj) {
+    loits_hip_batched_args a = args(desc, x, density, 0, false, 0.0, 0.0, tables, scratch, events,
+                                    kept, total, py::none(), py::none(), bins);
+    int32_t* pb = packed_bins.is_none() ? nullptr : packed_bins.cast<at::Tensor>().data_ptr<int32_t>();
+    check(loits_hip_batched_pack(&a, packed.data_ptr(), pb, stream_of(stream_obj, x)),
+          "loits_hip_batched_pack");
+  }
+
+  void backward(at::Tensor desc, at::Tensor x, at::Tensor density, uint64_t seed, bool use_window,
+                double lo, double hi, at::Tensor tables, at::Tensor scratch, at::Tensor events,
+                at::Tensor kept, at::Tensor total, at::Tensor grad, at::Tensor grad_density,
+                bool recompute_tables, py::object stream_obj) {
+    loits_hip_batched_args a = args(desc, x, density, seed, use_window, lo, hi, tables, scratch,
+                                    events, kept, total, py::none(), py::none(), py::none());
+    check(loits_hip_batched_backward(&a, grad.data_ptr(), grad_density.data_ptr(),
+                                     recompute_tables ? 1 : 0, stream_of(stream_obj, x)),
+          "loits_hip_batched_backward");
+  }
+
+ private:
+  loits_hip_batched_args args(at::Tensor& desc, at::Tensor& x, at::Tensor& density, uint64_t seed,
+                              bool use_window, double lo, double hi, at::Tensor& tables,
+                              at::Tensor& scratch, at::Tensor& events, at::Tensor& kept,
+                              at::Tensor& total, const py::object& norm, const py::object& cdf,
+                              const py::object& bins) {
+    for (auto* t : {&desc, &x, &density, &tables, &scratch, &events, &kept, &total})
+      require_device(*t, "batched tensor");
+    const at::ScalarType want = layout_.dtype == LOITS_HIP_F32 ? at::kFloat : at::kDouble;
+    if (x.scalar_type() != want || density.scalar_type() != want || events.scalar_type() != want)
+      throw std::invalid_argument("x, density and events must have the plan's dtype");
+    if (x.numel() < layout_.total_nodes || density.numel() < layout_.total_nodes ||
+        events.numel() < layout_.capacity || tables.numel() < layout_.table_bytes ||
+        scratch.numel() < layout_.scratch_bytes || desc.numel() < layout_.desc_bytes ||
+        kept.numel() < layout_.n_instances || total.numel() < 1)
+      throw std::invalid_argument("a batched tensor is smaller than the plan requires");
+    loits_hip_batched_args a{};
+    a.layout = &layout_;
+    a.desc = desc.data_ptr();
+    a.x = x.data_ptr();
+    a.density = density.data_ptr();
+    a.seed = seed;
+    a.use_window = use_window ? 1 : 0;
+    a.window_lo = lo;
+    a.window_hi = hi;
+    a.tables = tables.data_ptr();
+    a.scratch = scratch.data_ptr();
+    a.events = events.data_ptr();
+    a.kept = kept.data_ptr<int64_t>();
+    a.total = total.data_ptr<int64_t>();
+    a.norm = norm.is_none() ? nullptr : norm.cast<at::Tensor>().data_ptr<double>();
+    a.cdf = cdf.is_none() ? nullptr : cdf.cast<at::Tensor>().data_ptr<double>();
+    a.bins = bins.is_none() ? nullptr : bins.cast<at::Tensor>().data_ptr<int32_t>();
+    return a;
+  }
+
+  std::vector<loits_hip_instance> inst_;
+  loits_hip_tuning tuning_{};
+  loits_hip_layout layout_{};
+  at::Tensor desc_host_;
+};
+
+py::tuple device_info(int device) {
+  char arch[64] = {0};
+  int cus = 0;
+  check(loits_hip_device_info(device, arch, sizeof(arch), &cus), "loits_hip_device_info");
+  return py::make_tuple(std::string(arch), cus);
+}
+
+}  // namespace
+
+PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
+  m.doc() = "MI355X LOITS sampler: pybind front end over the loits_hip C ABI";
+  m.def("forward", &forward, py::arg("x_bins"), py::arg("xsec_x"), py::arg("q_bins"),
+        py::arg("xsec_q"), py::arg("weights"), py::arg("acceptance"), py::arg("n_events"),
+        py::arg("seed"), py::arg("sequence"), py::arg("profile_regions") = false,
+        py::arg("stream") = py::none());
+  m.def("backward", &backward);
+  m.def("philox_fill", &philox_fill, py::arg("count"), py::arg("seed"), py::arg("stream_id"),
+        py::arg("device") = 0, py::arg("stream") = py::none());
+  m.def("philox_words", &philox_words, py::arg("first_block"), py::arg("blocks"),
+        py::arg("stream_id"), py::arg("seed"), py::arg("device") = 0,
+        py::arg("stream") = py::none());
+  m.def("device_info", &device_info, py::arg("device") = 0);
+  m.def("version", []() { return std::string(loits_hip_version()); });
+  py::class_<BatchedPlan>(m, "BatchedPlan")
+      .def(py::init<at::Tensor, int64_t, int64_t, int64_t, int64_t, int64_t, int64_t>())
+      .def("layout", &BatchedPlan::layout)
+      .def("descriptors", &BatchedPlan::descriptors)
+      .def("forward", &BatchedPlan::forward)
+      .def("pack", &BatchedPlan::pack)
+      .def("backward", &BatchedPlan::backward);
+}

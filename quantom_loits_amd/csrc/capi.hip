@@ -1,0 +1,392 @@
+// capi.hip -- the extern "C" boundary (include/loits_hip.h): argument
+// checks, layouts, thread-local error reporting and kernel launches.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "batched.hpp"
+#include "capi_internal.hpp"
+#include "loits_device.hpp"
+#include "loits_hip.h"
+
+namespace qloits {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+int fail(int status, const std::string& msg) {
+  g_last_error = msg;
+  return status;
+}
+
+int check_hip(hipError_t e, const char* what) {
+  if (e == hipSuccess) return LOITS_HIP_OK;
+  return fail(LOITS_HIP_ERUNTIME, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// ---------------------------------------------------------------------------
+// Philox probe kernels (rng/philox.hpp:17-61, cpp/loits_core.cpp:39-53).
+__global__ void philox_fill_kernel(float* values, int64_t count, uint64_t seed, uint64_t sid) {
+  const int64_t block = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t base = block * 4;
+  if (base >= count) return;
+  const Words4 w = philox4x32_10(uint64_t(block), sid, seed);
+  if (base + 3 < count) {
+    const float4 v = make_float4(uniform_float(w.w0), uniform_float(w.w1), uniform_float(w.w2),
+                                 uniform_float(w.w3));
+    if ((reinterpret_cast<uintptr_t>(values) & 15) == 0) {
+      reinterpret_cast<float4*>(values)[block] = v;
+      return;
+    }
+  }
+#pragma unroll
+  for (int e = 0; e < 4; ++e)
+    if (base + e < count) values[base + e] = uniform_float(word_of(w, e));
+}
+
+__global__ void philox_words_kernel(uint32_t* words, int64_t first, int64_t blocks,
+                                    uint64_t sid, uint64_t seed) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= blocks) return;
+  const Words4 w = philox4x32_10(uint64_t(first + i), sid, seed);
+  reinterpret_cast<uint4*>(words)[i] = make_uint4(w.w0, w.w1, w.w2, w.w3);
+}
+
+// ---------------------------------------------------------------------------
+// Batched layout.
+namespace {
+
+int ceil_log2(int64_t v) {
+  int l = 0;
+  while ((int64_t(1) << l) < v) ++l;
+  return l;
+}
+
+int64_t align16(int64_t v) { return (v + 15) & ~int64_t(15); }
+
+struct Plan {
+  std::vector<InstanceDesc> inst;
+  std::vector<ChunkDesc> chunks;
+  loits_hip_layout layout;
+};
+
+int make_plan(const loits_hip_instance* instances, int64_t n, int dtype,
+              const loits_hip_tuning* tuning, Plan* plan) {
+  if (!instances || n <= 0) return fail(LOITS_HIP_EINVAL, "batched: need at least one instance");
+  if (dtype != LOITS_HIP_F32 && dtype != LOITS_HIP_F64)
+    return fail(LOITS_HIP_EINVAL, "batched: dtype must be LOITS_HIP_F32 or LOITS_HIP_F64");
+  if (!tuning) return fail(LOITS_HIP_EINVAL, "batched: tuning is NULL");
+  const int threads = tuning->threads;
+  if (threads != 256 && threads != 512 && threads != 1024)
+    return fail(LOITS_HIP_EINVAL, "batched: threads must be 256, 512 or 1024");
+  if (tuning->guide_shift < 0 || tuning->guide_shift > 3)
+    return fail(LOITS_HIP_EINVAL, "batched: guide_shift must be in [0, 3]");
+  const int64_t per_iter = int64_t(4) * threads;
+  int64_t chunk_events = tuning->chunk_events;
+  if (chunk_events <= 0) chunk_events = int64_t(1) << 62;
+  chunk_events = (chunk_events + per_iter - 1) / per_iter * per_iter;
+  const int64_t coef_size = dtype == LOITS_HIP_F32 ? 16 : 32;
+  const int64_t real_size = dtype == LOITS_HIP_F32 ? 4 : 8;
+  const int nwaves = threads / 64;
+
+  loits_hip_layout& L = plan->layout;
+  std::memset(&L, 0, sizeof(L));
+  L.n_instances = n;
+  L.dtype = dtype;
+  L.threads = threads;
+  L.scan_mode = tuning->scan_mode;
+  L.persistent = tuning->persistent;
+  int64_t out = 0, table_off = 0, mask_words = 0, partial_doubles = 0;
+  int64_t lds_f = 0, lds_b = 0;
+  plan->inst.resize(size_t(n));
+  plan->chunks.clear();
+  for (int64_t i = 0; i < n; ++i) {
+    const loits_hip_instance& s = instances[i];
+    if (s.n_nodes < 2 || s.n_nodes > (int64_t(1) << 20))
+      return fail(LOITS_HIP_EINVAL, "batched: instance " + std::to_string(i) +
+                                        " needs 2 <= n_nodes <= 2^20");
+    if (s.events < 0 || s.node_off < 0)
+      return fail(LOITS_HIP_EINVAL, "batched: instance " + std::to_string(i) +
+                                        " has a negative offset or event count");
+    if (s.out_off != out)
+      return fail(LOITS_HIP_EINVAL, "batched: instance " + std::to_string(i) +
+                                        " out_off must equal the events of the instances before it");
+    if (s.counter_base > ~uint64_t(0) - uint64_t(s.events))
+      return fail(LOITS_HIP_EINVAL, "batched: Philox counter range overflows 64 bits");
+    InstanceDesc& d = plan->inst[size_t(i)];
+    d.node_off = s.node_off;
+    d.n_nodes = s.n_nodes;
+    d.events = s.events;
+    d.out_off = s.out_off;
+    d.counter_base = s.counter_base;
+    d.stream = s.stream;
+    d.accept = s.accept ? 1 : 0;
+    d.guide_log2 = std::min(20, std::max(4, ceil_log2(s.n_nodes) + tuning->guide_shift));
+    d.table_off = table_off;
+    const TableLayout tl = table_layout(s.n_nodes, d.guide_log2, coef_size);
+    table_off += align16(tl.bytes);
+    lds_f = std::max(lds_f, tl.cdf - tl.thr);
+    lds_b = std::max(lds_b, (tl.coef - tl.thr) + pad4(s.n_nodes) * (real_size + 16));
+    L.max_nodes = std::max(L.max_nodes, s.n_nodes);
+    L.total_nodes = std::max(L.total_nodes, s.node_off + s.n_nodes);
+    const int64_t nch = std::max<int64_t>(1, (s.events + chunk_events - 1) / chunk_events);
+    if (plan->chunks.size() + size_t(nch) > size_t(INT32_MAX))
+      return fail(LOITS_HIP_EUNSUPPORTED, "batched: too many tiles");
+    d.first_chunk = int32_t(plan->chunks.size());
+    d.n_chunks = int32_t(nch);
+    for (int64_t c = 0; c < nch; ++c) {
+      ChunkDesc ch{};
+      ch.slot_lo = c * chunk_events;
+      ch.slot_hi = std::min(s.events, (c + 1) * chunk_events);
+      ch.instance = int32_t(i);
+      ch.index = int32_t(c);
+      const uint64_t p_lo = s.counter_base + uint64_t(ch.slot_lo);
+      const uint64_t p_hi = s.counter_base + uint64_t(ch.slot_hi);
+      const int64_t blocks = ch.slot_hi > ch.slot_lo ? int64_t(((p_hi + 3) >> 2) - (p_lo >> 2)) : 0;
+      const int64_t iters = (blocks + threads - 1) / threads;
+      ch.mask_off = mask_words;
+      mask_words += iters * nwaves * 4;
+      ch.partial_off = nch > 1 ? partial_doubles : 0;
+      if (nch > 1) partial_doubles += 2 * pad4(s.n_nodes);
+      plan->chunks.push_back(ch);
+    }
+    out += s.events;
+  }
+  L.capacity = out;
+  L.n_chunks = int64_t(plan->chunks.size());
+  L.desc_bytes = align16(n * int64_t(sizeof(InstanceDesc))) +
+                 L.n_chunks * int64_t(sizeof(ChunkDesc));
+  L.table_bytes = std::max<int64_t>(table_off, 16);
+  L.lds_forward = lds_f;
+  L.lds_backward = lds_b;
+  if (lds_f > kMaxLds || lds_b > kMaxLds)
+    return fail(LOITS_HIP_EUNSUPPORTED,
+                "batched: curve tables of " + std::to_string(L.max_nodes) +
+                    " nodes exceed the LDS budget (" + std::to_string(std::max(lds_f, lds_b)) +
+                    " > " + std::to_string(kMaxLds) + " bytes)");
+  if ((L.max_nodes + threads - 1) / threads > 32)
+    return fail(LOITS_HIP_EUNSUPPORTED, "batched: n_nodes / threads must be <= 32");
+  L.mask_words = mask_words;
+  L.partial_doubles = partial_doubles;
+  // scratch: flags | chunk_kept | chunk_row | done | keepmask | partials
+  L.scratch_bytes = 64 + align16(8 * L.n_chunks) * 2 + align16(4 * n) + align16(8 * mask_words) +
+                    align16(8 * partial_doubles);
+  L.grid_forward = int32_t(L.n_chunks);
+  L.grid_backward = int32_t(L.n_chunks);
+  return LOITS_HIP_OK;
+}
+
+BatchedParams params_of(const loits_hip_batched_args* a) {
+  const loits_hip_layout& L = *a->layout;
+  BatchedParams P{};
+  const uint8_t* desc = static_cast<const uint8_t*>(a->desc);
+  P.inst = reinterpret_cast<const InstanceDesc*>(desc);
+  P.chunks = reinterpret_cast<const ChunkDesc*>(
+      desc + align16(L.n_instances * int64_t(sizeof(InstanceDesc))));
+  P.n_instances = L.n_instances;
+  P.n_chunks = L.n_chunks;
+  P.x = a->x;
+  P.density = a->density;
+  P.tables = static_cast<uint8_t*>(a->tables);
+  P.seed = a->seed;
+  P.use_window = a->use_window;
+  P.win_lo = a->window_lo;
+  P.win_hi = a->window_hi;
+  P.events = a->events;
+  P.bins = a->bins;
+  P.kept = a->kept;
+  P.total = a->total;
+  P.norm_out = a->norm;
+  P.cdf_out = a->cdf;
+  uint8_t* s = static_cast<uint8_t*>(a->scratch);
+  P.flags = reinterpret_cast<int32_t*>(s);
+  s += 64;
+  P.chunk_kept = reinterpret_cast<int64_t*>(s);
+  s += align16(8 * L.n_chunks);
+  P.chunk_row = reinterpret_cast<int64_t*>(s);
+  s += align16(8 * L.n_chunks);
+  P.done = reinterpret_cast<uint32_t*>(s);
+  s += align16(4 * L.n_instances);
+  P.keepmask = reinterpret_cast<uint64_t*>(s);
+  s += align16(8 * L.mask_words);
+  P.partials = reinterpret_cast<double*>(s);
+  return P;
+}
+
+int check_args(const loits_hip_batched_args* a) {
+  if (!a || !a->layout) return fail(LOITS_HIP_EINVAL, "batched: args or layout is NULL");
+  if (!a->desc || !a->x || !a->density || !a->tables || !a->scratch || !a->events || !a->kept ||
+      !a->total)
+    return fail(LOITS_HIP_EINVAL, "batched: a required pointer is NULL");
+  if (a->use_window && !(a->window_lo <= a->window_hi))
+    return fail(LOITS_HIP_EINVAL, "batched: window_lo must be <= window_hi");
+  return LOITS_HIP_OK;
+}
+
+}  // namespace
+
+}  // namespace qloits
+
+using namespace qloits;
+
+extern "C" {
+
+const char* loits_hip_last_error(void) { return g_last_error.c_str(); }
+
+const char* loits_hip_version(void) { return "quantom-loits-hip 0.1 gfx950"; }
+
+int loits_hip_device_info(int device, char* arch, int arch_len, int* cu_count) {
+  hipDeviceProp_t prop;
+  const hipError_t e = hipGetDeviceProperties(&prop, device);
+  if (e != hipSuccess) return check_hip(e, "hipGetDeviceProperties");
+  if (arch && arch_len > 0) {
+    std::strncpy(arch, prop.gcnArchName, size_t(arch_len - 1));
+    arch[arch_len - 1] = '\0';
+  }
+  if (cu_count) *cu_count = prop.multiProcessorCount;
+  return LOITS_HIP_OK;
+}
+
+int loits_hip_philox_fill(float* values, int64_t count, uint64_t seed, uint64_t stream_id,
+                          void* stream) {
+  if (count < 0 || (count > 0 && !values))
+    return fail(LOITS_HIP_EINVAL, "philox_fill: bad count or NULL output");
+  if (count == 0) return LOITS_HIP_OK;
+  const int64_t blocks = (count + 3) / 4;
+  const int64_t grid = (blocks + 255) / 256;
+  if (grid > INT32_MAX) return fail(LOITS_HIP_EUNSUPPORTED, "philox_fill: count too large");
+  hipLaunchKernelGGL(philox_fill_kernel, dim3(unsigned(grid)), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), values, count, seed, stream_id);
+  return check_hip(hipGetLastError(), "philox_fill launch");
+}
+
+int loits_hip_philox_words(uint32_t* words, int64_t first_block, int64_t blocks,
+                           uint64_t stream_id, uint64_t seed, void* stream) {
+  if (blocks < 0 || (blocks > 0 && !words) || first_block < 0)
+    return fail(LOITS_HIP_EINVAL, "philox_words: bad arguments");
+  if (blocks == 0) return LOITS_HIP_OK;
+  if ((reinterpret_cast<uintptr_t>(words) & 15) != 0)
+    return fail(LOITS_HIP_EINVAL, "philox_words: output must be 16-byte aligned");
+  const int64_t grid = (blocks + 255) / 256;
+  if (grid > INT32_MAX) return fail(LOITS_HIP_EUNSUPPORTED, "philox_words: too many blocks");
+  hipLaunchKernelGGL(philox_words_kernel, dim3(unsigned(grid)), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), words, first_block, blocks, stream_id,
+                     seed);
+  return check_hip(hipGetLastError(), "philox_words launch");
+}
+
+int loits_hip_batched_layout(const loits_hip_instance* instances, int64_t n, int dtype,
+                             const loits_hip_tuning* tuning, loits_hip_layout* layout) {
+  try {
+    if (!layout) return fail(LOITS_HIP_EINVAL, "batched_layout: layout is NULL");
+    Plan plan;
+    const int st = make_plan(instances, n, dtype, tuning, &plan);
+    if (st != LOITS_HIP_OK) return st;
+    *layout = plan.layout;
+    return LOITS_HIP_OK;
+  } catch (const std::exception& e) {
+    return fail(LOITS_HIP_ERUNTIME, std::string("batched_layout: ") + e.what());
+  }
+}
+
+int loits_hip_batched_encode(const loits_hip_instance* instances, int64_t n,
+                             const loits_hip_tuning* tuning, const loits_hip_layout* layout,
+                             void* host_desc) {
+  try {
+    if (!layout || !host_desc) return fail(LOITS_HIP_EINVAL, "batched_encode: NULL argument");
+    Plan plan;
+    const int st = make_plan(instances, n, layout->dtype, tuning, &plan);
+    if (st != LOITS_HIP_OK) return st;
+    if (std::memcmp(&plan.layout, layout, sizeof(loits_hip_layout)) != 0)
+      return fail(LOITS_HIP_EINVAL, "batched_encode: layout does not match instances/tuning");
+    uint8_t* out = static_cast<uint8_t*>(host_desc);
+    std::memset(out, 0, size_t(layout->desc_bytes));
+    std::memcpy(out, plan.inst.data(), plan.inst.size() * sizeof(InstanceDesc));
+    std::memcpy(out + align16(n * int64_t(sizeof(InstanceDesc))), plan.chunks.data(),
+                plan.chunks.size() * sizeof(ChunkDesc));
+    return LOITS_HIP_OK;
+  } catch (const std::exception& e) {
+    return fail(LOITS_HIP_ERUNTIME, std::string("batched_encode: ") + e.what());
+  }
+}
+
+int loits_hip_batched_forward(const loits_hip_batched_args* args, void* stream) {
+  try {
+    const int st = check_args(args);
+    if (st != LOITS_HIP_OK) return st;
+    const loits_hip_layout& L = *args->layout;
+    const BatchedParams P = params_of(args);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const int tthreads = std::min(L.threads, 1024);
+    const bool f32 = L.dtype == LOITS_HIP_F32;
+    int rc = check_hip(f32 ? launch_tables<float>(P, tthreads, L.scan_mode == 1, s)
+                           : launch_tables<double>(P, tthreads, L.scan_mode == 1, s),
+                       "batched tables launch");
+    if (rc != LOITS_HIP_OK) return rc;
+    rc = check_hip(f32 ? launch_sample<float>(P, L.threads, L.lds_forward, s)
+                       : launch_sample<double>(P, L.threads, L.lds_forward, s),
+                   "batched sample launch");
+    if (rc != LOITS_HIP_OK) return rc;
+    return check_hip(launch_finalize(P, s), "batched finalize launch");
+  } catch (const std::exception& e) {
+    return fail(LOITS_HIP_ERUNTIME, std::string("batched_forward: ") + e.what());
+  }
+}
+
+int loits_hip_batched_pack(const loits_hip_batched_args* args, void* packed, int32_t* packed_bins,
+                           void* stream) {
+  try {
+    const int st = check_args(args);
+    if (st != LOITS_HIP_OK) return st;
+    if (!packed) return fail(LOITS_HIP_EINVAL, "batched_pack: packed is NULL");
+    const BatchedParams P = params_of(args);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    return check_hip(args->layout->dtype == LOITS_HIP_F32
+                         ? launch_pack<float>(P, packed, packed_bins, s)
+                         : launch_pack<double>(P, packed, packed_bins, s),
+                     "batched pack launch");
+  } catch (const std::exception& e) {
+    return fail(LOITS_HIP_ERUNTIME, std::string("batched_pack: ") + e.what());
+  }
+}
+
+int loits_hip_batched_backward(const loits_hip_batched_args* args, const void* grad_events,
+                               void* grad_density, int32_t recompute_tables, void* stream) {
+  try {
+    const int st = check_args(args);
+    if (st != LOITS_HIP_OK) return st;
+    if (!grad_events && args->layout->capacity > 0)
+      return fail(LOITS_HIP_EINVAL, "batched_backward: grad_events is NULL");
+    if (!grad_density) return fail(LOITS_HIP_EINVAL, "batched_backward: grad_density is NULL");
+    const loits_hip_layout& L = *args->layout;
+    const BatchedParams P = params_of(args);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const bool f32 = L.dtype == LOITS_HIP_F32;
+    if (recompute_tables) {
+      BatchedParams Q = P;
+      Q.norm_out = nullptr;
+      Q.cdf_out = nullptr;
+      const int tthreads = std::min(L.threads, 1024);
+      const int rc = check_hip(f32 ? launch_tables<float>(Q, tthreads, L.scan_mode == 1, s)
+                                   : launch_tables<double>(Q, tthreads, L.scan_mode == 1, s),
+                               "batched tables launch");
+      if (rc != LOITS_HIP_OK) return rc;
+    }
+    const int maxper = int((L.max_nodes + L.threads - 1) / L.threads);
+    return check_hip(f32 ? launch_vjp<float>(P, L.threads, L.lds_backward, maxper, grad_events,
+                                             grad_density, s)
+                         : launch_vjp<double>(P, L.threads, L.lds_backward, maxper, grad_events,
+                                              grad_density, s),
+                     "batched vjp launch");
+  } catch (const std::exception& e) {
+    return fail(LOITS_HIP_ERUNTIME, std::string("batched_backward: ") + e.what());
+  }
+}
+
+}  // extern "C"

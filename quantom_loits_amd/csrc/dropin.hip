@@ -1,0 +1,695 @@
+// dropin.hip -- the reference drop-in (2-axis) path for gfx950.
+//
+// One call reproduces one reference `forward` / `backward`
+// (cpp/bindings.cpp:24-208) on device memory:
+//   alloc_kernel       counts = trunc(|w * n_events|), nmax, active (cpp/loits_core.cpp:22-37)
+//   curve_kernel       one thread per x- or q-curve: the reference's sequential
+//                      norm / rho / CDF chain (:55-165), integer thresholds and
+//                      per-bin slopes m_j (:193-194)
+//   sample_kernel      one tile of 4 * 256 padded slots per work-group: both
+//                      Philox streams for every padded slot (:39-53,
+//                      cpp/bindings.cpp:79-92), interval + interpolation for
+//                      active slots (:167-237), and the stable compaction
+//                      (:239-266) in a single pass, tiles ordered by a
+//                      decoupled look-back
+//   row_offsets_kernel openmp/loits_core.cpp:244-262 from the packed slots
+//   ivjp_kernel        one thread per cell walks its rows in order (the
+//                      accumulation order of openmp/loits_core.cpp:285-352)
+//   curve_vjp_kernel   one thread per curve: CDF and density adjoints (:340-472)
+// Built with -ffp-contract=off; every fused multiply-add of the reference
+// build is an explicit fma, so fp64 inputs give the reference's bits.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "capi_internal.hpp"
+#include "loits_device.hpp"
+#include "loits_hip.h"
+
+namespace qloits {
+namespace {
+
+constexpr int kTileThreads = 256;
+constexpr int64_t kTileSlots = 4 * kTileThreads;
+constexpr uint64_t kStatusShift = 62;
+constexpr uint64_t kAggregate = 1ull << kStatusShift;
+constexpr uint64_t kInclusive = 2ull << kStatusShift;
+constexpr uint64_t kValueMask = (1ull << kStatusShift) - 1;
+constexpr uint64_t kSpinLimit = 1ull << 28;
+
+template <typename Real>
+__device__ __forceinline__ double ld(const void* p, int64_t i) {
+  return double(static_cast<const Real*>(p)[i]);
+}
+
+// ---------------------------------------------------------------------------
+template <typename Real>
+__global__ void __launch_bounds__(1024) alloc_kernel(const void* weights, int64_t n,
+                                                     int64_t n_events, int64_t* counts,
+                                                     int64_t* out) {
+  __shared__ int64_t smax[16], ssum[16];
+  const double scale = double(n_events);
+  int64_t mx = 0, sum = 0;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+    const double v = __builtin_fabs(ld<Real>(weights, i) * scale);
+    int64_t c;
+    if (!(v == v)) c = 0;
+    else if (v >= 4611686018427387904.0) c = int64_t(1) << 62;
+    else c = static_cast<int64_t>(v);
+    counts[i] = c;
+    mx = c > mx ? c : mx;
+    sum += c;
+  }
+  for (int d = 32; d >= 1; d >>= 1) {
+    const int64_t om = __shfl_xor(mx, d, kWave), os = __shfl_xor(sum, d, kWave);
+    mx = om > mx ? om : mx;
+    sum += os;
+  }
+  const int wave = threadIdx.x >> 6;
+  if (lane_id() == 0) {
+    smax[wave] = mx;
+    ssum[wave] = sum;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int64_t m = 0, s = 0;
+    for (int w = 0; w < int(blockDim.x >> 6); ++w) {
+      m = smax[w] > m ? smax[w] : m;
+      s += ssum[w];
+    }
+    out[0] = m;
+    out[1] = s;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Curve tables.  Curve c < nxc is the x-curve stored at xsec_x[c] (c =
+// (b*ny + iy)*nx + ix, nodes x_bins[b, ix]); otherwise the q-curve
+// c - nxc stored at xsec_q (c' = (b*nx + ix)*ny + iy, nodes q_bins[b, iy]).
+struct CurveParams {
+  loits_hip_shape s;
+  const void* x_bins;
+  const void* xsec_x;
+  const void* q_bins;
+  const void* xsec_q;
+  const uint8_t* acc;
+  double* norm_x;
+  double* norm_q;
+  double* cdf_x;
+  double* cdf_q;
+  uint32_t* thr_x;
+  uint32_t* thr_q;
+  double* m_x;
+  double* m_q;
+  uint8_t* mono;  // [x curves | q curves]
+};
+
+__device__ __forceinline__ bool norm_term_fused(int64_t i, int64_t nterms) {
+  return nterms <= 3 || ((nterms & 1) && i == nterms - 1);
+}
+
+template <typename Real>
+__global__ void __launch_bounds__(256) curve_kernel(CurveParams P) {
+  const loits_hip_shape& s = P.s;
+  const int64_t nxc = s.batch * s.ny * s.nx, nqc = s.batch * s.nx * s.ny;
+  const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (c >= nxc + nqc) return;
+  const bool is_x = c < nxc;
+  const int64_t cc = is_x ? c : c - nxc;
+  int64_t b, ix, iy, k;
+  const void* bins;
+  const void* xsec;
+  double *norm, *cdf, *m;
+  uint32_t* thr;
+  int64_t bins_off;
+  if (is_x) {
+    b = cc / (s.ny * s.nx);
+    const int64_t r = cc - b * s.ny * s.nx;
+    iy = r / s.nx;
+    ix = r - iy * s.nx;
+    k = s.kx;
+    bins = P.x_bins;
+    bins_off = (b * s.nx + ix) * k;
+    xsec = P.xsec_x;
+    norm = P.norm_x;
+    cdf = P.cdf_x;
+    thr = P.thr_x;
+    m = P.m_x;
+  } else {
+    b = cc / (s.nx * s.ny);
+    const int64_t r = cc - b * s.nx * s.ny;
+    ix = r / s.ny;
+    iy = r - ix * s.ny;
+    k = s.kq;
+    bins = P.q_bins;
+    bins_off = (b * s.ny + iy) * k;
+    xsec = P.xsec_q;
+    norm = P.norm_q;
+    cdf = P.cdf_q;
+    thr = P.thr_q;
+    m = P.m_q;
+  }
+  const int64_t off = cc * k;
+  const double acc = P.acc[(b * s.nx + ix) * s.ny + iy] ? 1.0 : 0.0;
+  // norm: cpp/loits_core.cpp:73-80 (fusion pattern of the reference build)
+  double nrm = 0.0;
+  for (int64_t t = 1; t < k; ++t) {
+    const double h = 0.5 * (ld<Real>(bins, bins_off + t) - ld<Real>(bins, bins_off + t - 1));
+    const double sum = ld<Real>(xsec, off + t - 1) + ld<Real>(xsec, off + t);
+    nrm = norm_term_fused(t - 1, k - 1) ? __builtin_fma(h, sum, nrm) : nrm + h * sum;
+  }
+  norm[cc] = nrm;
+  const double inv = 1.0 / nrm;
+  // CDF: cpp/loits_core.cpp:130-135
+  double cum = 0.0, rho_prev = ld<Real>(xsec, off) * inv;
+  cdf[off] = 0.0;
+  uint32_t prev = 0u;
+  bool mono = true;
+  thr[off] = 0u;
+  for (int64_t t = 1; t < k; ++t) {
+    const double rho = ld<Real>(xsec, off + t) * inv;
+    cum = __builtin_fma(0.5 * (ld<Real>(bins, bins_off + t) - ld<Real>(bins, bins_off + t - 1)),
+                        rho_prev + rho, cum);
+    const double cv = cum * acc;
+    cdf[off + t] = cv;
+    const uint32_t tv = threshold_of(cv);
+    thr[off + t] = tv;
+    mono = mono && tv >= prev;
+    prev = tv;
+    rho_prev = rho;
+  }
+  // slopes: cpp/loits_core.cpp:192-194
+  for (int64_t j = 0; j + 1 < k; ++j) {
+    const double inv_d = 1.0 / (cdf[off + j + 1] - cdf[off + j] + kEpsilon);
+    m[off + j] = (ld<Real>(bins, bins_off + j + 1) - ld<Real>(bins, bins_off + j)) * inv_d;
+  }
+  P.mono[c] = mono ? 1 : 0;
+}
+
+// count(k) = #{t : T_t <= k}; interval = clamp(count - 1, 0, K - 2)
+__device__ __forceinline__ int64_t interval_of(const uint32_t* thr, int64_t K, bool mono,
+                                               uint32_t key) {
+  int64_t count;
+  if (mono) {
+    int64_t lo = 0, hi = K;
+    while (lo < hi) {
+      const int64_t mid = (lo + hi) >> 1;
+      if (thr[mid] <= key) lo = mid + 1; else hi = mid;
+    }
+    count = lo;
+  } else {
+    count = 0;
+    for (int64_t t = 0; t < K; ++t) count += thr[t] <= key ? 1 : 0;
+  }
+  int64_t j = count - 1;
+  j = j < 0 ? 0 : j;
+  return j > K - 2 ? K - 2 : j;
+}
+
+// ---------------------------------------------------------------------------
+struct SampleParams {
+  loits_hip_shape s;
+  int64_t nmax, total_slots, n_tiles;
+  uint64_t seed, sid_x, sid_q;
+  const void* x_bins;
+  const void* q_bins;
+  const int64_t* counts;
+  const double* cdf_x;
+  const double* cdf_q;
+  const uint32_t* thr_x;
+  const uint32_t* thr_q;
+  const double* m_x;
+  const double* m_q;
+  const uint8_t* mono;
+  float* u_x;
+  float* u_q;
+  int16_t* interval_x;
+  int16_t* interval_q;
+  void* events;
+  int64_t* packed;
+  uint64_t* tile_status;
+  uint32_t* tile_counter;
+  int64_t* valid;  // [0] kept, [1] error flag
+};
+
+template <typename Real>
+__global__ void __launch_bounds__(kTileThreads) sample_kernel(SampleParams P) {
+  __shared__ int32_t wave_tot[16];
+  __shared__ int64_t sh_tile, sh_excl;
+  const loits_hip_shape& s = P.s;
+  if (threadIdx.x == 0) sh_tile = int64_t(atomicAdd(P.tile_counter, 1u));
+  __syncthreads();
+  const int64_t tile = sh_tile;
+  const int64_t q = tile * kTileThreads + threadIdx.x;  // Philox block of this thread
+  const Words4 wx = philox4x32_10(uint64_t(q), P.sid_x, P.seed);
+  const Words4 wq = philox4x32_10(uint64_t(q), P.sid_q, P.seed);
+  double vx[4], vq[4];
+  int keepbits = 0;
+  const int64_t nxc = s.batch * s.ny * s.nx;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const int64_t p = 4 * q + e;
+    if (p >= P.total_slots) continue;
+    const uint32_t kx = word_of(wx, e) >> 8, kq = word_of(wq, e) >> 8;
+    P.u_x[p] = float(kx) * 0x1.0p-24f;
+    P.u_q[p] = float(kq) * 0x1.0p-24f;
+    const int64_t g = p / P.nmax, slot = p - g * P.nmax;
+    if (slot >= P.counts[g]) {
+      P.interval_x[p] = 0;
+      P.interval_q[p] = 0;
+      continue;
+    }
+    const int64_t b = g / s.cells, cell = g - b * s.cells;
+    const int64_t ix = cell / s.ny, iy = cell - ix * s.ny;
+    const int64_t cxi = (b * s.ny + iy) * s.nx + ix, cqi = (b * s.nx + ix) * s.ny + iy;
+    // x: cpp/loits_core.cpp:183-196
+    const int64_t ox = cxi * s.kx, bx = (b * s.nx + ix) * s.kx;
+    const int64_t jx = interval_of(P.thr_x + ox, s.kx, P.mono[cxi] != 0, kx);
+    const double ux = double(kx) * 0x1.0p-24;
+    vx[e] = __builtin_fma(P.m_x[ox + jx], ux - P.cdf_x[ox + jx], ld<Real>(P.x_bins, bx + jx));
+    // q: cpp/loits_core.cpp:219-232
+    const int64_t oq = cqi * s.kq, bq = (b * s.ny + iy) * s.kq;
+    const int64_t jq = interval_of(P.thr_q + oq, s.kq, P.mono[nxc + cqi] != 0, kq);
+    const double uq = double(kq) * 0x1.0p-24;
+    vq[e] = __builtin_fma(P.m_q[oq + jq], uq - P.cdf_q[oq + jq], ld<Real>(P.q_bins, bq + jq));
+    P.interval_x[p] = int16_t(jx);
+    P.interval_q[p] = int16_t(jq);
+    // keep rule: cpp/loits_core.cpp:256
+    if (__builtin_isfinite(vx[e]) && __builtin_isfinite(vq[e]) && vx[e] * vq[e] != 0.0)
+      keepbits |= 1 << e;
+  }
+  // in-tile ordered offsets
+  const int cnt = __popc(keepbits);
+  const uint64_t lt = lanemask_lt();
+  const uint64_t b0 = ballot(cnt & 1), b1 = ballot(cnt & 2), b2 = ballot(cnt & 4);
+  const int pre = popc64(b0 & lt) + 2 * popc64(b1 & lt) + 4 * popc64(b2 & lt);
+  const int wave = threadIdx.x >> 6;
+  if (lane_id() == 0) wave_tot[wave] = popc64(b0) + 2 * popc64(b1) + 4 * popc64(b2);
+  __syncthreads();
+  int base = 0, agg = 0;
+  for (int w = 0; w < kTileThreads / 64; ++w) {
+    agg += wave_tot[w];
+    base += w < wave ? wave_tot[w] : 0;
+  }
+  // decoupled look-back for the tile's first row
+  if (threadIdx.x == 0) {
+    uint64_t* st = P.tile_status;
+    if (tile == 0) {
+      __hip_atomic_store(&st[0], kInclusive | uint64_t(agg), __ATOMIC_RELEASE,
+                         __HIP_MEMORY_SCOPE_AGENT);
+      sh_excl = 0;
+    } else {
+      __hip_atomic_store(&st[tile], kAggregate | uint64_t(agg), __ATOMIC_RELEASE,
+                         __HIP_MEMORY_SCOPE_AGENT);
+      int64_t excl = 0;
+      int64_t prev = tile - 1;
+      uint64_t spins = 0;
+      while (prev >= 0) {
+        const uint64_t v = __hip_atomic_load(&st[prev], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        const uint64_t flag = v & ~kValueMask;
+        if (flag == 0) {
+          if (++spins > kSpinLimit) {  // a bounded wait: report instead of hanging
+            atomicOr(reinterpret_cast<unsigned long long*>(&P.valid[1]), 1ull);
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+          continue;
+        }
+        excl += int64_t(v & kValueMask);
+        if (flag == kInclusive) break;
+        --prev;
+      }
+      __hip_atomic_store(&st[tile], kInclusive | uint64_t(excl + agg), __ATOMIC_RELEASE,
+                         __HIP_MEMORY_SCOPE_AGENT);
+      sh_excl = excl;
+    }
+    if (tile == P.n_tiles - 1) P.valid[0] = sh_excl + agg;
+  }
+  __syncthreads();
+  int64_t row = sh_excl + base + pre;
+  Real* ev = static_cast<Real*>(P.events);
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    if ((keepbits >> e) & 1) {
+      ev[2 * row] = Real(vx[e]);
+      ev[2 * row + 1] = Real(vq[e]);
+      P.packed[row] = 4 * q + e;
+      ++row;
+    }
+  }
+}
+
+__global__ void row_offsets_kernel(const int64_t* packed, const int64_t* valid, int64_t nmax,
+                                   int64_t gcells, int64_t* row_offsets) {
+  const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (g > gcells) return;
+  const int64_t key = g * nmax;
+  int64_t lo = 0, hi = valid[0];
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (packed[mid] < key) lo = mid + 1; else hi = mid;
+  }
+  row_offsets[g] = lo;
+}
+
+// ---------------------------------------------------------------------------
+// Backward.
+struct VjpParams {
+  loits_hip_shape s;
+  int64_t nmax, nvalid;
+  const void* grad;
+  const void* x_bins;
+  const void* xsec_x;
+  const void* q_bins;
+  const void* xsec_q;
+  const uint8_t* acc;
+  const double* norm_x;
+  const double* norm_q;
+  const double* cdf_x;
+  const double* cdf_q;
+  const float* u_x;
+  const float* u_q;
+  const int16_t* interval_x;
+  const int16_t* interval_q;
+  const int64_t* packed;
+  double* gcdf_x;
+  double* gcdf_q;
+  double* grho;  // scratch per curve, same layout as gcdf (x then q)
+  void* grad_x;
+  void* grad_q;
+};
+
+// cpp/loits_core.cpp:294-301, one row
+__device__ __forceinline__ void ivjp_row(const double* curve, double* gcurve, double bj,
+                                         double bj1, int64_t j, double uv, double gev) {
+  const double c0 = curve[j];
+  const double d = curve[j + 1] - c0 + kEpsilon;
+  const double inv_d = 1.0 / d;
+  const double t = uv - c0;
+  const double upstream = gev * (bj1 - bj);
+  gcurve[j] = __builtin_fma(upstream, __builtin_fma(t * inv_d, inv_d, -inv_d), gcurve[j]);
+  gcurve[j + 1] = __builtin_fma(-(upstream * t * inv_d), inv_d, gcurve[j + 1]);
+}
+
+template <typename Real>
+__global__ void __launch_bounds__(256) ivjp_kernel(VjpParams P) {
+  const loits_hip_shape& s = P.s;
+  const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (g >= s.batch * s.cells) return;
+  const int64_t b = g / s.cells, cell = g - b * s.cells;
+  const int64_t ix = cell / s.ny, iy = cell - ix * s.ny;
+  const int64_t ox = ((b * s.ny + iy) * s.nx + ix) * s.kx, bx = (b * s.nx + ix) * s.kx;
+  const int64_t oq = ((b * s.nx + ix) * s.ny + iy) * s.kq, bq = (b * s.ny + iy) * s.kq;
+  for (int64_t t = 0; t < s.kx; ++t) P.gcdf_x[ox + t] = 0.0;
+  for (int64_t t = 0; t < s.kq; ++t) P.gcdf_q[oq + t] = 0.0;
+  // rows of this cell: packed slots in [g*nmax, (g+1)*nmax)
+  int64_t lo = 0, hi = P.nvalid;
+  const int64_t key = g * P.nmax;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (P.packed[mid] < key) lo = mid + 1; else hi = mid;
+  }
+  for (int64_t row = lo; row < P.nvalid && P.packed[row] < key + P.nmax; ++row) {
+    const int64_t p = P.packed[row];
+    const int64_t jx = P.interval_x[p], jq = P.interval_q[p];
+    ivjp_row(P.cdf_x + ox, P.gcdf_x + ox, ld<Real>(P.x_bins, bx + jx),
+             ld<Real>(P.x_bins, bx + jx + 1), jx, double(P.u_x[p]), ld<Real>(P.grad, 2 * row));
+    ivjp_row(P.cdf_q + oq, P.gcdf_q + oq, ld<Real>(P.q_bins, bq + jq),
+             ld<Real>(P.q_bins, bq + jq + 1), jq, double(P.u_q[p]),
+             ld<Real>(P.grad, 2 * row + 1));
+  }
+}
+
+template <typename Real>
+__global__ void __launch_bounds__(256) curve_vjp_kernel(VjpParams P) {
+  const loits_hip_shape& s = P.s;
+  const int64_t nxc = s.batch * s.ny * s.nx, nqc = s.batch * s.nx * s.ny;
+  const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (c >= nxc + nqc) return;
+  const bool is_x = c < nxc;
+  const int64_t cc = is_x ? c : c - nxc;
+  int64_t b, ix, iy, k, bins_off;
+  const void *bins, *xsec;
+  const double *gc, *norm;
+  void* out;
+  double* gr;
+  if (is_x) {
+    b = cc / (s.ny * s.nx);
+    const int64_t r = cc - b * s.ny * s.nx;
+    iy = r / s.nx;
+    ix = r - iy * s.nx;
+    k = s.kx;
+    bins = P.x_bins;
+    bins_off = (b * s.nx + ix) * k;
+    xsec = P.xsec_x;
+    gc = P.gcdf_x + cc * k;
+    norm = P.norm_x;
+    out = P.grad_x;
+    gr = P.grho + cc * k;
+  } else {
+    b = cc / (s.nx * s.ny);
+    const int64_t r = cc - b * s.nx * s.ny;
+    ix = r / s.ny;
+    iy = r - ix * s.ny;
+    k = s.kq;
+    bins = P.q_bins;
+    bins_off = (b * s.ny + iy) * k;
+    xsec = P.xsec_q;
+    gc = P.gcdf_q + cc * k;
+    norm = P.norm_q;
+    out = P.grad_q;
+    gr = P.grho + nxc * s.kx + cc * k;
+  }
+  const int64_t off = cc * k;
+  const double acc = P.acc[(b * s.nx + ix) * s.ny + iy] ? 1.0 : 0.0;
+  // CDF adjoint: cpp/loits_core.cpp:355-365
+  for (int64_t t = 0; t < k; ++t) gr[t] = 0.0;
+  double suffix = 0.0;
+  for (int64_t t = k - 1; t > 0; --t) {
+    suffix += gc[t];
+    const double h = 0.5 * (ld<Real>(bins, bins_off + t) - ld<Real>(bins, bins_off + t - 1)) * acc;
+    gr[t - 1] = __builtin_fma(h, suffix, gr[t - 1]);
+    gr[t] = __builtin_fma(h, suffix, gr[t]);
+  }
+  // density adjoint: cpp/loits_core.cpp:416-431 (grad_norm fusion pattern of
+  // the reference build: only an odd trailing term is fused)
+  const double n = norm[cc];
+  const double inv_n = 1.0 / n;
+  const double inv_n2 = inv_n * inv_n;
+  double grad_norm = 0.0;
+  Real* gx = static_cast<Real*>(out) + off;
+  for (int64_t t = 0; t < k; ++t) {
+    const double term = gr[t] * ld<Real>(xsec, off + t);
+    grad_norm = ((k & 1) && t == k - 1) ? __builtin_fma(-term, inv_n2, grad_norm)
+                                        : grad_norm - term * inv_n2;
+  }
+  double prev = gr[0] * inv_n;
+  for (int64_t t = 1; t < k; ++t) {
+    const double h = 0.5 * (ld<Real>(bins, bins_off + t) - ld<Real>(bins, bins_off + t - 1));
+    const double cur = gr[t] * inv_n;
+    gx[t - 1] = Real(__builtin_fma(h, grad_norm, prev));
+    prev = __builtin_fma(h, grad_norm, cur);
+  }
+  gx[k - 1] = Real(prev);
+}
+
+int64_t align16(int64_t v) { return (v + 15) & ~int64_t(15); }
+
+struct FwdWorkspace {
+  uint32_t* thr_x;
+  uint32_t* thr_q;
+  double* m_x;
+  double* m_q;
+  uint8_t* mono;
+  uint64_t* tile_status;
+  uint32_t* tile_counter;
+  int64_t bytes;
+};
+
+FwdWorkspace fwd_workspace(const loits_hip_shape& s, int64_t nmax, void* base) {
+  const int64_t nxe = s.batch * s.ny * s.nx * s.kx, nqe = s.batch * s.nx * s.ny * s.kq;
+  const int64_t curves = s.batch * s.cells * 2;
+  const int64_t slots = s.batch * s.cells * nmax;
+  const int64_t tiles = (slots + kTileSlots - 1) / kTileSlots;
+  uint8_t* p = static_cast<uint8_t*>(base);
+  FwdWorkspace w{};
+  int64_t off = 0;
+  w.tile_counter = reinterpret_cast<uint32_t*>(p + off);
+  off += 64;
+  w.tile_status = reinterpret_cast<uint64_t*>(p + off);
+  off += align16(8 * (tiles > 0 ? tiles : 1));
+  w.thr_x = reinterpret_cast<uint32_t*>(p + off);
+  off += align16(4 * nxe);
+  w.thr_q = reinterpret_cast<uint32_t*>(p + off);
+  off += align16(4 * nqe);
+  w.m_x = reinterpret_cast<double*>(p + off);
+  off += align16(8 * nxe);
+  w.m_q = reinterpret_cast<double*>(p + off);
+  off += align16(8 * nqe);
+  w.mono = p + off;
+  off += align16(curves);
+  w.bytes = off;
+  return w;
+}
+
+__global__ void zero_words(uint64_t* p, int64_t n) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0ull;
+}
+
+int check_shape(const loits_hip_shape& s) {
+  if (s.batch < 1 || s.nx < 1 || s.ny < 1 || s.kx < 2 || s.kq < 2 || s.cells != s.nx * s.ny)
+    return fail(LOITS_HIP_EINVAL, "shape: need batch, nx, ny >= 1, kx, kq >= 2, cells = nx*ny");
+  if (s.kx > 32767 || s.kq > 32767)
+    return fail(LOITS_HIP_EUNSUPPORTED, "shape: the int16 interval caps kx, kq at 32767");
+  return LOITS_HIP_OK;
+}
+
+unsigned blocks_for(int64_t n, int threads) { return unsigned((n + threads - 1) / threads); }
+
+}  // namespace
+}  // namespace qloits
+
+using namespace qloits;
+
+extern "C" {
+
+int loits_hip_allocate(const void* weights, int dtype, int64_t n, int64_t n_events,
+                       int64_t* counts, int64_t* nmax_active, void* stream) {
+  if (n < 0 || (n > 0 && (!weights || !counts)) || !nmax_active)
+    return fail(LOITS_HIP_EINVAL, "allocate: bad arguments");
+  if (n_events < 0) return fail(LOITS_HIP_EINVAL, "allocate: n_events must be >= 0");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (dtype == LOITS_HIP_F64)
+    hipLaunchKernelGGL(alloc_kernel<double>, dim3(1), dim3(1024), 0, s, weights, n, n_events,
+                       counts, nmax_active);
+  else if (dtype == LOITS_HIP_F32)
+    hipLaunchKernelGGL(alloc_kernel<float>, dim3(1), dim3(1024), 0, s, weights, n, n_events,
+                       counts, nmax_active);
+  else
+    return fail(LOITS_HIP_EINVAL, "allocate: bad dtype");
+  return check_hip(hipGetLastError(), "allocate launch");
+}
+
+int64_t loits_hip_forward_workspace_size(const loits_hip_shape* shape, int64_t nmax) {
+  if (!shape || check_shape(*shape) != LOITS_HIP_OK || nmax < 0) return -1;
+  return fwd_workspace(*shape, nmax, nullptr).bytes;
+}
+
+int loits_hip_forward(const loits_hip_forward_args* a, void* stream) {
+  if (!a) return fail(LOITS_HIP_EINVAL, "forward: args is NULL");
+  const loits_hip_shape& s = a->shape;
+  int rc = check_shape(s);
+  if (rc != LOITS_HIP_OK) return rc;
+  if (a->dtype != LOITS_HIP_F32 && a->dtype != LOITS_HIP_F64)
+    return fail(LOITS_HIP_EINVAL, "forward: bad dtype");
+  if (!a->x_bins || !a->xsec_x || !a->q_bins || !a->xsec_q || !a->acceptance || !a->counts ||
+      !a->norm_x || !a->norm_q || !a->cdf_x || !a->cdf_q || !a->workspace || !a->valid)
+    return fail(LOITS_HIP_EINVAL, "forward: a required pointer is NULL");
+  if (a->nmax < 0 || a->active < 0) return fail(LOITS_HIP_EINVAL, "forward: bad nmax/active");
+  const int64_t slots = s.batch * s.cells * a->nmax;
+  if (slots > 0 && (!a->u_x || !a->u_q || !a->interval_x || !a->interval_q))
+    return fail(LOITS_HIP_EINVAL, "forward: u/interval outputs are NULL");
+  if (a->active > 0 && (!a->events || !a->packed))
+    return fail(LOITS_HIP_EINVAL, "forward: events/packed outputs are NULL");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const FwdWorkspace w = fwd_workspace(s, a->nmax, a->workspace);
+  const int64_t tiles = (slots + kTileSlots - 1) / kTileSlots;
+  const bool f32 = a->dtype == LOITS_HIP_F32;
+
+  CurveParams C{s,          a->x_bins, a->xsec_x, a->q_bins, a->xsec_q, a->acceptance,
+                a->norm_x,  a->norm_q, a->cdf_x,  a->cdf_q,  w.thr_x,   w.thr_q,
+                w.m_x,      w.m_q,     w.mono};
+  const int64_t curves = 2 * s.batch * s.cells;
+  if (f32)
+    hipLaunchKernelGGL(curve_kernel<float>, dim3(blocks_for(curves, 256)), dim3(256), 0, st, C);
+  else
+    hipLaunchKernelGGL(curve_kernel<double>, dim3(blocks_for(curves, 256)), dim3(256), 0, st, C);
+  rc = check_hip(hipGetLastError(), "forward curve launch");
+  if (rc != LOITS_HIP_OK) return rc;
+  // zero: tile counter, look-back status words, valid[0..1]
+  const int64_t zwords = (64 + align16(8 * (tiles > 0 ? tiles : 1))) / 8;
+  hipLaunchKernelGGL(zero_words, dim3(blocks_for(zwords, 256)), dim3(256), 0, st,
+                     reinterpret_cast<uint64_t*>(w.tile_counter), zwords);
+  hipLaunchKernelGGL(zero_words, dim3(1), dim3(64), 0, st,
+                     reinterpret_cast<uint64_t*>(a->valid), int64_t(2));
+  rc = check_hip(hipGetLastError(), "forward zero launch");
+  if (rc != LOITS_HIP_OK) return rc;
+  if (tiles > 0) {
+    if (tiles > INT32_MAX) return fail(LOITS_HIP_EUNSUPPORTED, "forward: too many slots");
+    SampleParams S{s,          a->nmax,    slots,      tiles,       a->seed,
+                   2 * a->sequence,        2 * a->sequence + 1,     a->x_bins,
+                   a->q_bins,  a->counts,  a->cdf_x,   a->cdf_q,    w.thr_x,
+                   w.thr_q,    w.m_x,      w.m_q,      w.mono,      a->u_x,
+                   a->u_q,     a->interval_x,          a->interval_q,
+                   a->events,  a->packed,  w.tile_status,           w.tile_counter,
+                   a->valid};
+    if (f32)
+      hipLaunchKernelGGL(sample_kernel<float>, dim3(unsigned(tiles)), dim3(kTileThreads), 0, st, S);
+    else
+      hipLaunchKernelGGL(sample_kernel<double>, dim3(unsigned(tiles)), dim3(kTileThreads), 0, st, S);
+    rc = check_hip(hipGetLastError(), "forward sample launch");
+    if (rc != LOITS_HIP_OK) return rc;
+  }
+  if (a->row_offsets) {
+    const int64_t gcells = s.batch * s.cells;
+    hipLaunchKernelGGL(row_offsets_kernel, dim3(blocks_for(gcells + 1, 256)), dim3(256), 0, st,
+                       a->packed, a->valid, a->nmax, gcells, a->row_offsets);
+    rc = check_hip(hipGetLastError(), "forward row_offsets launch");
+  }
+  return rc;
+}
+
+int64_t loits_hip_backward_workspace_size(const loits_hip_shape* shape, int64_t nmax) {
+  (void)nmax;
+  if (!shape || check_shape(*shape) != LOITS_HIP_OK) return -1;
+  const int64_t nxe = shape->batch * shape->ny * shape->nx * shape->kx;
+  const int64_t nqe = shape->batch * shape->nx * shape->ny * shape->kq;
+  return 2 * (align16(8 * nxe) + align16(8 * nqe));
+}
+
+int loits_hip_backward(const loits_hip_backward_args* a, void* stream) {
+  if (!a) return fail(LOITS_HIP_EINVAL, "backward: args is NULL");
+  const loits_hip_shape& s = a->shape;
+  int rc = check_shape(s);
+  if (rc != LOITS_HIP_OK) return rc;
+  if (a->dtype != LOITS_HIP_F32 && a->dtype != LOITS_HIP_F64)
+    return fail(LOITS_HIP_EINVAL, "backward: bad dtype");
+  if (!a->x_bins || !a->xsec_x || !a->q_bins || !a->xsec_q || !a->acceptance || !a->norm_x ||
+      !a->norm_q || !a->cdf_x || !a->cdf_q || !a->grad_xsec_x || !a->grad_xsec_q ||
+      !a->workspace)
+    return fail(LOITS_HIP_EINVAL, "backward: a required pointer is NULL");
+  if (a->nvalid > 0 && (!a->grad_events || !a->packed || !a->u_x || !a->u_q || !a->interval_x ||
+                        !a->interval_q))
+    return fail(LOITS_HIP_EINVAL, "backward: state pointers are NULL");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const int64_t nxe = s.batch * s.ny * s.nx * s.kx, nqe = s.batch * s.nx * s.ny * s.kq;
+  uint8_t* ws = static_cast<uint8_t*>(a->workspace);
+  double* gcdf_x = reinterpret_cast<double*>(ws);
+  double* gcdf_q = reinterpret_cast<double*>(ws + align16(8 * nxe));
+  double* grho = reinterpret_cast<double*>(ws + align16(8 * nxe) + align16(8 * nqe));
+  VjpParams V{s,         a->nmax,      a->nvalid,  a->grad_events, a->x_bins, a->xsec_x,
+              a->q_bins, a->xsec_q,    a->acceptance, a->norm_x,   a->norm_q, a->cdf_x,
+              a->cdf_q,  a->u_x,       a->u_q,     a->interval_x,  a->interval_q,
+              a->packed, gcdf_x,       gcdf_q,     grho,           a->grad_xsec_x,
+              a->grad_xsec_q};
+  const bool f32 = a->dtype == LOITS_HIP_F32;
+  const int64_t gcells = s.batch * s.cells;
+  if (f32)
+    hipLaunchKernelGGL(ivjp_kernel<float>, dim3(blocks_for(gcells, 256)), dim3(256), 0, st, V);
+  else
+    hipLaunchKernelGGL(ivjp_kernel<double>, dim3(blocks_for(gcells, 256)), dim3(256), 0, st, V);
+  rc = check_hip(hipGetLastError(), "backward interpolation launch");
+  if (rc != LOITS_HIP_OK) return rc;
+  const int64_t curves = 2 * gcells;
+  if (f32)
+    hipLaunchKernelGGL(curve_vjp_kernel<float>, dim3(blocks_for(curves, 256)), dim3(256), 0, st,
+                       V);
+  else
+    hipLaunchKernelGGL(curve_vjp_kernel<double>, dim3(blocks_for(curves, 256)), dim3(256), 0, st,
+                       V);
+  return check_hip(hipGetLastError(), "backward curve launch");
+}
+
+}  // extern "C"

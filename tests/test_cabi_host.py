@@ -1,0 +1,170 @@
+"""C ABI and host logic without a GPU: exports, layouts, argument errors, registry."""
+import ctypes as C
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+import torch  # noqa: F401  (PyTorch's HIP runtime first, as in the package)
+
+ROOT = Path(__file__).resolve().parents[1]
+HEADER = ROOT / "include" / "loits_hip.h"
+
+
+def declared_functions():
+    text = HEADER.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(loits_hip_[a-z0-9_]+)\s*\(", text)))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    from quantom_loits_amd._native import library_path
+
+    path = library_path()
+    if not path.is_file():
+        pytest.skip("HIP library not built")
+    return C.CDLL(str(path))
+
+
+def test_library_exports_every_declared_symbol(lib):
+    names = declared_functions()
+    assert len(names) >= 14
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+class Instance(C.Structure):
+    _fields_ = [("node_off", C.c_int64), ("n_nodes", C.c_int64), ("events", C.c_int64),
+                ("out_off", C.c_int64), ("counter_base", C.c_uint64), ("stream", C.c_uint64),
+                ("accept", C.c_int32), ("reserved", C.c_int32)]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("threads", C.c_int32), ("guide_shift", C.c_int32), ("chunk_events", C.c_int64),
+                ("scan_mode", C.c_int32), ("persistent", C.c_int32)]
+
+
+class Layout(C.Structure):
+    _fields_ = [(n, C.c_int64) for n in (
+        "n_instances", "n_chunks", "capacity", "total_nodes", "max_nodes", "desc_bytes",
+        "table_bytes", "scratch_bytes", "lds_forward", "lds_backward", "mask_words",
+        "partial_doubles")] + [(n, C.c_int32) for n in (
+            "dtype", "threads", "scan_mode", "persistent", "grid_forward", "grid_backward")]
+
+
+def make_instances(nodes, events):
+    arr = (Instance * len(nodes))()
+    off = out = 0
+    for i, (n, e) in enumerate(zip(nodes, events)):
+        arr[i] = Instance(off, n, e, out, 1000 * i, 0, 1, 0)
+        off += n
+        out += e
+    return arr
+
+
+def last_error(lib):
+    lib.loits_hip_last_error.restype = C.c_char_p
+    return lib.loits_hip_last_error().decode()
+
+
+def test_batched_layout_sizes_and_tiles(lib):
+    inst = make_instances([2048] * 3, [10000, 0, 4096])
+    tun = Tuning(256, 1, 4096, 0, 0)
+    lay = Layout()
+    assert lib.loits_hip_batched_layout(inst, 3, 1, C.byref(tun), C.byref(lay)) == 0
+    assert lay.capacity == 14096 and lay.total_nodes == 3 * 2048 and lay.max_nodes == 2048
+    assert lay.n_chunks == 3 + 1 + 1  # ceil(10000/4096), empty instance keeps one tile, 1
+    assert lay.lds_forward > 0 and lay.lds_backward > lay.lds_forward - 32 * 2048
+    assert lay.partial_doubles == 3 * 2 * 2048  # only the multi-tile instance
+    blob = (C.c_uint8 * lay.desc_bytes)()
+    assert lib.loits_hip_batched_encode(inst, 3, C.byref(tun), C.byref(lay), blob) == 0
+    # a layout for different tuning is rejected by encode
+    tun2 = Tuning(512, 1, 4096, 0, 0)
+    assert lib.loits_hip_batched_encode(inst, 3, C.byref(tun2), C.byref(lay), blob) == 1
+    assert "does not match" in last_error(lib)
+
+
+@pytest.mark.parametrize("bad,msg", [
+    (dict(threads=300), "threads must be"),
+    (dict(nodes=[1]), "n_nodes"),
+    (dict(out_shift=5), "out_off"),
+    (dict(nodes=[40000]), "LDS budget"),
+])
+def test_batched_layout_rejects_bad_arguments(lib, bad, msg):
+    nodes = bad.get("nodes", [64])
+    inst = make_instances(nodes, [100] * len(nodes))
+    if "out_shift" in bad:
+        inst[0].out_off = bad["out_shift"]
+    tun = Tuning(bad.get("threads", 256), 1, 4096, 0, 0)
+    lay = Layout()
+    rc = lib.loits_hip_batched_layout(inst, len(nodes), 1, C.byref(tun), C.byref(lay))
+    assert rc in (1, 3) and msg in last_error(lib)
+
+
+def test_workspace_size_queries(lib):
+    class Shape(C.Structure):
+        _fields_ = [(n, C.c_int64) for n in ("batch", "nx", "ny", "kx", "kq", "cells")]
+
+    lib.loits_hip_forward_workspace_size.restype = C.c_int64
+    lib.loits_hip_backward_workspace_size.restype = C.c_int64
+    s = Shape(1, 3, 3, 5, 5, 9)
+    assert lib.loits_hip_forward_workspace_size(C.byref(s), 80) > 0
+    assert lib.loits_hip_backward_workspace_size(C.byref(s), 80) >= 2 * 8 * 45 * 2
+    bad = Shape(1, 3, 3, 1, 5, 9)  # kx < 2
+    assert lib.loits_hip_forward_workspace_size(C.byref(bad), 80) == -1
+
+
+def test_version_string(lib):
+    lib.loits_hip_version.restype = C.c_char_p
+    assert b"gfx950" in lib.loits_hip_version()
+
+
+def test_pybind_plan_layout_on_host():
+    from quantom_loits_amd.batched import BatchedLOITS, Tuning as T
+
+    s = BatchedLOITS.uniform(4, 1024, 65536, 0, dtype=torch.float32, device="cpu",
+                             tuning=T(threads=1024, chunk_events=1 << 16))
+    assert s.capacity == 4 * 65536 and s.layout["n_chunks"] == 4
+    with pytest.raises(RuntimeError, match="n_nodes"):
+        BatchedLOITS(np.array([1]), np.array([5]), device="cpu")
+
+
+# --- registry (loits.py semantics) -------------------------------------------
+def test_registry_reports_hip_unavailable_on_cpu():
+    from quantom_loits_amd import BackendUnavailableError, LOITS, backend_status
+    from quantom_loits_amd.loits import registered_backends
+
+    assert "hip" in registered_backends()
+    ok, reason = backend_status("hip", "cpu")
+    assert not ok and "GPU" in reason
+    with pytest.raises(BackendUnavailableError, match="unavailable for device=cpu"):
+        LOITS(backend="hip", device="cpu")
+    with pytest.raises(ValueError, match="not registered"):
+        LOITS(backend="nope")
+
+
+def test_registry_probe_semantics():
+    from quantom_loits_amd.loits import _BACKENDS, _BACKEND_PROBES, backend_status, register_backend
+
+    def boom(device):
+        raise RuntimeError("probe exploded")
+
+    try:
+        register_backend("t_plain", lambda **kw: None)
+        register_backend("t_bool", lambda **kw: None, lambda d: False)
+        register_backend("t_raise", lambda **kw: None, boom)
+        assert backend_status("t_plain") == (True, "")
+        assert backend_status("t_bool") == (False, "backend probe returned unavailable")
+        assert backend_status("t_raise") == (False, "probe exploded")
+    finally:
+        for n in ("t_plain", "t_bool", "t_raise"):
+            _BACKENDS.pop(n, None)
+            _BACKEND_PROBES.pop(n, None)
+
+
+def test_hip_backend_rejects_other_epsilon():
+    from quantom_loits_amd.backend import HipLOITS
+
+    with pytest.raises(ValueError, match="epsilon 1e-5"):
+        HipLOITS(device="cuda", epsilon=1e-4)

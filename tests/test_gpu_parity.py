@@ -1,0 +1,267 @@
+"""HIP path vs the oracle and the compiled reference's golden vectors (GPU).
+
+Bars: integer outputs (Philox words, intervals, bin indices, counts, packed
+slots, compaction order) bit-exact; float64 drop-in outputs bit-exact against
+the reference fixtures; float32 batched values within 1e-5 relative and
+gradients within 1e-4 (normwise per instance) of the float64 oracle.
+"""
+import numpy as np
+import pytest
+import torch
+
+from conftest import reference_case
+from oracle.oracle import uniform_batch
+
+pytestmark = pytest.mark.gpu
+
+CASES = ["manual", "b1_g3_k5", "b3_g2_k10", "b1_g1_k2", "b1_g2_k64", "b1_g1_k1024", "dyadic",
+         "acc_zero", "nonfinite", "zero_bins"]
+
+
+@pytest.fixture(scope="module")
+def ext(hip_device):
+    from quantom_loits_amd import load_extension
+
+    return load_extension()
+
+
+def test_device_is_gfx950(ext):
+    arch, cus = ext.device_info(0)
+    print("device", arch, cus)
+    assert arch.startswith("gfx950") and cus > 0
+
+
+def test_philox_words_match_compiled_reference(ext, golden_philox):
+    grid, words = golden_philox["grid"], golden_philox["words"]
+    for (b, s, k), w in zip(grid, words):
+        if int(b) > 2 ** 63 - 5:
+            continue  # first_block is int64 at the C ABI
+        got = ext.philox_words(int(b), 1, int(s), int(k)).cpu().numpy().view(np.uint32)
+        assert np.array_equal(got, w), (b, s, k)
+
+
+def test_philox_kats(ext, golden_philox):
+    # tests/test_native_rng.py:43-55 and the Random123 known answers
+    got = ext.philox_words(0, 1, 0, 0).cpu().numpy().view(np.uint32)
+    assert got.tolist() == [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]
+    for row in golden_philox["kat"][1:]:
+        c, k, e = row[:4], row[4:6], row[6:]
+        block = int(c[0]) | (int(c[1]) << 32)
+        if block > 2 ** 63 - 1:
+            continue
+        got = ext.philox_words(block, 1, int(c[2]) | (int(c[3]) << 32),
+                               int(k[0]) | (int(k[1]) << 32)).cpu().numpy().view(np.uint32)
+        assert np.array_equal(got, e)
+
+
+def test_philox_fill_matches_reference_fill(ext, golden_philox):
+    for key in golden_philox.files:
+        if key.startswith("fill_"):
+            _, n, seed, stream = key.split("_")
+            got = ext.philox_fill(int(n), int(seed), int(stream)).cpu().numpy()
+            assert np.array_equal(got, golden_philox[key]), key
+
+
+def _to_dev(arrays, dtype):
+    out = []
+    for a in arrays:
+        if a.dtype == bool:
+            out.append(torch.from_numpy(np.ascontiguousarray(a)).cuda())
+        else:
+            out.append(torch.from_numpy(np.ascontiguousarray(a)).to(dtype).cuda())
+    return out
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_dropin_forward_backward_bit_exact_vs_reference(ext, golden_reference, name):
+    c = reference_case(golden_reference, name)
+    n_events, seed, sequence = (int(v) for v in c["scalars"])
+    args = _to_dev([c["x_bins"], c["xsec_x"], c["q_bins"], c["xsec_q"], c["weights"],
+                    c["acceptance"]], torch.float64)
+    print("case", name, flush=True)
+    st = ext.forward(*args, n_events, seed, sequence, False)
+    events, norm_x, norm_q, cdf_x, cdf_q, u_x, u_q, ix, iq, packed, row_offsets = st
+    np.testing.assert_array_equal(events.cpu().numpy(), c["events"])
+    for got, key in ((norm_x, "norm_x"), (norm_q, "norm_q"), (cdf_x, "cdf_x"), (cdf_q, "cdf_q"),
+                     (packed, "packed"), (row_offsets, "row_offsets")):
+        np.testing.assert_array_equal(got.cpu().numpy(), c[key], err_msg=key)
+    counts = c["counts"].reshape(-1)
+    nmax = u_x.shape[2]
+    active = np.zeros(u_x.numel(), dtype=bool)
+    for g, cnt in enumerate(counts):
+        active[g * nmax: g * nmax + cnt] = True
+    assert np.array_equal(ix.cpu().numpy()[active], c["interval_x"])
+    assert np.array_equal(iq.cpu().numpy()[active], c["interval_q"])
+    if "u_x" in c:
+        assert np.array_equal(u_x.cpu().numpy(), c["u_x"])
+        assert np.array_equal(u_q.cpu().numpy(), c["u_q"])
+    g = torch.from_numpy(c["grad_events"]).cuda()
+    gx, gq = ext.backward(g, args[0], args[1], args[2], args[3], args[5], norm_x, norm_q, cdf_x,
+                          cdf_q, u_x, u_q, ix, iq, packed, row_offsets, False)
+    np.testing.assert_array_equal(gx.cpu().numpy(), c["grad_xsec_x"])
+    np.testing.assert_array_equal(gq.cpu().numpy(), c["grad_xsec_q"])
+
+
+def test_dropin_float32_inputs_close_to_float64_reference(ext, golden_reference):
+    c = reference_case(golden_reference, "b3_g2_k10")
+    n_events, seed, sequence = (int(v) for v in c["scalars"])
+    raw = [c["x_bins"], c["xsec_x"], c["q_bins"], c["xsec_q"], c["weights"], c["acceptance"]]
+    # float32-representable inputs so that the float64 oracle sees the same data
+    raw32 = [a if a.dtype == bool else a.astype(np.float32).astype(np.float64) for a in raw]
+    from oracle.oracle import Oracle
+    ref = Oracle("port").forward(*raw32, n_events, seed, sequence)
+    st = ext.forward(*_to_dev(raw32, torch.float32), n_events, seed, sequence, False)
+    assert np.array_equal(st[9].cpu().numpy(), ref["packed"])
+    np.testing.assert_allclose(st[0].cpu().numpy().astype(np.float64), ref["events"], rtol=1e-6)
+
+
+def _batched_inputs(g, dtype):
+    x = torch.from_numpy(np.ascontiguousarray(g["x"])).to(dtype).cuda().reshape(-1)
+    d = torch.from_numpy(np.ascontiguousarray(g["density"])).to(dtype).cuda().reshape(-1)
+    return x, d
+
+
+@pytest.mark.parametrize("scan_mode", [1, 0])
+def test_batched_float64_matches_config0_fixture(golden_config0, hip_device, scan_mode):
+    from quantom_loits_amd.batched import BatchedLOITS, Tuning
+
+    g = golden_config0
+    B, N = g["x"].shape
+    E = int(g["n_events"])
+    print("batched f64 scan_mode", scan_mode, flush=True)
+    s = BatchedLOITS.uniform(B, N, E, int(g["sequence"]), dtype=torch.float64,
+                             tuning=Tuning(threads=256, chunk_events=1024, scan_mode=scan_mode))
+    x, d = _batched_inputs(g, torch.float64)
+    values, kept, st, ex = s.run_forward(x, d, int(g["seed"]), want_bins=True, want_tables=True)
+    assert kept.cpu().tolist() == [E] * B
+    assert np.array_equal(ex["bins"].cpu().numpy(), g["bins"])
+    if scan_mode == 1:  # the reference's sequential chain: bit-exact
+        assert np.array_equal(values.cpu().numpy(), g["values"])
+        assert np.array_equal(ex["cdf"].cpu().numpy().reshape(B, N), g["cdf"])
+        assert np.array_equal(ex["norm"].cpu().numpy(), g["norm"])
+    else:
+        np.testing.assert_allclose(values.cpu().numpy(), g["values"], rtol=1e-12)
+    grad = torch.from_numpy(g["grad_values"]).cuda()
+    gd = torch.empty_like(d)
+    s.backward_raw(x, d, int(g["seed"]), st, grad, gd)
+    ref = g["grad_density"]
+    got = gd.cpu().numpy().reshape(B, N)
+    err = np.abs(got - ref).max(axis=1) / np.abs(ref).max(axis=1)
+    print("f64 grad normwise rel err", err.max())
+    assert err.max() < 1e-10
+
+
+@pytest.mark.parametrize("threads,chunk", [(1024, 1 << 18), (256, 4096), (512, 2048)])
+def test_batched_float32_matches_config0_fixture(golden_config0, hip_device, threads, chunk):
+    from quantom_loits_amd.batched import BatchedLOITS, Tuning
+
+    g = golden_config0
+    B, N = g["x"].shape
+    E = int(g["n_events"])
+    print("batched f32", threads, chunk, flush=True)
+    s = BatchedLOITS.uniform(B, N, E, int(g["sequence"]), dtype=torch.float32,
+                             tuning=Tuning(threads=threads, chunk_events=chunk))
+    x, d = _batched_inputs(g, torch.float32)
+    values, kept, st, ex = s.run_forward(x, d, int(g["seed"]), want_bins=True)
+    assert np.array_equal(ex["bins"].cpu().numpy(), g["bins"])
+    v = values.cpu().numpy().astype(np.float64)
+    rel = np.abs(v - g["values"]) / np.abs(g["values"])
+    print("f32 value rel err", rel.max())
+    assert rel.max() <= 1e-5
+    grad = torch.from_numpy(g["grad_values"].astype(np.float32)).cuda()
+    gd = torch.empty_like(d)
+    s.backward_raw(x, d, int(g["seed"]), st, grad, gd)
+    ref = g["grad_density"]
+    got = gd.cpu().numpy().astype(np.float64).reshape(B, N)
+    err = np.abs(got - ref).max(axis=1) / np.abs(ref).max(axis=1)
+    print("f32 grad normwise rel err", err.max())
+    assert err.max() <= 1e-4
+
+
+def test_batched_ragged_window_matches_oracle(oracle_port, hip_device):
+    from quantom_loits_amd.batched import BatchedLOITS, Tuning
+    from quantom_loits_amd.workloads import ragged_family
+
+    w = ragged_family(12, n_range=(16, 2048), e_range=(1, 20000), seed=3, dtype=np.float64)
+    n = len(w["n_nodes"])
+    rng = np.random.default_rng(4)
+    base = rng.integers(0, 2 ** 40, n).astype(np.uint64)  # unaligned counter bases
+    streams = rng.integers(0, 2 ** 33, n).astype(np.uint64)
+    accept = np.ones(n, np.uint8)
+    accept[3] = 0
+    window = (0.05, 0.95)
+    ref_v, ref_b, ref_k, _, _ = oracle_port.batched_forward(
+        w["x"], w["density"], w["node_off"], w["n_nodes"], w["events"], base, streams, 99,
+        accept=accept, window=window)
+    for dtype in (torch.float64, torch.float32):
+        print("ragged window", dtype, flush=True)
+        s = BatchedLOITS(w["n_nodes"], w["events"], base, streams, accept, w["node_off"],
+                         dtype=dtype, window=window,
+                         tuning=Tuning(threads=256, chunk_events=4096, scan_mode=1))
+        x = torch.from_numpy(w["x"]).to(dtype).cuda()
+        d = torch.from_numpy(w["density"]).to(dtype).cuda()
+        if dtype == torch.float32:  # the oracle must see the same float32 data
+            x64, d64 = x.double().cpu().numpy(), d.double().cpu().numpy()
+            ref_v, ref_b, ref_k, _, _ = oracle_port.batched_forward(
+                x64, d64, w["node_off"], w["n_nodes"], w["events"], base, streams, 99,
+                accept=accept, window=window)
+        values, kept, st, ex = s.run_forward(x, d, 99, want_bins=True)
+        assert kept.cpu().numpy().tolist() == ref_k.tolist()
+        assert np.array_equal(ex["bins"].cpu().numpy(), ref_b)
+        v = values.cpu().numpy().astype(np.float64)
+        if dtype == torch.float64:
+            assert np.array_equal(v, ref_v)
+        else:
+            assert np.max(np.abs(v - ref_v) / np.abs(ref_v)) <= 1e-5
+        rng2 = np.random.default_rng(5)
+        gv = rng2.standard_normal(len(ref_v))
+        ref_g = oracle_port.batched_backward(x.double().cpu().numpy(), d.double().cpu().numpy(),
+                                             w["node_off"], w["n_nodes"], w["events"], base,
+                                             streams, 99, gv, accept=accept, window=window)
+        gd = torch.empty_like(d)
+        s.backward_raw(x, d, 99, st, torch.from_numpy(gv).to(dtype).cuda(), gd)
+        got = gd.double().cpu().numpy()
+        for i in range(n):
+            sl = slice(int(w["node_off"][i]), int(w["node_off"][i] + w["n_nodes"][i]))
+            scale = np.abs(ref_g[sl]).max()
+            if scale == 0:
+                assert np.abs(got[sl]).max() == 0
+                continue
+            err = np.abs(got[sl] - ref_g[sl]).max() / scale
+            assert err <= (1e-10 if dtype == torch.float64 else 1e-4), (i, err)
+
+
+def test_public_autograd_path(hip_device):
+    from quantom_loits_amd import LOITS
+    from quantom_loits_amd.workloads import reference_1d_inputs, uniform_family
+
+    torch.manual_seed(17)
+    x, d = uniform_family(2, 16, seed=1, dtype=np.float64)
+    outs = [torch.from_numpy(a).cuda() for a in reference_1d_inputs(x, d)]
+    xsec = outs[1].clone().requires_grad_(True)
+    outs[1] = xsec
+    sampler = LOITS(backend="hip", device="cuda")
+    events = sampler(tuple(outs), 50)
+    events.square().mean().backward()
+    assert events.ndim == 2 and events.shape[1] == 2 and events.shape[0] == 100
+    assert xsec.grad is not None and torch.isfinite(xsec.grad).all()
+    assert sampler.impl.sequence == 1
+
+
+def test_batched_autograd_function(hip_device, oracle_port):
+    from quantom_loits_amd.batched import BatchedLOITS, Tuning
+    from quantom_loits_amd.workloads import uniform_family
+
+    B, N, E = 3, 128, 3000
+    x, d = uniform_family(B, N, seed=6, dtype=np.float64)
+    s = BatchedLOITS.uniform(B, N, E, 2, dtype=torch.float64, tuning=Tuning(threads=256,
+                                                                           chunk_events=1024))
+    xt = torch.from_numpy(x.reshape(-1)).cuda()
+    dt = torch.from_numpy(d.reshape(-1)).cuda().requires_grad_(True)
+    vals = s(xt, dt, 7)
+    w = torch.randn_like(vals)
+    (vals * w).sum().backward()
+    desc = uniform_batch(B, N, E, 2)
+    ref = oracle_port.batched_backward(x.reshape(-1), d.reshape(-1), *desc, 7,
+                                       w.detach().cpu().numpy())
+    np.testing.assert_allclose(dt.grad.cpu().numpy(), ref, rtol=1e-9, atol=1e-9 * np.abs(ref).max())
