@@ -162,8 +162,10 @@ int loits_hip_backward(const loits_hip_backward_args* args, void* stream);
  * density at the same offsets; slot s (0 <= s < events) draws the uniform of
  * Philox counter p = counter_base + s on stream `stream` (word p % 4 of block
  * p / 4), exactly as the reference's slot p does (cpp/bindings.cpp:79-92).
- * Kept events of the instance land in capacity rows [out_off, out_off+events)
- * of the capacity-layout output, compacted to the front of that range. */
+ * Capacity row out_off + s belongs to slot s.  Each wave of a work-group owns
+ * a contiguous run of slots (a segment) and writes the segment's kept events,
+ * in slot order, to the front of the segment's capacity rows; when every slot
+ * is kept the capacity layout is already the packed output. */
 typedef struct loits_hip_instance {
   int64_t node_off;
   int64_t n_nodes;      /* >= 2 */
@@ -178,7 +180,7 @@ typedef struct loits_hip_instance {
 typedef struct loits_hip_tuning {
   int32_t threads;          /* work-group size: 256, 512 or 1024 */
   int32_t guide_shift;      /* guide cells = next_pow2(n_nodes) << guide_shift (0..3) */
-  int64_t chunk_events;     /* events per work-group tile (multiple of 4*threads) */
+  int64_t chunk_events;     /* events per work-group (rounded up to 4*threads) */
   int32_t scan_mode;        /* 0 = parallel CDF scan, 1 = the reference's sequential chain */
   int32_t persistent;       /* 1 = persistent grid pulling tiles from a counter */
 } loits_hip_tuning;
@@ -190,8 +192,10 @@ typedef struct loits_hip_layout {
   int64_t scratch_bytes;    /* per-call scratch (counters, partial VJP sums) */
   int64_t lds_forward;      /* dynamic LDS of the sampling kernel */
   int64_t lds_backward;     /* dynamic LDS of the VJP kernel */
+  int64_t lds_tables;       /* dynamic LDS of the tables kernel (threshold sort) */
+  int64_t n_segments;       /* chunks * waves per work-group */
   int64_t mask_words;       /* 64-bit keep-mask words in scratch */
-  int64_t partial_doubles;  /* partial VJP sums in scratch (multi-tile instances) */
+  int64_t partial_doubles;  /* partial VJP sums in scratch (multi-chunk instances) */
   int32_t dtype, threads, scan_mode, persistent;
   int32_t grid_forward, grid_backward;
 } loits_hip_layout;
@@ -229,7 +233,9 @@ typedef struct loits_hip_batched_args {
  * per-instance stable compaction into the capacity layout.  No host sync. */
 int loits_hip_batched_forward(const loits_hip_batched_args* args, void* stream);
 
-/* Gather the capacity layout into a contiguous [total] array (stable order).
+/* Gather the segments of the capacity layout into a contiguous [total] array
+ * (stable order); does nothing when every slot was kept (the capacity layout
+ * is then already packed).
  * `packed` has capacity rows; only the first *total are written. */
 int loits_hip_batched_pack(const loits_hip_batched_args* args, void* packed, int32_t* packed_bins,
                            void* stream);

@@ -3,19 +3,22 @@
 // One x-curve per instance (the reference's 1-D mapping, nx = ny = 1).
 //   tables_kernel   one work-group per instance: trapezoid norm, CDF (parallel
 //                   blocked scan or the reference's sequential chain), integer
-//                   thresholds, guide table and per-bin interpolation data
-//                   (cpp/loits_core.cpp:55-165).
-//   sample_kernel   one work-group per tile of an instance's slots: Philox
+//                   thresholds (sorted when the CDF is not monotone), a packed
+//                   guide table and per-bin interpolation data
+//                   (cpp/loits_core.cpp:55-165, 190-195).
+//   sample_kernel   one work-group per chunk of an instance's slots; each wave
+//                   owns a contiguous run of Philox blocks (a segment): Philox
 //                   uniforms (rng/philox.hpp), interval search over the LDS
-//                   tables (cpp/loits_core.cpp:12-18), inverse interpolation
-//                   (:190-195) and stable compaction (:239-266) with a
-//                   per-tile ordered scan; writes a 1-bit keep mask per slot.
-//   finalize_kernel per-tile kept counts -> row offsets, totals, dense flag.
-//   pack_kernel     gathers tiles into contiguous rows when events were dropped.
+//                   tables (cpp/loits_core.cpp:12-18) with four searches in
+//                   lockstep, inverse interpolation (:190-195) and a stable,
+//                   barrier-free compaction of the segment (:239-266); writes
+//                   a 1-bit keep mask per slot.
+//   finalize_kernel segment kept counts -> packed row offsets, totals, dense flag.
+//   pack_kernel     gathers segments into contiguous rows when events were dropped.
 //   vjp_kernel      recomputes Philox + search, accumulates the interpolation
 //                   adjoint in an LDS fp64 histogram (ds_add_f64), then the CDF
 //                   and density adjoints (cpp/loits_core.cpp:268-472) for the
-//                   instance in the last tile to finish.
+//                   instance in the last chunk to finish.
 // All floating-point code is compiled with -ffp-contract=off; fused
 // multiply-adds are explicit.
 #include <hip/hip_runtime.h>
@@ -67,17 +70,23 @@ __device__ double block_exclusive_suffix(double v, double* scratch) {
   return total - excl - v;
 }
 
-// Norm-sum fusion pattern of the reference build (see oracle/loits_oracle.c):
-// the norm of nterms terms is fully fused when nterms <= 3, otherwise only an
-// odd trailing term is fused.
+// Norm-sum fusion pattern of the reference build (oracle/loits_oracle.c,
+// curve_density): all terms fused when there are at most three, otherwise
+// only an odd trailing term.
 __device__ __forceinline__ bool norm_term_fused(int64_t i, int64_t nterms) {
   return nterms <= 3 || ((nterms & 1) && i == nterms - 1);
+}
+
+__device__ __forceinline__ int ceil_log2_dev(int64_t v) {
+  int l = 0;
+  while ((int64_t(1) << l) < v) ++l;
+  return l;
 }
 
 template <typename Real>
 struct Tables {
   TableHeader* hdr;
-  uint32_t* thr;
+  uint32_t* srt;
   uint32_t* guide;
   typename CoefOf<Real>::type* coef;
   double* cdf;
@@ -92,7 +101,7 @@ __device__ __forceinline__ Tables<Real> tables_of(uint8_t* base, const InstanceD
   t.L = table_layout(in.n_nodes, in.guide_log2, sizeof(Coef));
   uint8_t* blob = base + in.table_off;
   t.hdr = reinterpret_cast<TableHeader*>(blob);
-  t.thr = reinterpret_cast<uint32_t*>(blob + t.L.thr);
+  t.srt = reinterpret_cast<uint32_t*>(blob + t.L.thr);
   t.guide = reinterpret_cast<uint32_t*>(blob + t.L.guide);
   t.coef = reinterpret_cast<Coef*>(blob + t.L.coef);
   t.cdf = reinterpret_cast<double*>(blob + t.L.cdf);
@@ -101,9 +110,11 @@ __device__ __forceinline__ Tables<Real> tables_of(uint8_t* base, const InstanceD
 }
 
 // ---------------------------------------------------------------------------
-// Tables: one work-group per instance.
+// Tables: one work-group per instance.  Dynamic LDS: 4 * next_pow2(max n)
+// bytes, used only to sort the thresholds of a non-monotone CDF.
 template <typename Real, bool SEQUENTIAL>
 __global__ void __launch_bounds__(1024) tables_kernel(BatchedParams P) {
+  extern __shared__ __align__(16) uint32_t sort_lds[];
   __shared__ double scratch[16];
   __shared__ double sh_norm, sh_inv;
   const InstanceDesc in = P.inst[blockIdx.x];
@@ -171,47 +182,135 @@ __global__ void __launch_bounds__(1024) tables_kernel(BatchedParams P) {
   }
   const double nrm = sh_norm;
 
-  // Thresholds, monotonicity, per-bin interpolation data.
-  bool mono = true;
+  // Thresholds, monotonicity, per-bin interpolation data, value-range check.
+  bool mono = true, finite = true, pos = true, neg = true;
   for (int64_t t = tid; t < n; t += nt) {
     const double c = T.cdf[t];
     const uint32_t thr = threshold_of(c);
-    T.thr[t] = thr;
+    T.srt[t] = thr;
     if (t > 0) mono = mono && thr >= threshold_of(T.cdf[t - 1]);
     if (P.cdf_out) P.cdf_out[in.node_off + t] = c;
+    const double xt = double(x[t]);
+    const double ax = __builtin_fabs(xt);
+    finite = finite && __builtin_isfinite(c) && ax >= 1e-30 && ax <= 1e29;
+    pos = pos && xt > 0.0;
+    neg = neg && xt < 0.0;
     if (t + 1 < n) {
       const double c1 = T.cdf[t + 1];
-      const double inv_d = 1.0 / (c1 - c + kEpsilon);       // cpp/loits_core.cpp:193
-      const double m = (double(x[t + 1]) - double(x[t])) * inv_d;  // :194
+      const double inv_d = 1.0 / (c1 - c + kEpsilon);              // cpp/loits_core.cpp:193
+      const double m = (double(x[t + 1]) - xt) * inv_d;            // :194
+      finite = finite && __builtin_isfinite(m);
       if constexpr (sizeof(Real) == 4) {
         const float r = static_cast<float>(double(thr) * 0x1.0p-24 - c);
-        T.coef[t] = CoefF32{thr, r, static_cast<float>(m), static_cast<float>(x[t])};
+        T.coef[t] = CoefF32{thr, r, static_cast<float>(m), static_cast<float>(xt)};
         T.rlo[t] = r;
       } else {
-        T.coef[t] = CoefF64{c, m, double(x[t]), thr, 0u};
+        T.coef[t] = CoefF64{c, m, xt, thr, 0u};
       }
     }
   }
   mono = __syncthreads_and(mono);
+  finite = __syncthreads_and(finite);
+  pos = __syncthreads_and(pos);
+  neg = __syncthreads_and(neg);
+  const bool safe = finite && (pos || neg);
 
-  // Guide: guide[g] = #{t : T_t < g << shift} (monotone thresholds only).
+  if (!mono) {
+    // count(k) = #{t : T_t <= k} needs the thresholds in sorted order; bins
+    // keep their own T_j in coef.  Bitonic sort in LDS.
+    int64_t pw = 1;
+    while (pw < n) pw <<= 1;
+    for (int64_t i = tid; i < pw; i += nt) sort_lds[i] = i < n ? T.srt[i] : kNever;
+    __syncthreads();
+    for (int64_t k = 2; k <= pw; k <<= 1) {
+      for (int64_t j = k >> 1; j > 0; j >>= 1) {
+        for (int64_t i = tid; i < pw; i += nt) {
+          const int64_t ixj = i ^ j;
+          if (ixj > i) {
+            const uint32_t a = sort_lds[i], b = sort_lds[ixj];
+            const bool up = (i & k) == 0;
+            if ((a > b) == up) {
+              sort_lds[i] = b;
+              sort_lds[ixj] = a;
+            }
+          }
+        }
+        __syncthreads();
+      }
+    }
+    for (int64_t i = tid; i < n; i += nt) T.srt[i] = sort_lds[i];
+    __syncthreads();
+  }
+
+  // Level-1 guide by scatter: srt position t is the lower bound of every cell
+  // start key g << shift with srt_{t-1} < key <= srt_t (srt_{-1} = -inf,
+  // srt_n = +inf).  Each cell word is written as two 16-bit halves.
   const int64_t G = int64_t(1) << in.guide_log2;
   const int shift = 24 - in.guide_log2;
-  if (mono) {
-    for (int64_t g = tid; g <= G; g += nt) {
-      const uint64_t key = uint64_t(g) << shift;
-      int64_t lo = 0, hi = n;
-      while (lo < hi) {
-        const int64_t mid = (lo + hi) >> 1;
-        if (uint64_t(T.thr[mid]) < key) lo = mid + 1; else hi = mid;
-      }
-      T.guide[g] = static_cast<uint32_t>(lo);
+  uint16_t* g16 = reinterpret_cast<uint16_t*>(T.guide);
+  for (int64_t t = tid; t <= n; t += nt) {
+    const int64_t ga = t == 0 ? 0 : int64_t(T.srt[t - 1] >> shift) + 1;
+    int64_t gb = t == n ? G : int64_t(T.srt[t] >> shift);
+    gb = gb > G ? G : gb;
+    for (int64_t g = ga; g <= gb; ++g) {
+      if (g < G) g16[2 * g] = static_cast<uint16_t>(t);      // lo of cell g
+      if (g >= 1) g16[2 * g - 1] = static_cast<uint16_t>(t);  // hi of cell g - 1
     }
   }
+  // sorted thresholds in LDS for the level-2 searches (the sort already left
+  // them there when the CDF was not monotone)
+  if (mono)
+    for (int64_t i = tid; i < n; i += nt) sort_lds[i] = T.srt[i];
+  __syncthreads();
+
+  // Level 2: crowded cells (hi - lo >= 2), in cell order, get kSub sub-cells
+  // each while the 2 G words of level 2 last.
+  const int64_t cap_cells = (2 * G) >> kSubLog2;
+  int32_t* crowd_list = reinterpret_cast<int32_t*>(sort_lds + pad4(int64_t(1) << ceil_log2_dev(n)));
+  double carry = 0.0;
+  for (int64_t base = 0; base < G; base += nt) {
+    const int64_t g = base + tid;
+    bool crowd = false;
+    if (g < G) {
+      const uint32_t w = T.guide[g];
+      crowd = int(w >> 16) - int(w & 0xFFFFu) >= 2;
+    }
+    double tot;
+    const int64_t rank = int64_t(carry + block_exclusive_scan(crowd ? 1.0 : 0.0, scratch, &tot));
+    if (crowd && rank < cap_cells) crowd_list[rank] = int32_t(g);
+    carry += tot;
+  }
+  __syncthreads();
+  const int64_t n_assigned = int64_t(carry) < cap_cells ? int64_t(carry) : cap_cells;
+  const int sub_shift = shift - kSubLog2;
+  for (int64_t item = tid; item < n_assigned * kSub; item += nt) {
+    const int64_t r = item >> kSubLog2, sub = item & (kSub - 1);
+    const int64_t g = crowd_list[r];
+    const uint32_t w = T.guide[g];
+    const int lo0 = int(w & 0xFFFFu), hi0 = int(w >> 16);
+    const uint64_t key_lo = (uint64_t(g) << shift) + (uint64_t(sub) << sub_shift);
+    const uint64_t key_hi = key_lo + (uint64_t(1) << sub_shift);
+    int a = lo0, b = hi0;  // first position with srt >= key_lo
+    while (a < b) {
+      const int mid = (a + b) >> 1;
+      if (uint64_t(sort_lds[mid]) < key_lo) a = mid + 1; else b = mid;
+    }
+    int c2 = a, d2 = hi0;  // first position with srt >= key_hi
+    while (c2 < d2) {
+      const int mid = (c2 + d2) >> 1;
+      if (uint64_t(sort_lds[mid]) < key_hi) c2 = mid + 1; else d2 = mid;
+    }
+    T.guide[G + r * kSub + sub] = uint32_t(a) | (uint32_t(c2) << 16);
+  }
+  __syncthreads();  // every level-1 word of an assigned cell has been read
+  for (int64_t r = tid; r < n_assigned; r += nt)
+    T.guide[crowd_list[r]] = kIndirect | uint32_t(G + r * kSub);
+
   if (tid == 0) {
     T.hdr->norm = nrm;
     T.hdr->inv_n = sh_inv;
-    T.hdr->flags = mono ? kMonotone : 0u;
+    const bool last_full = T.srt[n - 1] >= 16777214u;
+    T.hdr->flags = (mono ? kMonotone : 0u) | ((mono && safe && acc == 1.0 && last_full) ? kSafe : 0u);
     T.hdr->n = static_cast<uint32_t>(n);
     T.hdr->guide_log2 = static_cast<uint32_t>(in.guide_log2);
     if (P.norm_out) P.norm_out[blockIdx.x] = nrm;
@@ -219,76 +318,126 @@ __global__ void __launch_bounds__(1024) tables_kernel(BatchedParams P) {
 }
 
 // ---------------------------------------------------------------------------
-// Interval search: count(k) = #{t : T_t <= k}; j = clamp(count - 1, 0, n - 2).
-__device__ __forceinline__ int find_bin(uint32_t k, const uint32_t* thr, const uint32_t* guide,
-                                        int shift, int n, bool mono) {
-  int count;
-  if (mono) {
-    const uint32_t g = k >> shift;
-    int lo = static_cast<int>(guide[g]), hi = static_cast<int>(guide[g + 1]);
-    while (lo < hi) {
-      const int mid = (lo + hi) >> 1;
-      if (thr[mid] <= k) lo = mid + 1; else hi = mid;
-    }
-    count = lo;
-  } else {
-    count = 0;
-    for (int t = 0; t < n; ++t) count += thr[t] <= k ? 1 : 0;
-  }
-  int j = count - 1;
-  j = j < 0 ? 0 : j;
-  return j > n - 2 ? n - 2 : j;
+// Four interval searches: count(k) = #{t : srt_t <= k}, j = clamp(count - 1,
+// 0, n - 2).  The two-level guide narrows each search to [lo, lo + len); one
+// bisection step (always taken) settles len <= 1, and the loop after it runs
+// only for the rare cells that still hold several thresholds.
+__device__ __forceinline__ void bisect_step(uint32_t k, const uint32_t* srt, int n, int& lo,
+                                            int& len) {
+  const int half = len >> 1;
+  int idx = lo + half;
+  idx = idx < n ? idx : n - 1;
+  const bool le = srt[idx] <= k;
+  const bool act = len > 0;
+  lo = (act && le) ? idx + 1 : lo;
+  len = act ? (le ? len - half - 1 : half) : 0;
 }
 
+__device__ __forceinline__ void find_bins4(const uint32_t (&k)[4], const uint32_t* srt,
+                                           const uint32_t* guide, int shift, int n,
+                                           int (&j)[4]) {
+  int lo[4], len[4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    uint32_t w = guide[k[e] >> shift];
+    if (w & kIndirect)
+      w = guide[(w & ~kIndirect) + ((k[e] >> (shift - kSubLog2)) & (kSub - 1))];
+    lo[e] = int(w & 0xFFFFu);
+    len[e] = int((w >> 16) & 0x7FFFu) - lo[e];
+  }
+#pragma unroll
+  for (int e = 0; e < 4; ++e) bisect_step(k[e], srt, n, lo[e], len[e]);
+  while ((len[0] | len[1] | len[2] | len[3]) != 0) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) bisect_step(k[e], srt, n, lo[e], len[e]);
+  }
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    int jj = lo[e] - 1;
+    jj = jj < 0 ? 0 : jj;
+    j[e] = jj > n - 2 ? n - 2 : jj;
+  }
+}
+
+template <bool WINDOW>
 __device__ __forceinline__ bool keep_value(double v, const BatchedParams& P) {
   if (!(__builtin_isfinite(v) && v != 0.0)) return false;
-  if (P.use_window && !(v >= P.win_lo && v <= P.win_hi)) return false;
+  if (WINDOW && !(v >= P.win_lo && v <= P.win_hi)) return false;
   return true;
 }
 
 // The reference's fp64 interpolation for bin j (cpp/loits_core.cpp:190-195).
 template <typename Real>
-__device__ __forceinline__ double interp_reference(const double* cdf, const Real* x, int j,
-                                                   uint32_t k) {
+__device__ __noinline__ double interp_reference(const double* cdf, const Real* x, int j,
+                                                uint32_t k) {
   const double c0 = cdf[j];
   const double inv_d = 1.0 / (cdf[j + 1] - c0 + kEpsilon);
   const double m = (double(x[j + 1]) - double(x[j])) * inv_d;
   return __builtin_fma(m, double(k) * 0x1.0p-24 - c0, double(x[j]));
 }
 
-// Value and keep flag of one slot.  fp32: single-precision evaluation of the
-// same formula; results whose keep decision could differ from the fp64
-// decision (near 0, near overflow, within 1e-6 of a window edge, non-finite)
-// are re-evaluated in fp64 from the global tables.
-template <typename Real>
-__device__ __forceinline__ Real slot_value(uint32_t k, int j,
+// Values and keep flags of a lane's four slots.  fp64: the reference's
+// operands and operation order.  fp32: single-precision evaluation of the same
+// formula, u - C_j = (k - T_j) 2^-24 + r_j; a value whose keep decision could
+// differ from the fp64 decision (|v| < 1e-30 or > 1e30, non-finite, or within
+// 2e-6 |v| of a window edge) is re-evaluated in fp64 from the global tables.
+template <typename Real, bool WINDOW>
+__device__ __forceinline__ int slot_values(const uint32_t (&k)[4], const int (&j)[4],
                                            const typename CoefOf<Real>::type* coef,
                                            const double* cdf_g, const Real* x_g,
-                                           const BatchedParams& P, bool* keep) {
+                                           const BatchedParams& P, bool safe_curve,
+                                           Real (&v)[4]) {
+  using Coef = typename CoefOf<Real>::type;
+  Coef cf[4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) cf[e] = coef[j[e]];
+  int keep = 0;
   if constexpr (sizeof(Real) == 4) {
-    const CoefF32 cf = coef[j];
-    const float t = static_cast<float>(static_cast<int32_t>(k - cf.thr)) * 0x1.0p-24f + cf.r;
-    const float v = __builtin_fmaf(cf.m, t, cf.x);
-    const float a = __builtin_fabsf(v);
-    bool safe = a >= 1e-30f && a <= 1e30f;
-    if (P.use_window) {
-      const float lo = static_cast<float>(P.win_lo), hi = static_cast<float>(P.win_hi);
-      const float margin = 2e-6f * a;
-      safe = safe && __builtin_fabsf(v - lo) > margin && __builtin_fabsf(v - hi) > margin;
+    if (!WINDOW && safe_curve) {  // every value of this curve is kept (kSafe)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const float t =
+            static_cast<float>(static_cast<int32_t>(k[e] - cf[e].thr)) * 0x1.0p-24f + cf[e].r;
+        v[e] = __builtin_fmaf(cf[e].m, t, cf[e].x);
+      }
+      return 0xF;
     }
-    if (__builtin_expect(safe, 1)) {
-      *keep = !P.use_window || (v >= static_cast<float>(P.win_lo) && v <= static_cast<float>(P.win_hi));
-      return v;
+    const float lo = static_cast<float>(P.win_lo), hi = static_cast<float>(P.win_hi);
+    int unsafe = 0;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const float t =
+          static_cast<float>(static_cast<int32_t>(k[e] - cf[e].thr)) * 0x1.0p-24f + cf[e].r;
+      v[e] = __builtin_fmaf(cf[e].m, t, cf[e].x);
+      const float a = __builtin_fabsf(v[e]);
+      bool safe = a >= 1e-30f && a <= 1e30f;
+      bool in = true;
+      if (WINDOW) {
+        const float margin = 2e-6f * a;
+        safe = safe && __builtin_fabsf(v[e] - lo) > margin && __builtin_fabsf(v[e] - hi) > margin;
+        in = v[e] >= lo && v[e] <= hi;
+      }
+      keep |= (safe && in) ? (1 << e) : 0;
+      unsafe |= safe ? 0 : (1 << e);
     }
-    const double v64 = interp_reference(cdf_g, x_g, j, k);
-    *keep = keep_value(v64, P);
-    return static_cast<float>(v64);
+    if (__builtin_expect(unsafe != 0, 0)) {
+#pragma unroll 1
+      for (int e = 0; e < 4; ++e) {
+        if ((unsafe >> e) & 1) {
+          const double v64 = interp_reference(cdf_g, x_g, j[e], k[e]);
+          v[e] = static_cast<float>(v64);
+          keep |= keep_value<WINDOW>(v64, P) ? (1 << e) : 0;
+        }
+      }
+    }
   } else {
-    const CoefF64 cf = coef[j];
-    const double v = __builtin_fma(cf.m, double(k) * 0x1.0p-24 - cf.c, cf.x);
-    *keep = keep_value(v, P);
-    return v;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      v[e] = __builtin_fma(cf[e].m, double(k[e]) * 0x1.0p-24 - cf[e].c, cf[e].x);
+      keep |= keep_value<WINDOW>(v[e], P) ? (1 << e) : 0;
+    }
   }
+  return keep;
 }
 
 __device__ __forceinline__ void lds_copy(void* dst, const void* src, int64_t bytes) {
@@ -297,35 +446,31 @@ __device__ __forceinline__ void lds_copy(void* dst, const void* src, int64_t byt
   for (int64_t i = threadIdx.x; i < bytes / 16; i += blockDim.x) d[i] = s[i];
 }
 
-// Ordered work-group scan of per-thread counts (0..4): returns this thread's
-// exclusive offset and the work-group total.  One barrier; the per-wave totals
-// are double-buffered by `parity`.
-__device__ __forceinline__ int wg_scan_counts(int cnt, int32_t (*wave_tot)[16], int parity,
-                                              int* total) {
+// Exclusive prefix of per-lane counts (0..4) within a wave, and the wave total.
+__device__ __forceinline__ int wave_scan_counts(int cnt, int* total) {
   const uint64_t lt = lanemask_lt();
   const uint64_t b0 = ballot(cnt & 1), b1 = ballot(cnt & 2), b2 = ballot(cnt & 4);
-  const int pre = popc64(b0 & lt) + 2 * popc64(b1 & lt) + 4 * popc64(b2 & lt);
-  const int wt = popc64(b0) + 2 * popc64(b1) + 4 * popc64(b2);
-  const int wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
-  if (lane_id() == 0) wave_tot[parity][wave] = wt;
-  __syncthreads();
-  int base = 0, all = 0;
-  for (int w = 0; w < nwaves; ++w) {
-    const int v = wave_tot[parity][w];
-    all += v;
-    base += w < wave ? v : 0;
-  }
-  *total = all;
-  return base + pre;
+  *total = popc64(b0) + 2 * popc64(b1) + 4 * popc64(b2);
+  return popc64(b0 & lt) + 2 * popc64(b1 & lt) + 4 * popc64(b2 & lt);
+}
+
+// Keep-mask words of a chunk: wave w's iteration `it` owns words
+// [(w * per_wave + it) * 4, +4), one 64-bit ballot per slot position e.
+__device__ __forceinline__ int64_t chunk_blocks(const InstanceDesc& in, const ChunkDesc& ch) {
+  if (ch.slot_hi <= ch.slot_lo) return 0;
+  return int64_t(((in.counter_base + uint64_t(ch.slot_hi) + 3) >> 2) -
+                 ((in.counter_base + uint64_t(ch.slot_lo)) >> 2));
 }
 
 // ---------------------------------------------------------------------------
-// Sampling: one work-group per tile (chunk).
-template <typename Real>
+// Sampling: one work-group per chunk, one segment per wave.  While every slot
+// of a segment so far was kept, rows are positional and no keep mask is
+// written; the first iteration with a dropped slot is recorded, and masks are
+// written from that iteration on.
+template <typename Real, bool WINDOW>
 __global__ void __launch_bounds__(1024) sample_kernel(BatchedParams P) {
   using Coef = typename CoefOf<Real>::type;
   extern __shared__ __align__(16) uint8_t lds[];
-  __shared__ int32_t wave_tot[2][16];
   const int c = blockIdx.x;
   const ChunkDesc ch = P.chunks[c];
   const InstanceDesc in = P.inst[ch.instance];
@@ -333,78 +478,96 @@ __global__ void __launch_bounds__(1024) sample_kernel(BatchedParams P) {
   const int n = static_cast<int>(in.n_nodes);
   const int shift = 24 - in.guide_log2;
 
-  lds_copy(lds, T.thr, T.L.cdf - T.L.thr);
+  lds_copy(lds, T.srt, T.L.cdf - T.L.thr);
   __syncthreads();
-  const uint32_t* s_thr = reinterpret_cast<const uint32_t*>(lds);
+  const uint32_t* s_srt = reinterpret_cast<const uint32_t*>(lds);
   const uint32_t* s_guide = reinterpret_cast<const uint32_t*>(lds + (T.L.guide - T.L.thr));
   const Coef* s_coef = reinterpret_cast<const Coef*>(lds + (T.L.coef - T.L.thr));
-  const bool mono = (T.hdr->flags & kMonotone) != 0;
   const Real* x_g = static_cast<const Real*>(P.x) + in.node_off;
 
-  const uint64_t p_lo = in.counter_base + uint64_t(ch.slot_lo);
-  const uint64_t p_hi = in.counter_base + uint64_t(ch.slot_hi);
-  const uint64_t q_begin = p_lo >> 2;
-  const uint64_t q_end = ch.slot_hi > ch.slot_lo ? (p_hi + 3) >> 2 : q_begin;  // empty tile: no work
-  Real* out = static_cast<Real*>(P.events) + in.out_off + ch.slot_lo;
-  int32_t* bins_out = P.bins ? P.bins + in.out_off + ch.slot_lo : nullptr;
-  uint64_t* mask = P.keepmask + ch.mask_off;
-  const int wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+  const int nw = blockDim.x >> 6, w = threadIdx.x >> 6, lane = lane_id();
+  const Segment sg = segment_of(in.counter_base, ch.slot_lo, ch.slot_hi, w, nw);
+  const int64_t seg = int64_t(c) * nw + w;
+  const int64_t base_row = in.out_off + int64_t(sg.p_lo - in.counter_base);
+  Real* out = static_cast<Real*>(P.events) + base_row;
+  int32_t* bins_out = P.bins ? P.bins + base_row : nullptr;
+  const int64_t per_wave = int64_t(blocks_per_wave(uint64_t(chunk_blocks(in, ch)), nw) / 64);
+  uint64_t* mask = P.keepmask + ch.mask_off + int64_t(w) * per_wave * 4;
+  const bool safe_curve = (T.hdr->flags & kSafe) != 0;
+  // 32-bit slot positions relative to counter 4 qa: live slots are [off0, span)
+  const uint32_t nblk = uint32_t(sg.qb - sg.qa);
+  const uint32_t off0 = uint32_t(sg.p_lo - 4 * sg.qa);
+  const uint32_t span = uint32_t(sg.p_hi - 4 * sg.qa);
   int64_t running = 0;
-  int parity = 0, it = 0;
-  for (uint64_t q0 = q_begin; q0 < q_end; q0 += blockDim.x, ++it) {
-    const uint64_t q = q0 + threadIdx.x;
-    const Words4 w = philox4x32_10(q, in.stream, P.seed);
+  int first_drop = int(sg.iters);  // iterations before this one kept every slot
+  int it = 0;
+  for (uint32_t i0 = 0; i0 < nblk; i0 += 64, ++it) {
+    const uint32_t i = i0 + uint32_t(lane);
+    const Words4 wd = philox4x32_10_uniform(sg.qa + i, in.stream, P.seed);
+    const uint32_t k[4] = {wd.w0 >> 8, wd.w1 >> 8, wd.w2 >> 8, wd.w3 >> 8};
+    int j[4];
+    find_bins4(k, s_srt, s_guide, shift, n, j);
     Real v[4];
-    int jb[4];
-    int keepbits = 0;
+    const int keep = slot_values<Real, WINDOW>(k, j, s_coef, T.cdf, x_g, P, safe_curve, v);
+    const bool full_iter = (i0 > 0 || off0 == 0) && 4 * (i0 + 64) <= span;
+    int live = 0xF;
+    if (!full_iter) {
+      live = 0;
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const uint64_t p = 4 * q + e;
-      const bool live = q < q_end && p >= p_lo && p < p_hi;
-      const uint32_t k = word_of(w, e) >> 8;
-      const int j = find_bin(k, s_thr, s_guide, shift, n, mono);
-      bool keep;
-      v[e] = slot_value<Real>(k, j, s_coef, T.cdf, x_g, P, &keep);
-      jb[e] = j;
-      keepbits |= (live && keep) ? (1 << e) : 0;
+      for (int e = 0; e < 4; ++e) {
+        const uint32_t r = 4 * i + e;
+        live |= (i < nblk && r >= off0 && r < span) ? (1 << e) : 0;
+      }
     }
-    // keep mask: 4 ballots per wave per iteration, one bit per slot
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const uint64_t b = ballot((keepbits >> e) & 1);
-      if (lane_id() == e) mask[(int64_t(it) * nwaves + wave) * 4 + e] = b;
-    }
+    const int keepbits = keep & live;
+    int64_t row;
     int total;
-    const int off = wg_scan_counts(__popc(keepbits), wave_tot, parity, &total);
-    parity ^= 1;
-    int64_t row = running + off;
-    if (keepbits == 0xF && ((row & 3) == 0) && ((in.out_off + ch.slot_lo) & 3) == 0) {
+    if (first_drop == int(sg.iters) && ballot(keepbits != live) == 0ull) {
+      // every slot so far kept: slot 4i + e is row 4i + e - off0
+      const int32_t r0 = int32_t(4 * i) - int32_t(off0);
+      row = r0 < 0 ? 0 : r0;
+      const uint32_t a = 4 * i0 > off0 ? 4 * i0 : off0;
+      const uint32_t b = 4 * (i0 + 64) < span ? 4 * (i0 + 64) : span;
+      total = b > a ? int(b - a) : 0;
+    } else {
+      if (first_drop == int(sg.iters)) first_drop = it;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const uint64_t b = ballot((keepbits >> e) & 1);
+        if (lane == e) mask[int64_t(it) * 4 + e] = b;
+      }
+      row = running + wave_scan_counts(__popc(keepbits), &total);
+    }
+    if (keepbits == 0xF && ((base_row + row) & 3) == 0) {
       if constexpr (sizeof(Real) == 4) {
         *reinterpret_cast<float4*>(out + row) = make_float4(v[0], v[1], v[2], v[3]);
       } else {
         reinterpret_cast<double2*>(out + row)[0] = make_double2(v[0], v[1]);
         reinterpret_cast<double2*>(out + row)[1] = make_double2(v[2], v[3]);
       }
-      if (bins_out)
-        *reinterpret_cast<int4*>(bins_out + row) = make_int4(jb[0], jb[1], jb[2], jb[3]);
+      if (bins_out) *reinterpret_cast<int4*>(bins_out + row) = make_int4(j[0], j[1], j[2], j[3]);
     } else {
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         if ((keepbits >> e) & 1) {
           out[row] = v[e];
-          if (bins_out) bins_out[row] = jb[e];
+          if (bins_out) bins_out[row] = j[e];
           ++row;
         }
       }
     }
     running += total;
   }
-  if (threadIdx.x == 0) P.chunk_kept[c] = running;
+  if (lane == 0) {
+    P.seg_kept[seg] = running;
+    P.seg_cap[seg] = int64_t(sg.p_hi - sg.p_lo);
+    P.seg_drop[seg] = first_drop;
+  }
 }
 
 // ---------------------------------------------------------------------------
-// Finalize: one work-group.  chunk_row = exclusive scan of chunk_kept; kept
-// per instance; total; dense flag (every tile kept all of its slots).
+// Finalize: one work-group.  seg_row = exclusive scan of seg_kept; kept per
+// instance; total; dense flag (every segment kept all of its slots).
 __global__ void __launch_bounds__(1024) finalize_kernel(BatchedParams P) {
   __shared__ double scratch[16];
   __shared__ int sh_sparse;
@@ -412,17 +575,16 @@ __global__ void __launch_bounds__(1024) finalize_kernel(BatchedParams P) {
   __syncthreads();
   double carry = 0.0;  // counts < 2^53: exact in double
   bool sparse = false;
-  for (int64_t base = 0; base < P.n_chunks; base += blockDim.x) {
-    const int64_t c = base + threadIdx.x;
+  for (int64_t base = 0; base < P.n_segments; base += blockDim.x) {
+    const int64_t sgi = base + threadIdx.x;
     double v = 0.0;
-    if (c < P.n_chunks) {
-      const ChunkDesc ch = P.chunks[c];
-      v = double(P.chunk_kept[c]);
-      sparse = sparse || P.chunk_kept[c] != ch.slot_hi - ch.slot_lo;
+    if (sgi < P.n_segments) {
+      v = double(P.seg_kept[sgi]);
+      sparse = sparse || P.seg_kept[sgi] != P.seg_cap[sgi];
     }
     double tot;
     const double ex = block_exclusive_scan(v, scratch, &tot);
-    if (c < P.n_chunks) P.chunk_row[c] = static_cast<int64_t>(carry + ex);
+    if (sgi < P.n_segments) P.seg_row[sgi] = static_cast<int64_t>(carry + ex);
     carry += tot;
   }
   if (sparse) atomicOr(&sh_sparse, 1);
@@ -430,7 +592,8 @@ __global__ void __launch_bounds__(1024) finalize_kernel(BatchedParams P) {
   for (int64_t i = threadIdx.x; i < P.n_instances; i += blockDim.x) {
     const InstanceDesc in = P.inst[i];
     int64_t k = 0;
-    for (int c = in.first_chunk; c < in.first_chunk + in.n_chunks; ++c) k += P.chunk_kept[c];
+    const int64_t s0 = int64_t(in.first_chunk) * P.waves, s1 = s0 + int64_t(in.n_chunks) * P.waves;
+    for (int64_t s = s0; s < s1; ++s) k += P.seg_kept[s];
     P.kept[i] = k;
   }
   if (threadIdx.x == 0) {
@@ -439,17 +602,20 @@ __global__ void __launch_bounds__(1024) finalize_kernel(BatchedParams P) {
   }
 }
 
-// Pack: one work-group per tile; a no-op when every tile is full.
+// Pack: one work-group per chunk, one segment per wave; a no-op when dense.
 template <typename Real>
-__global__ void __launch_bounds__(256) pack_kernel(BatchedParams P, Real* packed,
-                                                   int32_t* packed_bins) {
+__global__ void __launch_bounds__(1024) pack_kernel(BatchedParams P, Real* packed,
+                                                    int32_t* packed_bins) {
   if (P.flags[0]) return;
   const ChunkDesc ch = P.chunks[blockIdx.x];
   const InstanceDesc in = P.inst[ch.instance];
-  const int64_t cnt = P.chunk_kept[blockIdx.x];
-  const int64_t src = in.out_off + ch.slot_lo, dst = P.chunk_row[blockIdx.x];
+  const int nw = blockDim.x >> 6, w = threadIdx.x >> 6;
+  const Segment sg = segment_of(in.counter_base, ch.slot_lo, ch.slot_hi, w, nw);
+  const int64_t seg = int64_t(blockIdx.x) * nw + w;
+  const int64_t cnt = P.seg_kept[seg];
+  const int64_t src = in.out_off + int64_t(sg.p_lo - in.counter_base), dst = P.seg_row[seg];
   const Real* ev = static_cast<const Real*>(P.events);
-  for (int64_t r = threadIdx.x; r < cnt; r += blockDim.x) {
+  for (int64_t r = lane_id(); r < cnt; r += 64) {
     packed[dst + r] = ev[src + r];
     if (packed_bins && P.bins) packed_bins[dst + r] = P.bins[src + r];
   }
@@ -492,15 +658,13 @@ __device__ void vjp_tail(const BatchedParams& P, const InstanceDesc& in, const T
     }
     gc[k] = g;
   }
-  // local suffix sums, then the work-group suffix of the segment totals
   double seg = 0.0;
 #pragma unroll
   for (int k = MAXPER - 1; k >= 0; --k) {
     seg += gc[k];
-    gc[k] = seg;  // gc[k] now holds sum_{k' >= k} within the segment
+    gc[k] = seg;  // sum over k' >= k within the segment
   }
   const double after = block_exclusive_suffix(seg, scratch);
-  // S_t = gc[k] + after; grho_t = acc * (h_t S_t [t>=1] + h_{t+1} S_{t+1} [t<=n-2])
   double grho[MAXPER];
   double local_norm = 0.0;
 #pragma unroll
@@ -530,12 +694,12 @@ __device__ void vjp_tail(const BatchedParams& P, const InstanceDesc& in, const T
   }
 }
 
-// VJP: one work-group per tile; the tile that completes an instance runs the tail.
+// VJP: one work-group per chunk, one segment per wave; the chunk that
+// completes an instance runs the tail.
 template <typename Real, int MAXPER>
 __global__ void __launch_bounds__(1024) vjp_kernel(BatchedParams P, const Real* grad,
                                                    Real* grad_density) {
   extern __shared__ __align__(16) uint8_t lds[];
-  __shared__ int32_t wave_tot[2][16];
   __shared__ double scratch[16];
   __shared__ int sh_last;
   const int c = blockIdx.x;
@@ -545,10 +709,10 @@ __global__ void __launch_bounds__(1024) vjp_kernel(BatchedParams P, const Real* 
   const int n = static_cast<int>(in.n_nodes);
   const int shift = 24 - in.guide_log2;
 
-  // LDS: thr | guide | (r: fp32, C: fp64) | h0 | h1
+  // LDS: srt | guide | (r: fp32, C: fp64) | h0 | h1
   const int64_t search_bytes = T.L.coef - T.L.thr;
   const int64_t nb = pad4(n);
-  lds_copy(lds, T.thr, search_bytes);
+  lds_copy(lds, T.srt, search_bytes);
   uint8_t* lds_t = lds + search_bytes;
   if constexpr (sizeof(Real) == 4) lds_copy(lds_t, T.rlo, nb * 4);
   else lds_copy(lds_t, T.cdf, nb * 8);
@@ -556,66 +720,92 @@ __global__ void __launch_bounds__(1024) vjp_kernel(BatchedParams P, const Real* 
   double* h1 = h0 + nb;
   for (int64_t i = threadIdx.x; i < 2 * nb; i += blockDim.x) h0[i] = 0.0;
   __syncthreads();
-  const uint32_t* s_thr = reinterpret_cast<const uint32_t*>(lds);
+  const uint32_t* s_srt = reinterpret_cast<const uint32_t*>(lds);
   const uint32_t* s_guide = reinterpret_cast<const uint32_t*>(lds + (T.L.guide - T.L.thr));
   const bool mono = (T.hdr->flags & kMonotone) != 0;
 
-  const uint64_t p_lo = in.counter_base + uint64_t(ch.slot_lo);
-  const uint64_t p_hi = in.counter_base + uint64_t(ch.slot_hi);
-  const uint64_t q_begin = p_lo >> 2;
-  const uint64_t q_end = ch.slot_hi > ch.slot_lo ? (p_hi + 3) >> 2 : q_begin;  // empty tile: no work
-  const bool full = P.chunk_kept[c] == ch.slot_hi - ch.slot_lo;
-  const int64_t row0 = P.chunk_row[c];
-  const uint64_t* mask = P.keepmask + ch.mask_off;
-  const int wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
-  int64_t running = 0;
-  int parity = 0, it = 0;
-  for (uint64_t q0 = q_begin; q0 < q_end; q0 += blockDim.x, ++it) {
-    const uint64_t q = q0 + threadIdx.x;
-    const Words4 w = philox4x32_10(q, in.stream, P.seed);
-    int keepbits = 0;
-    if (full) {
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const uint64_t p = 4 * q + e;
-        keepbits |= (q < q_end && p >= p_lo && p < p_hi) ? (1 << e) : 0;
-      }
-    } else {
-      const uint64_t* mw = mask + (int64_t(it) * nwaves + wave) * 4;
-#pragma unroll
-      for (int e = 0; e < 4; ++e) keepbits |= int((mw[e] >> lane_id()) & 1ull) << e;
-    }
+  const int nw = blockDim.x >> 6, w = threadIdx.x >> 6, lane = lane_id();
+  const Segment sg = segment_of(in.counter_base, ch.slot_lo, ch.slot_hi, w, nw);
+  const int64_t seg = int64_t(c) * nw + w;
+  const int first_drop = P.seg_drop[seg];
+  const int64_t row0 = P.seg_row[seg];
+  const int64_t per_wave = int64_t(blocks_per_wave(uint64_t(chunk_blocks(in, ch)), nw) / 64);
+  const uint64_t* mask = P.keepmask + ch.mask_off + int64_t(w) * per_wave * 4;
+  const float* s_r = reinterpret_cast<const float*>(lds_t);
+  const double* s_c = reinterpret_cast<const double*>(lds_t);
+  // kept slots before the first masked iteration: every live slot before it
+  const uint32_t nblk = uint32_t(sg.qb - sg.qa);
+  const uint32_t off0 = uint32_t(sg.p_lo - 4 * sg.qa);
+  const uint32_t span = uint32_t(sg.p_hi - 4 * sg.qa);
+  // kept slots before the first masked iteration: every live slot before it
+  int64_t running = int64_t(4 * 64 * int64_t(first_drop)) - int64_t(off0);
+  running = running < 0 ? 0 : running;
+  int it = 0;
+  for (uint32_t i0 = 0; i0 < nblk; i0 += 64, ++it) {
+    const uint32_t i = i0 + uint32_t(lane);
+    const Words4 wd = philox4x32_10_uniform(sg.qa + i, in.stream, P.seed);
+    const uint32_t k[4] = {wd.w0 >> 8, wd.w1 >> 8, wd.w2 >> 8, wd.w3 >> 8};
+    const bool full_iter = (i0 > 0 || off0 == 0) && 4 * (i0 + 64) <= span;
+    int keepbits = 0xF;
     int64_t row;
-    if (full) {
-      row = row0 + int64_t(4 * q) - int64_t(p_lo);  // first slot of this block
+    if (it < first_drop) {  // every live slot of these iterations was kept
+      if (!full_iter) {
+        keepbits = 0;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const uint32_t r = 4 * i + e;
+          keepbits |= (i < nblk && r >= off0 && r < span) ? (1 << e) : 0;
+        }
+      }
+      const int32_t r0 = int32_t(4 * i) - int32_t(off0);
+      row = row0 + (r0 < 0 ? 0 : r0);
     } else {
+      const uint64_t* mw = mask + int64_t(it) * 4;
+      keepbits = 0;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) keepbits |= int((mw[e] >> lane) & 1ull) << e;
       int total;
-      const int off = wg_scan_counts(__popc(keepbits), wave_tot, parity, &total);
-      parity ^= 1;
-      row = row0 + running + off;
+      row = row0 + running + wave_scan_counts(__popc(keepbits), &total);
       running += total;
     }
+    int j[4];
+    find_bins4(k, s_srt, s_guide, shift, n, j);
+    double t[4], g[4];
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-      if (full) {
-        if (!((keepbits >> e) & 1)) continue;
-      } else if (!((keepbits >> e) & 1)) {
-        continue;
-      }
-      const uint32_t k = word_of(w, e) >> 8;
-      const int j = find_bin(k, s_thr, s_guide, shift, n, mono);
-      double t;
       if constexpr (sizeof(Real) == 4) {
-        const float* s_r = reinterpret_cast<const float*>(lds_t);
-        t = double(static_cast<float>(static_cast<int32_t>(k - s_thr[j])) * 0x1.0p-24f + s_r[j]);
+        const uint32_t tj = mono ? s_srt[j[e]] : T.coef[j[e]].thr;
+        t[e] = double(static_cast<float>(static_cast<int32_t>(k[e] - tj)) * 0x1.0p-24f + s_r[j[e]]);
       } else {
-        const double* s_c = reinterpret_cast<const double*>(lds_t);
-        t = double(k) * 0x1.0p-24 - s_c[j];
+        t[e] = double(k[e]) * 0x1.0p-24 - s_c[j[e]];
       }
-      const int64_t r = full ? row + e : row++;
-      const double g = double(grad[r]);
-      atomicAdd(&h0[j], g);
-      atomicAdd(&h1[j], g * t);
+    }
+    if (keepbits == 0xF) {
+      if constexpr (sizeof(Real) == 4) {
+        if ((row & 3) == 0) {
+          const float4 g4 = *reinterpret_cast<const float4*>(grad + row);
+          g[0] = g4.x; g[1] = g4.y; g[2] = g4.z; g[3] = g4.w;
+        } else {
+#pragma unroll
+          for (int e = 0; e < 4; ++e) g[e] = double(grad[row + e]);
+        }
+      } else {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) g[e] = double(grad[row + e]);
+      }
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        atomicAdd(&h0[j[e]], g[e]);
+        atomicAdd(&h1[j[e]], g[e] * t[e]);
+      }
+    } else {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        if (!((keepbits >> e) & 1)) continue;
+        const double ge = double(grad[row++]);
+        atomicAdd(&h0[j[e]], ge);
+        atomicAdd(&h1[j[e]], ge * t[e]);
+      }
     }
   }
   __syncthreads();
@@ -624,7 +814,7 @@ __global__ void __launch_bounds__(1024) vjp_kernel(BatchedParams P, const Real* 
     vjp_tail<Real, MAXPER>(P, in, T, h0, h1, grad_density, scratch);
     return;
   }
-  // multi-tile instance: publish partial sums; the last tile reduces them in order
+  // multi-chunk instance: publish partial sums; the last chunk reduces them in order
   double* part = P.partials + ch.partial_off;
   for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
     part[i] = h0[i];
@@ -655,22 +845,21 @@ __global__ void __launch_bounds__(1024) vjp_kernel(BatchedParams P, const Real* 
 }
 
 // ---------------------------------------------------------------------------
-// Host-side launchers (called from capi.cpp).
+// Host-side launchers (called from capi.hip).
 template <typename Real>
-hipError_t launch_tables(const BatchedParams& P, int threads, bool sequential,
+hipError_t launch_tables(const BatchedParams& P, int threads, bool sequential, int64_t lds,
                          hipStream_t stream) {
-  if (sequential)
-    hipLaunchKernelGGL((tables_kernel<Real, true>), dim3(P.n_instances), dim3(threads), 0,
-                       stream, P);
-  else
-    hipLaunchKernelGGL((tables_kernel<Real, false>), dim3(P.n_instances), dim3(threads), 0,
-                       stream, P);
+  auto kern = sequential ? tables_kernel<Real, true> : tables_kernel<Real, false>;
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(kern, dim3(P.n_instances), dim3(threads), size_t(lds), stream, P);
   return hipGetLastError();
 }
 
 template <typename Real>
 hipError_t launch_sample(const BatchedParams& P, int threads, int64_t lds, hipStream_t stream) {
-  auto kern = sample_kernel<Real>;
+  auto kern = P.use_window ? sample_kernel<Real, true> : sample_kernel<Real, false>;
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
   if (e != hipSuccess) return e;
@@ -684,9 +873,9 @@ hipError_t launch_finalize(const BatchedParams& P, hipStream_t stream) {
 }
 
 template <typename Real>
-hipError_t launch_pack(const BatchedParams& P, void* packed, int32_t* packed_bins,
+hipError_t launch_pack(const BatchedParams& P, int threads, void* packed, int32_t* packed_bins,
                        hipStream_t stream) {
-  hipLaunchKernelGGL((pack_kernel<Real>), dim3(P.n_chunks), dim3(256), 0, stream, P,
+  hipLaunchKernelGGL((pack_kernel<Real>), dim3(P.n_chunks), dim3(threads), 0, stream, P,
                      static_cast<Real*>(packed), packed_bins);
   return hipGetLastError();
 }
@@ -713,12 +902,12 @@ hipError_t launch_vjp(const BatchedParams& P, int threads, int64_t lds, int maxp
   return hipErrorInvalidValue;
 }
 
-template hipError_t launch_tables<float>(const BatchedParams&, int, bool, hipStream_t);
-template hipError_t launch_tables<double>(const BatchedParams&, int, bool, hipStream_t);
+template hipError_t launch_tables<float>(const BatchedParams&, int, bool, int64_t, hipStream_t);
+template hipError_t launch_tables<double>(const BatchedParams&, int, bool, int64_t, hipStream_t);
 template hipError_t launch_sample<float>(const BatchedParams&, int, int64_t, hipStream_t);
 template hipError_t launch_sample<double>(const BatchedParams&, int, int64_t, hipStream_t);
-template hipError_t launch_pack<float>(const BatchedParams&, void*, int32_t*, hipStream_t);
-template hipError_t launch_pack<double>(const BatchedParams&, void*, int32_t*, hipStream_t);
+template hipError_t launch_pack<float>(const BatchedParams&, int, void*, int32_t*, hipStream_t);
+template hipError_t launch_pack<double>(const BatchedParams&, int, void*, int32_t*, hipStream_t);
 template hipError_t launch_vjp<float>(const BatchedParams&, int, int64_t, int, const void*, void*,
                                       hipStream_t);
 template hipError_t launch_vjp<double>(const BatchedParams&, int, int64_t, int, const void*, void*,

@@ -26,22 +26,28 @@ struct BatchedParams {
   int64_t* total;        // [1]
   double* norm_out;      // optional [n_instances]
   double* cdf_out;       // optional [total_nodes]
+  int64_t n_segments;    // n_chunks * waves
+  int32_t waves;         // waves per work-group
+  int32_t pad2;
   // scratch
   int32_t* flags;        // [0] = dense
-  int64_t* chunk_kept;   // [n_chunks]
-  int64_t* chunk_row;    // [n_chunks]
+  int64_t* seg_kept;     // [n_segments]
+  int64_t* seg_cap;      // [n_segments]
+  int64_t* seg_row;      // [n_segments] first packed row
+  int32_t* seg_drop;     // [n_segments] first iteration with a dropped slot
   uint32_t* done;        // [n_instances]
-  uint64_t* keepmask;    // per-tile keep bits
-  double* partials;      // per-tile partial VJP sums
+  uint64_t* keepmask;    // per-segment keep bits
+  double* partials;      // per-chunk partial VJP sums
 };
 
 template <typename Real>
-hipError_t launch_tables(const BatchedParams& P, int threads, bool sequential, hipStream_t stream);
+hipError_t launch_tables(const BatchedParams& P, int threads, bool sequential, int64_t lds,
+                         hipStream_t stream);
 template <typename Real>
 hipError_t launch_sample(const BatchedParams& P, int threads, int64_t lds, hipStream_t stream);
 hipError_t launch_finalize(const BatchedParams& P, hipStream_t stream);
 template <typename Real>
-hipError_t launch_pack(const BatchedParams& P, void* packed, int32_t* packed_bins,
+hipError_t launch_pack(const BatchedParams& P, int threads, void* packed, int32_t* packed_bins,
                        hipStream_t stream);
 template <typename Real>
 hipError_t launch_vjp(const BatchedParams& P, int threads, int64_t lds, int maxper,

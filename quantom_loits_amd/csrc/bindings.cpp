@@ -300,6 +300,8 @@ class BatchedPlan {
     d["scratch_bytes"] = layout_.scratch_bytes;
     d["lds_forward"] = layout_.lds_forward;
     d["lds_backward"] = layout_.lds_backward;
+    d["lds_tables"] = layout_.lds_tables;
+    d["n_segments"] = layout_.n_segments;
     d["mask_words"] = layout_.mask_words;
     d["partial_doubles"] = layout_.partial_doubles;
     d["dtype"] = layout_.dtype;

@@ -89,7 +89,7 @@ int make_plan(const loits_hip_instance* instances, int64_t n, int dtype,
     return fail(LOITS_HIP_EINVAL, "batched: guide_shift must be in [0, 3]");
   const int64_t per_iter = int64_t(4) * threads;
   int64_t chunk_events = tuning->chunk_events;
-  if (chunk_events <= 0) chunk_events = int64_t(1) << 62;
+  if (chunk_events <= 0 || chunk_events > (int64_t(1) << 30)) chunk_events = int64_t(1) << 30;
   chunk_events = (chunk_events + per_iter - 1) / per_iter * per_iter;
   const int64_t coef_size = dtype == LOITS_HIP_F32 ? 16 : 32;
   const int64_t real_size = dtype == LOITS_HIP_F32 ? 4 : 8;
@@ -103,14 +103,14 @@ int make_plan(const loits_hip_instance* instances, int64_t n, int dtype,
   L.scan_mode = tuning->scan_mode;
   L.persistent = tuning->persistent;
   int64_t out = 0, table_off = 0, mask_words = 0, partial_doubles = 0;
-  int64_t lds_f = 0, lds_b = 0;
+  int64_t lds_f = 0, lds_b = 0, lds_t = 0;
   plan->inst.resize(size_t(n));
   plan->chunks.clear();
   for (int64_t i = 0; i < n; ++i) {
     const loits_hip_instance& s = instances[i];
-    if (s.n_nodes < 2 || s.n_nodes > (int64_t(1) << 20))
+    if (s.n_nodes < 2 || s.n_nodes > 32767)
       return fail(LOITS_HIP_EINVAL, "batched: instance " + std::to_string(i) +
-                                        " needs 2 <= n_nodes <= 2^20");
+                                        " needs 2 <= n_nodes <= 32767");
     if (s.events < 0 || s.node_off < 0)
       return fail(LOITS_HIP_EINVAL, "batched: instance " + std::to_string(i) +
                                         " has a negative offset or event count");
@@ -127,11 +127,14 @@ int make_plan(const loits_hip_instance* instances, int64_t n, int dtype,
     d.counter_base = s.counter_base;
     d.stream = s.stream;
     d.accept = s.accept ? 1 : 0;
-    d.guide_log2 = std::min(20, std::max(4, ceil_log2(s.n_nodes) + tuning->guide_shift));
+    d.guide_log2 = std::min(18, std::max(4, ceil_log2(s.n_nodes) + tuning->guide_shift));
     d.table_off = table_off;
     const TableLayout tl = table_layout(s.n_nodes, d.guide_log2, coef_size);
     table_off += align16(tl.bytes);
     lds_f = std::max(lds_f, tl.cdf - tl.thr);
+    // sort/search copy of the thresholds + the crowded-cell list of level 2
+    lds_t = std::max(lds_t, 4 * (pad4(int64_t(1) << ceil_log2(s.n_nodes)) +
+                                 ((int64_t(2) << d.guide_log2) >> kSubLog2)));
     lds_b = std::max(lds_b, (tl.coef - tl.thr) + pad4(s.n_nodes) * (real_size + 16));
     L.max_nodes = std::max(L.max_nodes, s.n_nodes);
     L.total_nodes = std::max(L.total_nodes, s.node_off + s.n_nodes);
@@ -148,10 +151,10 @@ int make_plan(const loits_hip_instance* instances, int64_t n, int dtype,
       ch.index = int32_t(c);
       const uint64_t p_lo = s.counter_base + uint64_t(ch.slot_lo);
       const uint64_t p_hi = s.counter_base + uint64_t(ch.slot_hi);
-      const int64_t blocks = ch.slot_hi > ch.slot_lo ? int64_t(((p_hi + 3) >> 2) - (p_lo >> 2)) : 0;
-      const int64_t iters = (blocks + threads - 1) / threads;
+      const uint64_t blocks = ch.slot_hi > ch.slot_lo ? ((p_hi + 3) >> 2) - (p_lo >> 2) : 0;
+      const int64_t per_wave = int64_t(blocks_per_wave(blocks, nwaves) / 64);
       ch.mask_off = mask_words;
-      mask_words += iters * nwaves * 4;
+      mask_words += per_wave * nwaves * 4;
       ch.partial_off = nch > 1 ? partial_doubles : 0;
       if (nch > 1) partial_doubles += 2 * pad4(s.n_nodes);
       plan->chunks.push_back(ch);
@@ -165,7 +168,9 @@ int make_plan(const loits_hip_instance* instances, int64_t n, int dtype,
   L.table_bytes = std::max<int64_t>(table_off, 16);
   L.lds_forward = lds_f;
   L.lds_backward = lds_b;
-  if (lds_f > kMaxLds || lds_b > kMaxLds)
+  L.lds_tables = lds_t;
+  L.n_segments = L.n_chunks * nwaves;
+  if (lds_f > kMaxLds || lds_b > kMaxLds || lds_t > kMaxLds)
     return fail(LOITS_HIP_EUNSUPPORTED,
                 "batched: curve tables of " + std::to_string(L.max_nodes) +
                     " nodes exceed the LDS budget (" + std::to_string(std::max(lds_f, lds_b)) +
@@ -174,9 +179,9 @@ int make_plan(const loits_hip_instance* instances, int64_t n, int dtype,
     return fail(LOITS_HIP_EUNSUPPORTED, "batched: n_nodes / threads must be <= 32");
   L.mask_words = mask_words;
   L.partial_doubles = partial_doubles;
-  // scratch: flags | chunk_kept | chunk_row | done | keepmask | partials
-  L.scratch_bytes = 64 + align16(8 * L.n_chunks) * 2 + align16(4 * n) + align16(8 * mask_words) +
-                    align16(8 * partial_doubles);
+  // scratch: flags | seg_kept | seg_cap | seg_row | seg_drop | done | keepmask | partials
+  L.scratch_bytes = 64 + align16(8 * L.n_segments) * 3 + align16(4 * L.n_segments) + align16(4 * n) +
+                    align16(8 * mask_words) + align16(8 * partial_doubles);
   L.grid_forward = int32_t(L.n_chunks);
   L.grid_backward = int32_t(L.n_chunks);
   return LOITS_HIP_OK;
@@ -205,12 +210,18 @@ BatchedParams params_of(const loits_hip_batched_args* a) {
   P.norm_out = a->norm;
   P.cdf_out = a->cdf;
   uint8_t* s = static_cast<uint8_t*>(a->scratch);
+  P.n_segments = L.n_segments;
+  P.waves = L.threads / 64;
   P.flags = reinterpret_cast<int32_t*>(s);
   s += 64;
-  P.chunk_kept = reinterpret_cast<int64_t*>(s);
-  s += align16(8 * L.n_chunks);
-  P.chunk_row = reinterpret_cast<int64_t*>(s);
-  s += align16(8 * L.n_chunks);
+  P.seg_kept = reinterpret_cast<int64_t*>(s);
+  s += align16(8 * L.n_segments);
+  P.seg_cap = reinterpret_cast<int64_t*>(s);
+  s += align16(8 * L.n_segments);
+  P.seg_row = reinterpret_cast<int64_t*>(s);
+  s += align16(8 * L.n_segments);
+  P.seg_drop = reinterpret_cast<int32_t*>(s);
+  s += align16(4 * L.n_segments);
   P.done = reinterpret_cast<uint32_t*>(s);
   s += align16(4 * L.n_instances);
   P.keepmask = reinterpret_cast<uint64_t*>(s);
@@ -325,8 +336,8 @@ int loits_hip_batched_forward(const loits_hip_batched_args* args, void* stream) 
     hipStream_t s = static_cast<hipStream_t>(stream);
     const int tthreads = std::min(L.threads, 1024);
     const bool f32 = L.dtype == LOITS_HIP_F32;
-    int rc = check_hip(f32 ? launch_tables<float>(P, tthreads, L.scan_mode == 1, s)
-                           : launch_tables<double>(P, tthreads, L.scan_mode == 1, s),
+    int rc = check_hip(f32 ? launch_tables<float>(P, tthreads, L.scan_mode == 1, L.lds_tables, s)
+                           : launch_tables<double>(P, tthreads, L.scan_mode == 1, L.lds_tables, s),
                        "batched tables launch");
     if (rc != LOITS_HIP_OK) return rc;
     rc = check_hip(f32 ? launch_sample<float>(P, L.threads, L.lds_forward, s)
@@ -348,8 +359,8 @@ int loits_hip_batched_pack(const loits_hip_batched_args* args, void* packed, int
     const BatchedParams P = params_of(args);
     hipStream_t s = static_cast<hipStream_t>(stream);
     return check_hip(args->layout->dtype == LOITS_HIP_F32
-                         ? launch_pack<float>(P, packed, packed_bins, s)
-                         : launch_pack<double>(P, packed, packed_bins, s),
+                         ? launch_pack<float>(P, args->layout->threads, packed, packed_bins, s)
+                         : launch_pack<double>(P, args->layout->threads, packed, packed_bins, s),
                      "batched pack launch");
   } catch (const std::exception& e) {
     return fail(LOITS_HIP_ERUNTIME, std::string("batched_pack: ") + e.what());
@@ -373,8 +384,8 @@ int loits_hip_batched_backward(const loits_hip_batched_args* args, const void* g
       Q.norm_out = nullptr;
       Q.cdf_out = nullptr;
       const int tthreads = std::min(L.threads, 1024);
-      const int rc = check_hip(f32 ? launch_tables<float>(Q, tthreads, L.scan_mode == 1, s)
-                                   : launch_tables<double>(Q, tthreads, L.scan_mode == 1, s),
+      const int rc = check_hip(f32 ? launch_tables<float>(Q, tthreads, L.scan_mode == 1, L.lds_tables, s)
+                                   : launch_tables<double>(Q, tthreads, L.scan_mode == 1, L.lds_tables, s),
                                "batched tables launch");
       if (rc != LOITS_HIP_OK) return rc;
     }

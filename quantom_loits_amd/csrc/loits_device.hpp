@@ -45,6 +45,54 @@ __host__ __device__ __forceinline__ Words4 philox4x32_10(uint64_t block, uint64_
   return {c0, c1, c2, c3};
 }
 
+// a ^ b ^ c in one VALU instruction (gfx950 v_bitop3_b32, truth table 0x96;
+// c wave-uniform, held in an SGPR)
+__host__ __device__ __forceinline__ uint32_t xor3_su(uint32_t a, uint32_t b, uint32_t c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  uint32_t r;
+  asm("v_bitop3_b32 %0, %1, %2, %3 bitop3:0x96" : "=v"(r) : "v"(a), "v"(b), "s"(c));
+  return r;
+#else
+  return a ^ b ^ c;
+#endif
+}
+
+// Device Philox4x32-10 for a wave-uniform stream and seed: round 0 multiplies
+// the uniform stream word on the scalar unit; the other rounds fold the two
+// XORs into one v_bitop3_b32.
+__host__ __device__ __forceinline__ Words4 philox4x32_10_uniform(uint64_t block, uint64_t stream,
+                                                        uint64_t seed) {
+  uint32_t c0 = static_cast<uint32_t>(block), c1 = static_cast<uint32_t>(block >> 32);
+  uint32_t c2 = static_cast<uint32_t>(stream), c3 = static_cast<uint32_t>(stream >> 32);
+  uint32_t k0 = static_cast<uint32_t>(seed), k1 = static_cast<uint32_t>(seed >> 32);
+  {
+    const uint64_t p0 = static_cast<uint64_t>(0xD2511F53u) * c0;
+    const uint64_t p1 = static_cast<uint64_t>(0xCD9E8D57u) * c2;
+    const uint32_t n0 = static_cast<uint32_t>(p1 >> 32) ^ c1 ^ k0;
+    const uint32_t n2 = static_cast<uint32_t>(p0 >> 32) ^ c3 ^ k1;
+    c1 = static_cast<uint32_t>(p1);
+    c3 = static_cast<uint32_t>(p0);
+    c0 = n0;
+    c2 = n2;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+#pragma unroll
+  for (int r = 1; r < 10; ++r) {
+    const uint64_t p0 = static_cast<uint64_t>(0xD2511F53u) * c0;
+    const uint64_t p1 = static_cast<uint64_t>(0xCD9E8D57u) * c2;
+    const uint32_t n0 = xor3_su(static_cast<uint32_t>(p1 >> 32), c1, k0);
+    const uint32_t n2 = xor3_su(static_cast<uint32_t>(p0 >> 32), c3, k1);
+    c1 = static_cast<uint32_t>(p1);
+    c3 = static_cast<uint32_t>(p0);
+    c0 = n0;
+    c2 = n2;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  return {c0, c1, c2, c3};
+}
+
 __host__ __device__ __forceinline__ uint32_t word_of(const Words4& w, int e) {
   return e == 0 ? w.w0 : e == 1 ? w.w1 : e == 2 ? w.w2 : w.w3;
 }
@@ -124,19 +172,59 @@ struct InstanceDesc {
   int32_t n_chunks;
 };
 
+// A chunk is the work of one work-group; each of its waves owns a contiguous
+// run of Philox blocks (a "segment") and compacts it independently.
 struct ChunkDesc {
   int64_t slot_lo;      // first slot (relative to the instance)
   int64_t slot_hi;      // one past the last slot
-  int64_t mask_off;     // first 64-bit word of this tile's keep mask
-  int64_t partial_off;  // first double of this tile's partial VJP sums (multi-tile instances)
+  int64_t mask_off;     // first 64-bit word of this chunk's keep mask
+  int64_t partial_off;  // first double of this chunk's partial VJP sums (multi-chunk instances)
   int32_t instance;
   int32_t index;        // position among the instance's chunks
 };
 
+// Blocks of wave w of a chunk: [qa, qb), and the slot counters it covers.
+struct Segment {
+  uint64_t qa, qb;      // Philox blocks
+  uint64_t p_lo, p_hi;  // counters (slots + counter_base), p_lo <= p_hi
+  int64_t iters;        // 64-block iterations
+};
+
+__host__ __device__ __forceinline__ uint64_t blocks_per_wave(uint64_t nblk, int nwaves) {
+  const uint64_t per = (nblk + uint64_t(64 * nwaves) - 1) / uint64_t(64 * nwaves);
+  return per * 64;
+}
+
+__host__ __device__ __forceinline__ Segment segment_of(uint64_t counter_base, int64_t slot_lo,
+                                                       int64_t slot_hi, int w, int nwaves) {
+  Segment s{};
+  const uint64_t p_lo = counter_base + uint64_t(slot_lo), p_hi = counter_base + uint64_t(slot_hi);
+  const uint64_t q_begin = p_lo >> 2;
+  const uint64_t q_end = slot_hi > slot_lo ? (p_hi + 3) >> 2 : q_begin;
+  const uint64_t bw = blocks_per_wave(q_end - q_begin, nwaves);
+  uint64_t qa = q_begin + bw * uint64_t(w);
+  uint64_t qb = qa + bw;
+  if (qa > q_end) qa = q_end;
+  if (qb > q_end) qb = q_end;
+  s.qa = qa;
+  s.qb = qb;
+  uint64_t lo = qa * 4, hi = qb * 4;
+  lo = lo < p_lo ? p_lo : lo;
+  hi = hi > p_hi ? p_hi : hi;
+  s.p_lo = lo;
+  s.p_hi = hi > lo ? hi : lo;
+  s.iters = int64_t((qb - qa + 63) / 64);
+  return s;
+}
+
 // Table blob of one instance (byte offsets from table_off), all 16-byte aligned:
 //   TableHeader
-//   uint32 thr[n_pad4]          thresholds T_t
-//   uint32 guide[g_pad4]        guide[c] = #{t : T_t < c << (24 - guide_log2)}, c <= G
+//   uint32 srt[n_pad4]          thresholds T_t sorted ascending (the search array;
+//                               equal to T for a monotone CDF)
+//   uint32 guide[G]             level 1: cell g: lo | hi << 16 with lo = #{t : srt_t < g << s},
+//                               hi = #{t : srt_t < (g + 1) << s}, s = 24 - guide_log2,
+//                               or kIndirect | offset into level 2
+//   uint32 guide2[2 G]          level 2 sub-cells (kSub per redirected cell)
 //   Coef   coef[n_pad4]         per-bin interpolation data (bin j uses nodes j, j+1)
 //   double cdf[n_pad4]          C_t (fp64)
 //   float  rlo[n_pad4]          r_j = T_j * 2^-24 - C_j rounded to float (fp32 path)
@@ -152,6 +240,19 @@ struct TableHeader {
 static_assert(sizeof(TableHeader) == 64, "header size");
 
 constexpr uint32_t kMonotone = 1u;
+// Every value the curve can produce is finite, non-zero and representable in
+// float (monotone thresholds, acceptance 1, C_{n-1} within 2^-23 of 1, finite
+// per-bin data, 1e-30 <= |x_t| <= 1e29 with one sign): the fp32 keep test
+// without a window cut is then always true.
+constexpr uint32_t kSafe = 2u;
+
+// Two-level guide.  Level 1 has G = 2^guide_log2 cells over the 24-bit k
+// range; a direct cell word is lo | hi << 16 (bit 31 clear).  A crowded cell
+// (hi - lo >= 2) may redirect (bit 31 set, low bits = word offset into level
+// 2) to kSub sub-cells of the same form covering its k range.
+constexpr int kSubLog2 = 6;
+constexpr int kSub = 1 << kSubLog2;
+constexpr uint32_t kIndirect = 0x80000000u;
 
 // fp32 path: u - C_j = (k - T_j) * 2^-24 + r_j with r_j = T_j * 2^-24 - C_j
 // rounded to float; value = x_j + m_j * (u - C_j).
@@ -193,7 +294,7 @@ __host__ __device__ __forceinline__ TableLayout table_layout(int64_t n, int guid
   TableLayout t;
   t.thr = sizeof(TableHeader);
   t.guide = t.thr + pad4(n) * 4;
-  t.coef = t.guide + pad4((int64_t(1) << guide_log2) + 1) * 4;
+  t.coef = t.guide + 3 * (int64_t(1) << guide_log2) * 4;
   t.cdf = t.coef + pad4(n) * coef_size;
   t.rlo = t.cdf + pad4(n) * 8;
   t.bytes = t.rlo + pad4(n) * 4;

@@ -48,8 +48,8 @@ class Tuning(C.Structure):
 class Layout(C.Structure):
     _fields_ = [(n, C.c_int64) for n in (
         "n_instances", "n_chunks", "capacity", "total_nodes", "max_nodes", "desc_bytes",
-        "table_bytes", "scratch_bytes", "lds_forward", "lds_backward", "mask_words",
-        "partial_doubles")] + [(n, C.c_int32) for n in (
+        "table_bytes", "scratch_bytes", "lds_forward", "lds_backward", "lds_tables", "n_segments",
+        "mask_words", "partial_doubles")] + [(n, C.c_int32) for n in (
             "dtype", "threads", "scan_mode", "persistent", "grid_forward", "grid_backward")]
 
 
@@ -75,6 +75,7 @@ def test_batched_layout_sizes_and_tiles(lib):
     assert lib.loits_hip_batched_layout(inst, 3, 1, C.byref(tun), C.byref(lay)) == 0
     assert lay.capacity == 14096 and lay.total_nodes == 3 * 2048 and lay.max_nodes == 2048
     assert lay.n_chunks == 3 + 1 + 1  # ceil(10000/4096), empty instance keeps one tile, 1
+    assert lay.n_segments == lay.n_chunks * 4  # 256 threads = 4 waves
     assert lay.lds_forward > 0 and lay.lds_backward > lay.lds_forward - 32 * 2048
     assert lay.partial_doubles == 3 * 2 * 2048  # only the multi-tile instance
     blob = (C.c_uint8 * lay.desc_bytes)()
@@ -89,7 +90,8 @@ def test_batched_layout_sizes_and_tiles(lib):
     (dict(threads=300), "threads must be"),
     (dict(nodes=[1]), "n_nodes"),
     (dict(out_shift=5), "out_off"),
-    (dict(nodes=[40000]), "LDS budget"),
+    (dict(nodes=[40000]), "n_nodes <= 32767"),
+    (dict(nodes=[20000]), "LDS budget"),
 ])
 def test_batched_layout_rejects_bad_arguments(lib, bad, msg):
     nodes = bad.get("nodes", [64])

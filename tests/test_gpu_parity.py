@@ -265,3 +265,32 @@ def test_batched_autograd_function(hip_device, oracle_port):
     ref = oracle_port.batched_backward(x.reshape(-1), d.reshape(-1), *desc, 7,
                                        w.detach().cpu().numpy())
     np.testing.assert_allclose(dt.grad.cpu().numpy(), ref, rtol=1e-9, atol=1e-9 * np.abs(ref).max())
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_batched_non_monotone_cdf_uses_reference_count(oracle_port, hip_device, dtype):
+    """Negative densities make the CDF decrease; the reference's interval is then
+    the count #{t : u >= C_t} - 1 (cpp/loits_core.cpp:12-18), not a sorted search."""
+    from quantom_loits_amd.batched import BatchedLOITS, Tuning
+
+    B, N, E = 3, 300, 20000
+    x = np.tile(np.linspace(0.001, 0.999, N), (B, 1))
+    dens = 1.0 + 0.3 * np.sin(np.linspace(0, 12, N))[None, :] * np.array([[1.0], [4.0], [6.0]])
+    assert (dens < 0).any()
+    if dtype == torch.float32:
+        x, dens = x.astype(np.float32).astype(np.float64), dens.astype(np.float32).astype(np.float64)
+    desc = uniform_batch(B, N, E, 1)
+    ref_v, ref_b, ref_k, _, _ = oracle_port.batched_forward(x.reshape(-1), dens.reshape(-1),
+                                                            *desc, 5)
+    s = BatchedLOITS.uniform(B, N, E, 1, dtype=dtype, tuning=Tuning(threads=256, chunk_events=8192,
+                                                                    scan_mode=1))
+    xt = torch.from_numpy(x.reshape(-1)).to(dtype).cuda()
+    dt = torch.from_numpy(dens.reshape(-1)).to(dtype).cuda()
+    values, kept, st, ex = s.run_forward(xt, dt, 5, want_bins=True)
+    assert kept.cpu().numpy().tolist() == ref_k.tolist()
+    assert np.array_equal(ex["bins"].cpu().numpy(), ref_b)
+    v = values.cpu().numpy().astype(np.float64)
+    if dtype == torch.float64:
+        assert np.array_equal(v, ref_v)
+    else:
+        assert np.max(np.abs(v - ref_v) / np.abs(ref_v)) <= 1e-5
