@@ -264,8 +264,8 @@ __global__ void __launch_bounds__(1024) tables_kernel(BatchedParams P) {
   __syncthreads();
 
   // Level 2: crowded cells (hi - lo >= 2), in cell order, get kSub sub-cells
-  // each while the 2 G words of level 2 last.
-  const int64_t cap_cells = (2 * G) >> kSubLog2;
+  // each while the G words of level 2 last.
+  const int64_t cap_cells = G >> kSubLog2;
   int32_t* crowd_list = reinterpret_cast<int32_t*>(sort_lds + pad4(int64_t(1) << ceil_log2_dev(n)));
   double carry = 0.0;
   for (int64_t base = 0; base < G; base += nt) {

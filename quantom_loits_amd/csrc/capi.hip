@@ -134,7 +134,7 @@ int make_plan(const loits_hip_instance* instances, int64_t n, int dtype,
     lds_f = std::max(lds_f, tl.cdf - tl.thr);
     // sort/search copy of the thresholds + the crowded-cell list of level 2
     lds_t = std::max(lds_t, 4 * (pad4(int64_t(1) << ceil_log2(s.n_nodes)) +
-                                 ((int64_t(2) << d.guide_log2) >> kSubLog2)));
+                                 ((int64_t(1) << d.guide_log2) >> kSubLog2)));
     lds_b = std::max(lds_b, (tl.coef - tl.thr) + pad4(s.n_nodes) * (real_size + 16));
     L.max_nodes = std::max(L.max_nodes, s.n_nodes);
     L.total_nodes = std::max(L.total_nodes, s.node_off + s.n_nodes);

@@ -224,7 +224,7 @@ __host__ __device__ __forceinline__ Segment segment_of(uint64_t counter_base, in
 //   uint32 guide[G]             level 1: cell g: lo | hi << 16 with lo = #{t : srt_t < g << s},
 //                               hi = #{t : srt_t < (g + 1) << s}, s = 24 - guide_log2,
 //                               or kIndirect | offset into level 2
-//   uint32 guide2[2 G]          level 2 sub-cells (kSub per redirected cell)
+//   uint32 guide2[G]            level 2 sub-cells (kSub per redirected cell)
 //   Coef   coef[n_pad4]         per-bin interpolation data (bin j uses nodes j, j+1)
 //   double cdf[n_pad4]          C_t (fp64)
 //   float  rlo[n_pad4]          r_j = T_j * 2^-24 - C_j rounded to float (fp32 path)
@@ -294,7 +294,7 @@ __host__ __device__ __forceinline__ TableLayout table_layout(int64_t n, int guid
   TableLayout t;
   t.thr = sizeof(TableHeader);
   t.guide = t.thr + pad4(n) * 4;
-  t.coef = t.guide + 3 * (int64_t(1) << guide_log2) * 4;
+  t.coef = t.guide + 2 * (int64_t(1) << guide_log2) * 4;
   t.cdf = t.coef + pad4(n) * coef_size;
   t.rlo = t.cdf + pad4(n) * 8;
   t.bytes = t.rlo + pad4(n) * 4;

@@ -294,3 +294,43 @@ def test_batched_non_monotone_cdf_uses_reference_count(oracle_port, hip_device, 
         assert np.array_equal(v, ref_v)
     else:
         assert np.max(np.abs(v - ref_v) / np.abs(ref_v)) <= 1e-5
+
+
+@pytest.mark.parametrize("guide_shift", [0, 1])
+def test_batched_spiky_densities_exercise_guide_overflow(oracle_port, hip_device, guide_shift):
+    """Narrow peaks on a tiny floor put most thresholds into a few crowded guide
+    cells, more than level 2 of the guide can hold: the bisection fallback must
+    give the reference's bins exactly."""
+    from quantom_loits_amd.batched import BatchedLOITS, Tuning
+
+    B, N, E = 4, 2048, 40000
+    xg = np.linspace(0.001, 0.999, N)
+    rng = np.random.default_rng(11)
+    dens = []
+    for b in range(B):
+        d = np.full(N, 1e-9)
+        for mu in rng.uniform(0.1, 0.9, 6):
+            d += np.exp(-0.5 * ((xg - mu) / rng.uniform(1e-4, 2e-3)) ** 2)
+        dens.append(d)
+    x = np.tile(xg, (B, 1))
+    dens = np.array(dens)
+    desc = uniform_batch(B, N, E, 3)
+    ref_v, ref_b, ref_k, _, _ = oracle_port.batched_forward(x.reshape(-1), dens.reshape(-1),
+                                                            *desc, 77)
+    s = BatchedLOITS.uniform(B, N, E, 3, dtype=torch.float64,
+                             tuning=Tuning(threads=512, chunk_events=16384, scan_mode=1,
+                                           guide_shift=guide_shift))
+    xt = torch.from_numpy(x.reshape(-1)).cuda()
+    dt = torch.from_numpy(dens.reshape(-1)).cuda()
+    values, kept, st, ex = s.run_forward(xt, dt, 77, want_bins=True)
+    assert kept.cpu().numpy().tolist() == ref_k.tolist()
+    assert np.array_equal(ex["bins"].cpu().numpy(), ref_b)
+    assert np.array_equal(values.cpu().numpy(), ref_v)
+    gv = np.random.default_rng(2).standard_normal(len(ref_v))
+    ref_g = oracle_port.batched_backward(x.reshape(-1), dens.reshape(-1), *desc, 77, gv)
+    gd = torch.empty_like(dt)
+    s.backward_raw(xt, dt, 77, st, torch.from_numpy(gv).cuda(), gd)
+    got = gd.cpu().numpy().reshape(B, N)
+    ref = ref_g.reshape(B, N)
+    err = np.abs(got - ref).max(axis=1) / np.abs(ref).max(axis=1)
+    assert err.max() < 1e-9
