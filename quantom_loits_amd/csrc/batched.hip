@@ -46,15 +46,16 @@ __device__ double block_sum(double v, double* scratch) {
 }
 
 // Exclusive prefix (by thread index) of v; *total receives the sum.
-__device__ double block_exclusive_scan(double v, double* scratch, double* total) {
+template <typename T>
+__device__ T block_exclusive_scan(T v, T* scratch, T* total) {
   const int wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
-  const double inc = wave_inclusive_scan(v);
+  const T inc = wave_inclusive_scan(v);
   __syncthreads();
   if (lane_id() == kWave - 1) scratch[wave] = inc;
   __syncthreads();
-  double before = 0.0, all = 0.0;
+  T before = T(0), all = T(0);
   for (int w = 0; w < nwaves; ++w) {
-    const double s = scratch[w];
+    const T s = scratch[w];
     if (w < wave) before += s;
     all += s;
   }
@@ -90,7 +91,6 @@ struct Tables {
   uint32_t* guide;
   typename CoefOf<Real>::type* coef;
   double* cdf;
-  float* rlo;
   TableLayout L;
 };
 
@@ -105,28 +105,70 @@ __device__ __forceinline__ Tables<Real> tables_of(uint8_t* base, const InstanceD
   t.guide = reinterpret_cast<uint32_t*>(blob + t.L.guide);
   t.coef = reinterpret_cast<Coef*>(blob + t.L.coef);
   t.cdf = reinterpret_cast<double*>(blob + t.L.cdf);
-  t.rlo = reinterpret_cast<float*>(blob + t.L.rlo);
   return t;
 }
 
+// 16-byte copies by the whole work-group (LDS <-> global).
+__device__ __forceinline__ void lds_copy(void* dst, const void* src, int64_t bytes) {
+  const uint4* s = static_cast<const uint4*>(src);
+  uint4* d = static_cast<uint4*>(dst);
+  for (int64_t i = threadIdx.x; i < bytes / 16; i += blockDim.x) d[i] = s[i];
+}
+
 // ---------------------------------------------------------------------------
-// Tables: one work-group per instance.  Dynamic LDS: 4 * next_pow2(max n)
-// bytes, used only to sort the thresholds of a non-monotone CDF.
+// Table build: one work-group builds one instance's tables in LDS (layout
+// lds_build_layout): trapezoid norm and CDF (parallel blocked scan or the
+// reference's sequential chain), integer thresholds (sorted when the CDF is
+// not monotone), the two-level guide and per-bin interpolation data
+// (cpp/loits_core.cpp:55-165, 190-195).  A `writer` group also publishes them
+// to the global blob for the VJP and the optional norm / CDF outputs.
+struct BuildShared {
+  double scratch[16];
+  double norm, inv;
+  uint32_t bits;  // AND over nodes: 1 monotone, 2 finite, 4 x > 0, 8 x < 0
+  uint32_t flags;
+  int assigned, used;
+};
+
+template <typename Real>
+struct LdsTables {
+  uint32_t* srt;
+  uint32_t* guide;
+  typename CoefOf<Real>::type* coef;
+  double* cdf;
+  uint2* crowd;
+};
+
+template <typename Real>
+__device__ __forceinline__ LdsTables<Real> lds_tables_of(uint8_t* lds, const InstanceDesc& in) {
+  using Coef = typename CoefOf<Real>::type;
+  const LdsLayout l = lds_build_layout(in.n_nodes, in.guide_log2, sizeof(Coef));
+  LdsTables<Real> t;
+  t.srt = reinterpret_cast<uint32_t*>(lds + l.srt);
+  t.guide = reinterpret_cast<uint32_t*>(lds + l.guide);
+  t.coef = reinterpret_cast<Coef*>(lds + l.coef);
+  t.cdf = reinterpret_cast<double*>(lds + l.cdf);
+  t.crowd = reinterpret_cast<uint2*>(lds + l.crowd);
+  return t;
+}
+
 template <typename Real, bool SEQUENTIAL>
-__global__ void __launch_bounds__(1024) tables_kernel(BatchedParams P) {
-  extern __shared__ __align__(16) uint32_t sort_lds[];
-  __shared__ double scratch[16];
-  __shared__ double sh_norm, sh_inv;
-  const InstanceDesc in = P.inst[blockIdx.x];
+__device__ void build_tables(const BatchedParams& P, int inst, const InstanceDesc& in,
+                             uint8_t* lds, bool writer, BuildShared& sh) {
+  using Coef = typename CoefOf<Real>::type;
   const int64_t n = in.n_nodes;
   const Real* x = static_cast<const Real*>(P.x) + in.node_off;
   const Real* s = static_cast<const Real*>(P.density) + in.node_off;
-  Tables<Real> T = tables_of<Real>(P.tables, in);
+  const LdsTables<Real> L = lds_tables_of<Real>(lds, in);
   const double acc = in.accept ? 1.0 : 0.0;
   const int tid = threadIdx.x, nt = blockDim.x;
+  if (tid == 0) {
+    sh.bits = 0xFu;
+    sh.assigned = 0;
+    sh.used = 0;
+  }
 
-  if (tid == 0) P.done[blockIdx.x] = 0u;
-
+  // 1. norm and CDF
   if (SEQUENTIAL) {
     // The reference's own order: cpp/loits_core.cpp:73-80 then :130-135.
     if (tid == 0) {
@@ -139,223 +181,396 @@ __global__ void __launch_bounds__(1024) tables_kernel(BatchedParams P) {
       }
       const double inv = 1.0 / nrm;
       double cum = 0.0, rho_prev = double(s[0]) * inv;
-      T.cdf[0] = 0.0;
+      L.cdf[0] = 0.0;
       for (int64_t t = 1; t < n; ++t) {
         const double rho = double(s[t]) * inv;
         cum = __builtin_fma(0.5 * (double(x[t]) - double(x[t - 1])), rho_prev + rho, cum);
-        T.cdf[t] = cum * acc;
+        L.cdf[t] = cum * acc;
         rho_prev = rho;
       }
-      sh_norm = nrm;
-      sh_inv = inv;
+      sh.norm = nrm;
+      sh.inv = inv;
     }
-    __syncthreads();
   } else {
-    // Blocked parallel scan: each thread owns a contiguous segment.
+    // Blocked parallel scan: each thread owns a contiguous segment; segments
+    // of up to four nodes keep their nodes in registers.
     const int64_t per = (n + nt - 1) / nt;
     const int64_t t0 = tid * per, t1 = t0 + per < n ? t0 + per : n;
-    double local = 0.0;
-    for (int64_t t = t0 > 1 ? t0 : 1; t < t1; ++t)
-      local += 0.5 * (double(x[t]) - double(x[t - 1])) * (double(s[t - 1]) + double(s[t]));
-    const double nrm = block_sum(local, scratch);
-    const double inv = 1.0 / nrm;
-    double part = 0.0;
-    for (int64_t t = t0 > 1 ? t0 : 1; t < t1; ++t)
-      part += 0.5 * (double(x[t]) - double(x[t - 1])) *
-              (double(s[t - 1]) * inv + double(s[t]) * inv);
-    double total;
-    double run = block_exclusive_scan(part, scratch, &total);
-    for (int64_t t = t0; t < t1; ++t) {
-      if (t == 0) {
-        T.cdf[0] = 0.0;
-        continue;
+    double nrm, inv, run;
+    if (per <= 4) {
+      double xv[5], sv[5];  // nodes t0 - 1 .. t0 + 3
+#pragma unroll
+      for (int q = 0; q < 5; ++q) {
+        const int64_t t = t0 - 1 + q;
+        const bool ok = t >= 0 && t < t1;
+        xv[q] = ok ? double(x[t]) : 0.0;
+        sv[q] = ok ? double(s[t]) : 0.0;
       }
-      run += 0.5 * (double(x[t]) - double(x[t - 1])) *
-             (double(s[t - 1]) * inv + double(s[t]) * inv);
-      T.cdf[t] = run * acc;
+      double local = 0.0;
+#pragma unroll
+      for (int q = 1; q < 5; ++q) {
+        const int64_t t = t0 - 1 + q;
+        if (t >= 1 && t < t1) local += 0.5 * (xv[q] - xv[q - 1]) * (sv[q - 1] + sv[q]);
+      }
+      nrm = block_sum(local, sh.scratch);
+      inv = 1.0 / nrm;
+      double terms[4], part = 0.0;
+#pragma unroll
+      for (int q = 1; q < 5; ++q) {
+        const int64_t t = t0 - 1 + q;
+        terms[q - 1] = 0.5 * (xv[q] - xv[q - 1]) * (sv[q - 1] * inv + sv[q] * inv);
+        if (t >= 1 && t < t1) part += terms[q - 1];
+      }
+      double total;
+      run = block_exclusive_scan(part, sh.scratch, &total);
+#pragma unroll
+      for (int q = 1; q < 5; ++q) {
+        const int64_t t = t0 - 1 + q;
+        if (t >= t1) continue;
+        if (t == 0) {
+          L.cdf[0] = 0.0;
+          continue;
+        }
+        run += terms[q - 1];
+        L.cdf[t] = run * acc;
+      }
+    } else {
+      double local = 0.0;
+      for (int64_t t = t0 > 1 ? t0 : 1; t < t1; ++t)
+        local += 0.5 * (double(x[t]) - double(x[t - 1])) * (double(s[t - 1]) + double(s[t]));
+      nrm = block_sum(local, sh.scratch);
+      inv = 1.0 / nrm;
+      double part = 0.0;
+      for (int64_t t = t0 > 1 ? t0 : 1; t < t1; ++t)
+        part += 0.5 * (double(x[t]) - double(x[t - 1])) *
+                (double(s[t - 1]) * inv + double(s[t]) * inv);
+      double total;
+      run = block_exclusive_scan(part, sh.scratch, &total);
+      for (int64_t t = t0; t < t1; ++t) {
+        if (t == 0) {
+          L.cdf[0] = 0.0;
+          continue;
+        }
+        run += 0.5 * (double(x[t]) - double(x[t - 1])) *
+               (double(s[t - 1]) * inv + double(s[t]) * inv);
+        L.cdf[t] = run * acc;
+      }
     }
     if (tid == 0) {
-      sh_norm = nrm;
-      sh_inv = inv;
+      sh.norm = nrm;
+      sh.inv = inv;
     }
-    __syncthreads();
   }
-  const double nrm = sh_norm;
+  __syncthreads();
 
-  // Thresholds, monotonicity, per-bin interpolation data, value-range check.
-  bool mono = true, finite = true, pos = true, neg = true;
-  for (int64_t t = tid; t < n; t += nt) {
-    const double c = T.cdf[t];
+  // 2. thresholds, per-bin interpolation data, curve properties
+  const int64_t pw = pow2_ceil(n);
+  uint32_t bits = 0xFu;
+  for (int64_t t = tid; t < pad4(pw); t += nt) {
+    if (t >= n) {
+      L.srt[t] = kNever;
+      continue;
+    }
+    const double c = L.cdf[t];
     const uint32_t thr = threshold_of(c);
-    T.srt[t] = thr;
-    if (t > 0) mono = mono && thr >= threshold_of(T.cdf[t - 1]);
-    if (P.cdf_out) P.cdf_out[in.node_off + t] = c;
+    L.srt[t] = thr;
+    if (t > 0 && thr < threshold_of(L.cdf[t - 1])) bits &= ~1u;
     const double xt = double(x[t]);
     const double ax = __builtin_fabs(xt);
-    finite = finite && __builtin_isfinite(c) && ax >= 1e-30 && ax <= 1e29;
-    pos = pos && xt > 0.0;
-    neg = neg && xt < 0.0;
+    if (!(__builtin_isfinite(c) && ax >= 1e-30 && ax <= 1e29)) bits &= ~2u;
+    if (!(xt > 0.0)) bits &= ~4u;
+    if (!(xt < 0.0)) bits &= ~8u;
     if (t + 1 < n) {
-      const double c1 = T.cdf[t + 1];
+      const double c1 = L.cdf[t + 1];
       const double inv_d = 1.0 / (c1 - c + kEpsilon);              // cpp/loits_core.cpp:193
       const double m = (double(x[t + 1]) - xt) * inv_d;            // :194
-      finite = finite && __builtin_isfinite(m);
+      if (!__builtin_isfinite(m)) bits &= ~2u;
       if constexpr (sizeof(Real) == 4) {
-        const float r = static_cast<float>(double(thr) * 0x1.0p-24 - c);
-        T.coef[t] = CoefF32{thr, r, static_cast<float>(m), static_cast<float>(xt)};
-        T.rlo[t] = r;
+        const double r = double(thr) * 0x1.0p-24 - c;
+        L.coef[t] = CoefF32{thr, static_cast<float>(m * 0x1.0p-24),
+                            static_cast<float>(__builtin_fma(m, r, xt)), static_cast<float>(r)};
       } else {
-        T.coef[t] = CoefF64{c, m, xt, thr, 0u};
+        L.coef[t] = CoefF64{c, m, xt, thr, 0u};
       }
     }
   }
-  mono = __syncthreads_and(mono);
-  finite = __syncthreads_and(finite);
-  pos = __syncthreads_and(pos);
-  neg = __syncthreads_and(neg);
-  const bool safe = finite && (pos || neg);
+  {
+    uint32_t wb = 0u;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) wb |= ballot(!((bits >> q) & 1u)) == 0ull ? (1u << q) : 0u;
+    if (lane_id() == 0 && wb != 0xFu) atomicAnd(&sh.bits, wb);
+  }
+  __syncthreads();
+  const uint32_t all = sh.bits;
+  const bool mono = (all & 1u) != 0;
+  const bool safe = (all & 2u) != 0 && (all & 12u) != 0;
 
+  // 3. sorted thresholds (count(k) = #{t : T_t <= k}); bins keep their own T_j
   if (!mono) {
-    // count(k) = #{t : T_t <= k} needs the thresholds in sorted order; bins
-    // keep their own T_j in coef.  Bitonic sort in LDS.
-    int64_t pw = 1;
-    while (pw < n) pw <<= 1;
-    for (int64_t i = tid; i < pw; i += nt) sort_lds[i] = i < n ? T.srt[i] : kNever;
-    __syncthreads();
     for (int64_t k = 2; k <= pw; k <<= 1) {
       for (int64_t j = k >> 1; j > 0; j >>= 1) {
         for (int64_t i = tid; i < pw; i += nt) {
           const int64_t ixj = i ^ j;
           if (ixj > i) {
-            const uint32_t a = sort_lds[i], b = sort_lds[ixj];
+            const uint32_t a = L.srt[i], b = L.srt[ixj];
             const bool up = (i & k) == 0;
             if ((a > b) == up) {
-              sort_lds[i] = b;
-              sort_lds[ixj] = a;
+              L.srt[i] = b;
+              L.srt[ixj] = a;
             }
           }
         }
         __syncthreads();
       }
     }
-    for (int64_t i = tid; i < n; i += nt) T.srt[i] = sort_lds[i];
-    __syncthreads();
   }
 
-  // Level-1 guide by scatter: srt position t is the lower bound of every cell
-  // start key g << shift with srt_{t-1} < key <= srt_t (srt_{-1} = -inf,
-  // srt_n = +inf).  Each cell word is written as two 16-bit halves.
+  // 4. per-cell threshold counts h[g] = #{t : srt_t >> shift == g} (in the
+  //    level-2 region, unused until step 6); thresholds >= 2^24 fall outside
+  //    every cell.  Each thread owns `per` consecutive cells; an exclusive scan
+  //    of the counts gives c0 = #{t : srt_t < g << shift} for its first cell.
   const int64_t G = int64_t(1) << in.guide_log2;
   const int shift = 24 - in.guide_log2;
-  uint16_t* g16 = reinterpret_cast<uint16_t*>(T.guide);
-  for (int64_t t = tid; t <= n; t += nt) {
-    const int64_t ga = t == 0 ? 0 : int64_t(T.srt[t - 1] >> shift) + 1;
-    int64_t gb = t == n ? G : int64_t(T.srt[t] >> shift);
-    gb = gb > G ? G : gb;
-    for (int64_t g = ga; g <= gb; ++g) {
-      if (g < G) g16[2 * g] = static_cast<uint16_t>(t);      // lo of cell g
-      if (g >= 1) g16[2 * g - 1] = static_cast<uint16_t>(t);  // hi of cell g - 1
-    }
+  uint32_t* hist = L.guide + G;
+  for (int64_t g = tid; g < G; g += nt) hist[g] = 0u;
+  __syncthreads();
+  for (int64_t t = tid; t < n; t += nt) {
+    const uint32_t cell = L.srt[t] >> shift;
+    if (cell < uint32_t(G)) atomicAdd(&hist[cell], 1u);
   }
-  // sorted thresholds in LDS for the level-2 searches (the sort already left
-  // them there when the CDF was not monotone)
-  if (mono)
-    for (int64_t i = tid; i < n; i += nt) sort_lds[i] = T.srt[i];
+  __syncthreads();
+  const int per = int(G / nt);  // G >= 4096 and nt <= 1024 are powers of two
+  const int64_t g0 = int64_t(tid) * per;
+  uint32_t local = 0u;
+  for (int i = 0; i < per; i += 4) {
+    const uint4 v = *reinterpret_cast<const uint4*>(hist + g0 + i);
+    local += v.x + v.y + v.z + v.w;
+  }
+  uint32_t in_cells;
+  const uint32_t run0 =
+      block_exclusive_scan(local, reinterpret_cast<uint32_t*>(sh.scratch), &in_cells);
+  // A cell whose thresholds do not sit at one key changing the bin by at most
+  // one needs a search word (guide_encode); h < 2 never does.
+  const int nn = int(n);
+  auto crowded = [&](uint32_t c0, uint32_t h) -> bool {
+    if (h < 2u) return false;
+    const int j0 = int(c0) - 1 < 0 ? 0 : (int(c0) - 1 > nn - 2 ? nn - 2 : int(c0) - 1);
+    const int c1m = int(c0 + h) - 1;
+    const int j1 = c1m > nn - 2 ? nn - 2 : c1m;
+    if (j1 == j0) return false;
+    return !(j1 - j0 == 1 && L.srt[c0] == L.srt[c1m]);
+  };
+
+  // 5. level-1 words.  Crowded cells get 2^f level-2 sub-cells each, in cell
+  //    order, while the G level-2 words last.
+  //    Most cells hold at most one threshold and take the branch-free direct
+  //    word; cells with several run guide_encode and the crowding logic.
+  auto jof = [&](uint32_t c) -> int {
+    const int j = int(c) - 1;
+    return j < 0 ? 0 : (j > nn - 2 ? nn - 2 : j);
+  };
+  uint32_t crowd_n = 0u, need = 0u, run = run0;
+  for (int i = 0; i < per; i += 4) {
+    const uint4 v = *reinterpret_cast<const uint4*>(hist + g0 + i);
+    if ((v.x | v.y | v.z | v.w) > 1u) {  // some cell holds two or more
+      const uint32_t hv[4] = {v.x, v.y, v.z, v.w};
+      uint32_t c = run;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        if (crowded(c, hv[q])) {
+          ++crowd_n;
+          need += 1u << guide_fanout_log2(int(hv[q]), shift);
+        }
+        c += hv[q];
+      }
+    }
+    run += v.x + v.y + v.z + v.w;
+  }
+  uint64_t tot64;
+  const uint64_t ex = block_exclusive_scan((uint64_t(need) << 32) | crowd_n,
+                                           reinterpret_cast<uint64_t*>(sh.scratch), &tot64);
+  uint32_t rank = uint32_t(ex), woff = uint32_t(ex >> 32);
+  // Redirected cells are the crowded cells before the first one that would
+  // overrun level 2; that cell's thread (or thread 0 when all fit) records
+  // their number and the level-2 words they use.
+  if (tid == 0 && (tot64 >> 32) <= uint64_t(G)) {
+    sh.assigned = int(uint32_t(tot64));
+    sh.used = int(tot64 >> 32);
+  }
+  run = run0;
+  for (int i = 0; i < per; i += 4) {
+    const uint4 v = *reinterpret_cast<const uint4*>(hist + g0 + i);
+    const uint32_t hv[4] = {v.x, v.y, v.z, v.w};
+    uint32_t wv[4];
+    uint32_t c = run;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      // guide_encode for a cell with at most one threshold
+      const int j0 = jof(c), j1 = jof(c + hv[q]);
+      const uint32_t t = L.srt[int(c) < nn ? int(c) : nn - 1];
+      const uint32_t key_lo = uint32_t(g0 + i + q) << shift;
+      wv[q] = uint32_t(j0) | ((j1 != j0 ? t - key_lo : kNoPoint) << 15);
+      c += hv[q];
+    }
+    if ((v.x | v.y | v.z | v.w) > 1u) {  // some cell holds two or more
+      c = run;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const uint32_t c0 = c, h = hv[q], g = uint32_t(g0 + i + q);
+        if (h >= 2u) {
+          if (!crowded(c0, h)) {
+            wv[q] = guide_encode(int(c0), int(c0 + h), L.srt, g << shift, nn);
+          } else {
+            const int fl = guide_fanout_log2(int(h), shift);
+            if (woff + (1u << fl) <= uint32_t(G)) {
+              L.crowd[rank] = make_uint2(c0 | ((c0 + h) << 16), woff | (g << 15));
+              wv[q] = kSpecial | (uint32_t(fl) << 24) | uint32_t(G + woff);
+            } else {
+              if (woff <= uint32_t(G)) {  // the first cell that does not fit
+                sh.assigned = int(rank);
+                sh.used = int(woff);
+              }
+              wv[q] = kSearch | c0 | (h << 15);
+            }
+            woff += 1u << fl;
+            ++rank;
+          }
+        }
+        c += h;
+      }
+    }
+    *reinterpret_cast<uint4*>(L.guide + g0 + i) = make_uint4(wv[0], wv[1], wv[2], wv[3]);
+    run = c;
+  }
   __syncthreads();
 
-  // Level 2: crowded cells (hi - lo >= 2), in cell order, get kSub sub-cells
-  // each while the G words of level 2 last.
-  const int64_t cap_cells = G >> kSubLog2;
-  int32_t* crowd_list = reinterpret_cast<int32_t*>(sort_lds + pad4(int64_t(1) << ceil_log2_dev(n)));
-  double carry = 0.0;
-  for (int64_t base = 0; base < G; base += nt) {
-    const int64_t g = base + tid;
-    bool crowd = false;
-    if (g < G) {
-      const uint32_t w = T.guide[g];
-      crowd = int(w >> 16) - int(w & 0xFFFFu) >= 2;
+  // 6. level-2 words (the counts are dead).  Each thread fills a contiguous
+  //    run of words, walking the redirected cells in order; a sub-cell's
+  //    upper bound is the next sub-cell's lower bound.
+  const int used = sh.used, n_assigned = sh.assigned;
+  const int per_item = (used + nt - 1) / nt;
+  int item = tid * per_item;
+  const int item_end = item + per_item < used ? item + per_item : used;
+  if (item < item_end) {
+    int r = 0, hi_r = n_assigned - 1;  // last redirected cell with first word <= item
+    while (r < hi_r) {
+      const int mid = (r + hi_r + 1) >> 1;
+      if (int(L.crowd[mid].y & 0x7FFFu) <= item) r = mid; else hi_r = mid - 1;
     }
-    double tot;
-    const int64_t rank = int64_t(carry + block_exclusive_scan(crowd ? 1.0 : 0.0, scratch, &tot));
-    if (crowd && rank < cap_cells) crowd_list[rank] = int32_t(g);
-    carry += tot;
-  }
-  __syncthreads();
-  const int64_t n_assigned = int64_t(carry) < cap_cells ? int64_t(carry) : cap_cells;
-  const int sub_shift = shift - kSubLog2;
-  for (int64_t item = tid; item < n_assigned * kSub; item += nt) {
-    const int64_t r = item >> kSubLog2, sub = item & (kSub - 1);
-    const int64_t g = crowd_list[r];
-    const uint32_t w = T.guide[g];
-    const int lo0 = int(w & 0xFFFFu), hi0 = int(w >> 16);
-    const uint64_t key_lo = (uint64_t(g) << shift) + (uint64_t(sub) << sub_shift);
-    const uint64_t key_hi = key_lo + (uint64_t(1) << sub_shift);
-    int a = lo0, b = hi0;  // first position with srt >= key_lo
+    uint2 cr = L.crowd[r];
+    int lo0 = int(cr.x & 0xFFFFu), hi0 = int(cr.x >> 16);
+    int ss = shift - guide_fanout_log2(hi0 - lo0, shift);
+    int cell_end = int(cr.y & 0x7FFFu) + (1 << (shift - ss));
+    uint32_t key = ((cr.y >> 15) << shift) + (uint32_t(item - int(cr.y & 0x7FFFu)) << ss);
+    int a = lo0, b = hi0;  // first position with srt >= key
     while (a < b) {
       const int mid = (a + b) >> 1;
-      if (uint64_t(sort_lds[mid]) < key_lo) a = mid + 1; else b = mid;
+      if (L.srt[mid] < key) a = mid + 1; else b = mid;
     }
-    int c2 = a, d2 = hi0;  // first position with srt >= key_hi
-    while (c2 < d2) {
-      const int mid = (c2 + d2) >> 1;
-      if (uint64_t(sort_lds[mid]) < key_hi) c2 = mid + 1; else d2 = mid;
+    for (; item < item_end; ++item) {
+      if (item == cell_end) {
+        cr = L.crowd[++r];
+        lo0 = int(cr.x & 0xFFFFu);
+        hi0 = int(cr.x >> 16);
+        ss = shift - guide_fanout_log2(hi0 - lo0, shift);
+        cell_end = item + (1 << (shift - ss));
+        key = (cr.y >> 15) << shift;
+        a = lo0;
+      }
+      const uint32_t key_hi = key + (1u << ss);
+      int c2 = a, d2 = hi0;  // first position with srt >= key_hi
+      while (c2 < d2) {
+        const int mid = (c2 + d2) >> 1;
+        if (L.srt[mid] < key_hi) c2 = mid + 1; else d2 = mid;
+      }
+      L.guide[G + item] = guide_encode(a, c2, L.srt, key, nn);
+      a = c2;
+      key = key_hi;
     }
-    T.guide[G + r * kSub + sub] = uint32_t(a) | (uint32_t(c2) << 16);
   }
-  __syncthreads();  // every level-1 word of an assigned cell has been read
-  for (int64_t r = tid; r < n_assigned; r += nt)
-    T.guide[crowd_list[r]] = kIndirect | uint32_t(G + r * kSub);
-
   if (tid == 0) {
-    T.hdr->norm = nrm;
-    T.hdr->inv_n = sh_inv;
-    const bool last_full = T.srt[n - 1] >= 16777214u;
-    T.hdr->flags = (mono ? kMonotone : 0u) | ((mono && safe && acc == 1.0 && last_full) ? kSafe : 0u);
-    T.hdr->n = static_cast<uint32_t>(n);
-    T.hdr->guide_log2 = static_cast<uint32_t>(in.guide_log2);
-    if (P.norm_out) P.norm_out[blockIdx.x] = nrm;
+    const bool last_full = L.srt[n - 1] >= 16777214u;
+    sh.flags = (mono ? kMonotone : 0u) | ((mono && safe && acc == 1.0 && last_full) ? kSafe : 0u);
   }
+  __syncthreads();
+
+  // 7. publish
+  if (writer) {
+    Tables<Real> T = tables_of<Real>(P.tables, in);
+    lds_copy(T.srt, L.srt, pad4(n) * 4);
+    lds_copy(T.guide, L.guide, 2 * G * 4);
+    lds_copy(T.coef, L.coef, pad4(n) * int64_t(sizeof(Coef)));
+    lds_copy(T.cdf, L.cdf, pad4(n) * 8);
+    if (P.cdf_out)
+      for (int64_t t = tid; t < n; t += nt) P.cdf_out[in.node_off + t] = L.cdf[t];
+    if (tid == 0) {
+      T.hdr->norm = sh.norm;
+      T.hdr->inv_n = sh.inv;
+      T.hdr->flags = sh.flags;
+      T.hdr->n = static_cast<uint32_t>(n);
+      T.hdr->guide_log2 = static_cast<uint32_t>(in.guide_log2);
+      if (P.norm_out) P.norm_out[inst] = sh.norm;
+      P.done[inst] = 0u;
+    }
+  }
+}
+
+// Standalone build (the VJP's recompute path): one work-group per instance.
+template <typename Real, bool SEQUENTIAL>
+__global__ void __launch_bounds__(1024) tables_kernel(BatchedParams P) {
+  extern __shared__ __align__(16) uint8_t lds[];
+  __shared__ BuildShared sh;
+  build_tables<Real, SEQUENTIAL>(P, blockIdx.x, P.inst[blockIdx.x], lds, true, sh);
 }
 
 // ---------------------------------------------------------------------------
-// Four interval searches: count(k) = #{t : srt_t <= k}, j = clamp(count - 1,
-// 0, n - 2).  The two-level guide narrows each search to [lo, lo + len); one
-// bisection step (always taken) settles len <= 1, and the loop after it runs
-// only for the rare cells that still hold several thresholds.
-__device__ __forceinline__ void bisect_step(uint32_t k, const uint32_t* srt, int n, int& lo,
-                                            int& len) {
-  const int half = len >> 1;
-  int idx = lo + half;
-  idx = idx < n ? idx : n - 1;
-  const bool le = srt[idx] <= k;
-  const bool act = len > 0;
-  lo = (act && le) ? idx + 1 : lo;
-  len = act ? (le ? len - half - 1 : half) : 0;
-}
-
-__device__ __forceinline__ void find_bins4(const uint32_t (&k)[4], const uint32_t* srt,
-                                           const uint32_t* guide, int shift, int n,
+// Interval search: j = clamp(#{t : srt_t <= k} - 1, 0, n - 2).  One guide word
+// settles almost every key; keys in redirected cells read one level-2 word,
+// and the rare crowded (sub-)cells bisect the sorted thresholds.
+__device__ __forceinline__ void find_bins4(const uint32_t (&k)[4], const uint32_t* guide,
+                                           const uint32_t* srt, int shift, int n,
                                            int (&j)[4]) {
-  int lo[4], len[4];
+  const uint32_t cmask = (1u << shift) - 1u;
+  uint32_t w[4];
 #pragma unroll
-  for (int e = 0; e < 4; ++e) {
-    uint32_t w = guide[k[e] >> shift];
-    if (w & kIndirect)
-      w = guide[(w & ~kIndirect) + ((k[e] >> (shift - kSubLog2)) & (kSub - 1))];
-    lo[e] = int(w & 0xFFFFu);
-    len[e] = int((w >> 16) & 0x7FFFu) - lo[e];
-  }
+  for (int e = 0; e < 4; ++e) w[e] = guide[k[e] >> shift];
 #pragma unroll
-  for (int e = 0; e < 4; ++e) bisect_step(k[e], srt, n, lo[e], len[e]);
-  while ((len[0] | len[1] | len[2] | len[3]) != 0) {
+  for (int e = 0; e < 4; ++e) j[e] = guide_direct_bin(w[e], k[e] & cmask);
+  if (__builtin_expect(((w[0] | w[1] | w[2] | w[3]) & kSpecial) != 0, 0)) {
 #pragma unroll
-    for (int e = 0; e < 4; ++e) bisect_step(k[e], srt, n, lo[e], len[e]);
-  }
+    for (int e = 0; e < 4; ++e) {
+      if ((w[e] & kSearch) == kSpecial) {  // indirect
+        const int ss = shift - int((w[e] >> 24) & 0xFu);
+        w[e] = guide[(w[e] & 0xFFFFFFu) + ((k[e] & cmask) >> ss)];
+        j[e] = guide_direct_bin(w[e], k[e] & ((1u << ss) - 1u));
+      }
+    }
+    int lo[4], len[4];
 #pragma unroll
-  for (int e = 0; e < 4; ++e) {
-    int jj = lo[e] - 1;
-    jj = jj < 0 ? 0 : jj;
-    j[e] = jj > n - 2 ? n - 2 : jj;
+    for (int e = 0; e < 4; ++e) {
+      const bool search = (w[e] & kSearch) == kSearch;
+      lo[e] = int(w[e] & 0x7FFFu);
+      len[e] = search ? int((w[e] >> 15) & 0x7FFFu) : 0;
+    }
+    if ((len[0] | len[1] | len[2] | len[3]) != 0) {
+      while ((len[0] | len[1] | len[2] | len[3]) != 0) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int half = len[e] >> 1;
+          const int idx = lo[e] + half;  // < lo + len <= n while len > 0
+          const bool le = len[e] > 0 && srt[len[e] > 0 ? idx : 0] <= k[e];
+          lo[e] = le ? idx + 1 : lo[e];
+          len[e] = len[e] > 0 ? (le ? len[e] - half - 1 : half) : 0;
+        }
+      }
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        if ((w[e] & kSearch) == kSearch) {
+          const int jj = lo[e] - 1;
+          j[e] = jj < 0 ? 0 : (jj > n - 2 ? n - 2 : jj);
+        }
+      }
+    }
   }
 }
 
@@ -395,20 +610,17 @@ __device__ __forceinline__ int slot_values(const uint32_t (&k)[4], const int (&j
   if constexpr (sizeof(Real) == 4) {
     if (!WINDOW && safe_curve) {  // every value of this curve is kept (kSafe)
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const float t =
-            static_cast<float>(static_cast<int32_t>(k[e] - cf[e].thr)) * 0x1.0p-24f + cf[e].r;
-        v[e] = __builtin_fmaf(cf[e].m, t, cf[e].x);
-      }
+      for (int e = 0; e < 4; ++e)
+        v[e] = __builtin_fmaf(cf[e].mk, static_cast<float>(static_cast<int32_t>(k[e] - cf[e].thr)),
+                              cf[e].a);
       return 0xF;
     }
     const float lo = static_cast<float>(P.win_lo), hi = static_cast<float>(P.win_hi);
     int unsafe = 0;
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-      const float t =
-          static_cast<float>(static_cast<int32_t>(k[e] - cf[e].thr)) * 0x1.0p-24f + cf[e].r;
-      v[e] = __builtin_fmaf(cf[e].m, t, cf[e].x);
+      v[e] = __builtin_fmaf(cf[e].mk, static_cast<float>(static_cast<int32_t>(k[e] - cf[e].thr)),
+                            cf[e].a);
       const float a = __builtin_fabsf(v[e]);
       bool safe = a >= 1e-30f && a <= 1e30f;
       bool in = true;
@@ -440,12 +652,6 @@ __device__ __forceinline__ int slot_values(const uint32_t (&k)[4], const int (&j
   return keep;
 }
 
-__device__ __forceinline__ void lds_copy(void* dst, const void* src, int64_t bytes) {
-  const uint4* s = static_cast<const uint4*>(src);
-  uint4* d = static_cast<uint4*>(dst);
-  for (int64_t i = threadIdx.x; i < bytes / 16; i += blockDim.x) d[i] = s[i];
-}
-
 // Exclusive prefix of per-lane counts (0..4) within a wave, and the wave total.
 __device__ __forceinline__ int wave_scan_counts(int cnt, int* total) {
   const uint64_t lt = lanemask_lt();
@@ -467,22 +673,20 @@ __device__ __forceinline__ int64_t chunk_blocks(const InstanceDesc& in, const Ch
 // of a segment so far was kept, rows are positional and no keep mask is
 // written; the first iteration with a dropped slot is recorded, and masks are
 // written from that iteration on.
-template <typename Real, bool WINDOW>
+template <typename Real, bool WINDOW, bool SEQUENTIAL>
 __global__ void __launch_bounds__(1024) sample_kernel(BatchedParams P) {
-  using Coef = typename CoefOf<Real>::type;
   extern __shared__ __align__(16) uint8_t lds[];
+  __shared__ BuildShared sh;
   const int c = blockIdx.x;
   const ChunkDesc ch = P.chunks[c];
   const InstanceDesc in = P.inst[ch.instance];
-  Tables<Real> T = tables_of<Real>(P.tables, in);
+  build_tables<Real, SEQUENTIAL>(P, ch.instance, in, lds, ch.index == 0, sh);
+  const LdsTables<Real> L = lds_tables_of<Real>(lds, in);
   const int n = static_cast<int>(in.n_nodes);
   const int shift = 24 - in.guide_log2;
-
-  lds_copy(lds, T.srt, T.L.cdf - T.L.thr);
-  __syncthreads();
-  const uint32_t* s_srt = reinterpret_cast<const uint32_t*>(lds);
-  const uint32_t* s_guide = reinterpret_cast<const uint32_t*>(lds + (T.L.guide - T.L.thr));
-  const Coef* s_coef = reinterpret_cast<const Coef*>(lds + (T.L.coef - T.L.thr));
+  const uint32_t* s_srt = L.srt;
+  const uint32_t* s_guide = L.guide;
+  const auto* s_coef = L.coef;
   const Real* x_g = static_cast<const Real*>(P.x) + in.node_off;
 
   const int nw = blockDim.x >> 6, w = threadIdx.x >> 6, lane = lane_id();
@@ -493,7 +697,7 @@ __global__ void __launch_bounds__(1024) sample_kernel(BatchedParams P) {
   int32_t* bins_out = P.bins ? P.bins + base_row : nullptr;
   const int64_t per_wave = int64_t(blocks_per_wave(uint64_t(chunk_blocks(in, ch)), nw) / 64);
   uint64_t* mask = P.keepmask + ch.mask_off + int64_t(w) * per_wave * 4;
-  const bool safe_curve = (T.hdr->flags & kSafe) != 0;
+  const bool safe_curve = (sh.flags & kSafe) != 0;
   // 32-bit slot positions relative to counter 4 qa: live slots are [off0, span)
   const uint32_t nblk = uint32_t(sg.qb - sg.qa);
   const uint32_t off0 = uint32_t(sg.p_lo - 4 * sg.qa);
@@ -506,9 +710,9 @@ __global__ void __launch_bounds__(1024) sample_kernel(BatchedParams P) {
     const Words4 wd = philox4x32_10_uniform(sg.qa + i, in.stream, P.seed);
     const uint32_t k[4] = {wd.w0 >> 8, wd.w1 >> 8, wd.w2 >> 8, wd.w3 >> 8};
     int j[4];
-    find_bins4(k, s_srt, s_guide, shift, n, j);
+    find_bins4(k, s_guide, s_srt, shift, n, j);
     Real v[4];
-    const int keep = slot_values<Real, WINDOW>(k, j, s_coef, T.cdf, x_g, P, safe_curve, v);
+    const int keep = slot_values<Real, WINDOW>(k, j, s_coef, L.cdf, x_g, P, safe_curve, v);
     const bool full_iter = (i0 > 0 || off0 == 0) && 4 * (i0 + 64) <= span;
     int live = 0xF;
     if (!full_iter) {
@@ -699,6 +903,7 @@ __device__ void vjp_tail(const BatchedParams& P, const InstanceDesc& in, const T
 template <typename Real, int MAXPER>
 __global__ void __launch_bounds__(1024) vjp_kernel(BatchedParams P, const Real* grad,
                                                    Real* grad_density) {
+  using Coef = typename CoefOf<Real>::type;
   extern __shared__ __align__(16) uint8_t lds[];
   __shared__ double scratch[16];
   __shared__ int sh_last;
@@ -709,20 +914,16 @@ __global__ void __launch_bounds__(1024) vjp_kernel(BatchedParams P, const Real* 
   const int n = static_cast<int>(in.n_nodes);
   const int shift = 24 - in.guide_log2;
 
-  // LDS: srt | guide | (r: fp32, C: fp64) | h0 | h1
-  const int64_t search_bytes = T.L.coef - T.L.thr;
+  // LDS: srt | guide | coef | h0 | h1
   const int64_t nb = pad4(n);
-  lds_copy(lds, T.srt, search_bytes);
-  uint8_t* lds_t = lds + search_bytes;
-  if constexpr (sizeof(Real) == 4) lds_copy(lds_t, T.rlo, nb * 4);
-  else lds_copy(lds_t, T.cdf, nb * 8);
-  double* h0 = reinterpret_cast<double*>(lds_t + nb * sizeof(Real));
+  lds_copy(lds, T.srt, T.L.cdf - T.L.thr);
+  double* h0 = reinterpret_cast<double*>(lds + (T.L.cdf - T.L.thr));
   double* h1 = h0 + nb;
   for (int64_t i = threadIdx.x; i < 2 * nb; i += blockDim.x) h0[i] = 0.0;
   __syncthreads();
   const uint32_t* s_srt = reinterpret_cast<const uint32_t*>(lds);
   const uint32_t* s_guide = reinterpret_cast<const uint32_t*>(lds + (T.L.guide - T.L.thr));
-  const bool mono = (T.hdr->flags & kMonotone) != 0;
+  const Coef* s_coef = reinterpret_cast<const Coef*>(lds + (T.L.coef - T.L.thr));
 
   const int nw = blockDim.x >> 6, w = threadIdx.x >> 6, lane = lane_id();
   const Segment sg = segment_of(in.counter_base, ch.slot_lo, ch.slot_hi, w, nw);
@@ -731,8 +932,6 @@ __global__ void __launch_bounds__(1024) vjp_kernel(BatchedParams P, const Real* 
   const int64_t row0 = P.seg_row[seg];
   const int64_t per_wave = int64_t(blocks_per_wave(uint64_t(chunk_blocks(in, ch)), nw) / 64);
   const uint64_t* mask = P.keepmask + ch.mask_off + int64_t(w) * per_wave * 4;
-  const float* s_r = reinterpret_cast<const float*>(lds_t);
-  const double* s_c = reinterpret_cast<const double*>(lds_t);
   // kept slots before the first masked iteration: every live slot before it
   const uint32_t nblk = uint32_t(sg.qb - sg.qa);
   const uint32_t off0 = uint32_t(sg.p_lo - 4 * sg.qa);
@@ -769,15 +968,15 @@ __global__ void __launch_bounds__(1024) vjp_kernel(BatchedParams P, const Real* 
       running += total;
     }
     int j[4];
-    find_bins4(k, s_srt, s_guide, shift, n, j);
+    find_bins4(k, s_guide, s_srt, shift, n, j);
     double t[4], g[4];
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       if constexpr (sizeof(Real) == 4) {
-        const uint32_t tj = mono ? s_srt[j[e]] : T.coef[j[e]].thr;
-        t[e] = double(static_cast<float>(static_cast<int32_t>(k[e] - tj)) * 0x1.0p-24f + s_r[j[e]]);
+        const CoefF32& cf = s_coef[j[e]];
+        t[e] = double(static_cast<float>(static_cast<int32_t>(k[e] - cf.thr)) * 0x1.0p-24f + cf.r);
       } else {
-        t[e] = double(k[e]) * 0x1.0p-24 - s_c[j[e]];
+        t[e] = double(k[e]) * 0x1.0p-24 - s_coef[j[e]].c;
       }
     }
     if (keepbits == 0xF) {
@@ -858,8 +1057,12 @@ hipError_t launch_tables(const BatchedParams& P, int threads, bool sequential, i
 }
 
 template <typename Real>
-hipError_t launch_sample(const BatchedParams& P, int threads, int64_t lds, hipStream_t stream) {
-  auto kern = P.use_window ? sample_kernel<Real, true> : sample_kernel<Real, false>;
+hipError_t launch_sample(const BatchedParams& P, int threads, bool sequential, int64_t lds,
+                         hipStream_t stream) {
+  auto kern = P.use_window ? (sequential ? sample_kernel<Real, true, true>
+                                         : sample_kernel<Real, true, false>)
+                           : (sequential ? sample_kernel<Real, false, true>
+                                         : sample_kernel<Real, false, false>);
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
   if (e != hipSuccess) return e;
@@ -904,8 +1107,8 @@ hipError_t launch_vjp(const BatchedParams& P, int threads, int64_t lds, int maxp
 
 template hipError_t launch_tables<float>(const BatchedParams&, int, bool, int64_t, hipStream_t);
 template hipError_t launch_tables<double>(const BatchedParams&, int, bool, int64_t, hipStream_t);
-template hipError_t launch_sample<float>(const BatchedParams&, int, int64_t, hipStream_t);
-template hipError_t launch_sample<double>(const BatchedParams&, int, int64_t, hipStream_t);
+template hipError_t launch_sample<float>(const BatchedParams&, int, bool, int64_t, hipStream_t);
+template hipError_t launch_sample<double>(const BatchedParams&, int, bool, int64_t, hipStream_t);
 template hipError_t launch_pack<float>(const BatchedParams&, int, void*, int32_t*, hipStream_t);
 template hipError_t launch_pack<double>(const BatchedParams&, int, void*, int32_t*, hipStream_t);
 template hipError_t launch_vjp<float>(const BatchedParams&, int, int64_t, int, const void*, void*,

@@ -44,7 +44,8 @@ template <typename Real>
 hipError_t launch_tables(const BatchedParams& P, int threads, bool sequential, int64_t lds,
                          hipStream_t stream);
 template <typename Real>
-hipError_t launch_sample(const BatchedParams& P, int threads, int64_t lds, hipStream_t stream);
+hipError_t launch_sample(const BatchedParams& P, int threads, bool sequential, int64_t lds,
+                         hipStream_t stream);
 hipError_t launch_finalize(const BatchedParams& P, hipStream_t stream);
 template <typename Real>
 hipError_t launch_pack(const BatchedParams& P, int threads, void* packed, int32_t* packed_bins,

@@ -92,7 +92,6 @@ int make_plan(const loits_hip_instance* instances, int64_t n, int dtype,
   if (chunk_events <= 0 || chunk_events > (int64_t(1) << 30)) chunk_events = int64_t(1) << 30;
   chunk_events = (chunk_events + per_iter - 1) / per_iter * per_iter;
   const int64_t coef_size = dtype == LOITS_HIP_F32 ? 16 : 32;
-  const int64_t real_size = dtype == LOITS_HIP_F32 ? 4 : 8;
   const int nwaves = threads / 64;
 
   loits_hip_layout& L = plan->layout;
@@ -127,15 +126,21 @@ int make_plan(const loits_hip_instance* instances, int64_t n, int dtype,
     d.counter_base = s.counter_base;
     d.stream = s.stream;
     d.accept = s.accept ? 1 : 0;
-    d.guide_log2 = std::min(18, std::max(4, ceil_log2(s.n_nodes) + tuning->guide_shift));
+    // finest guide that leaves the forward's LDS build and the VJP's tables and
+    // histograms within LDS
+    d.guide_log2 = std::min(kMaxGuideLog2,
+                            std::max(kMinGuideLog2, ceil_log2(s.n_nodes) + tuning->guide_shift));
+    while (d.guide_log2 > kMinGuideLog2 &&
+           std::max(lds_build_layout(s.n_nodes, d.guide_log2, coef_size).bytes,
+                    lds_table_bytes(s.n_nodes, d.guide_log2, coef_size) + pad4(s.n_nodes) * 16) >
+               kMaxLds)
+      --d.guide_log2;
     d.table_off = table_off;
     const TableLayout tl = table_layout(s.n_nodes, d.guide_log2, coef_size);
     table_off += align16(tl.bytes);
-    lds_f = std::max(lds_f, tl.cdf - tl.thr);
-    // sort/search copy of the thresholds + the crowded-cell list of level 2
-    lds_t = std::max(lds_t, 4 * (pad4(int64_t(1) << ceil_log2(s.n_nodes)) +
-                                 ((int64_t(1) << d.guide_log2) >> kSubLog2)));
-    lds_b = std::max(lds_b, (tl.coef - tl.thr) + pad4(s.n_nodes) * (real_size + 16));
+    lds_f = std::max(lds_f, lds_build_layout(s.n_nodes, d.guide_log2, coef_size).bytes);
+    lds_t = lds_f;
+    lds_b = std::max(lds_b, (tl.cdf - tl.thr) + pad4(s.n_nodes) * 16);
     L.max_nodes = std::max(L.max_nodes, s.n_nodes);
     L.total_nodes = std::max(L.total_nodes, s.node_off + s.n_nodes);
     const int64_t nch = std::max<int64_t>(1, (s.events + chunk_events - 1) / chunk_events);
@@ -170,7 +175,7 @@ int make_plan(const loits_hip_instance* instances, int64_t n, int dtype,
   L.lds_backward = lds_b;
   L.lds_tables = lds_t;
   L.n_segments = L.n_chunks * nwaves;
-  if (lds_f > kMaxLds || lds_b > kMaxLds || lds_t > kMaxLds)
+  if (lds_f > kMaxLds || lds_t > kMaxLds)
     return fail(LOITS_HIP_EUNSUPPORTED,
                 "batched: curve tables of " + std::to_string(L.max_nodes) +
                     " nodes exceed the LDS budget (" + std::to_string(std::max(lds_f, lds_b)) +
@@ -334,15 +339,11 @@ int loits_hip_batched_forward(const loits_hip_batched_args* args, void* stream) 
     const loits_hip_layout& L = *args->layout;
     const BatchedParams P = params_of(args);
     hipStream_t s = static_cast<hipStream_t>(stream);
-    const int tthreads = std::min(L.threads, 1024);
     const bool f32 = L.dtype == LOITS_HIP_F32;
-    int rc = check_hip(f32 ? launch_tables<float>(P, tthreads, L.scan_mode == 1, L.lds_tables, s)
-                           : launch_tables<double>(P, tthreads, L.scan_mode == 1, L.lds_tables, s),
-                       "batched tables launch");
-    if (rc != LOITS_HIP_OK) return rc;
-    rc = check_hip(f32 ? launch_sample<float>(P, L.threads, L.lds_forward, s)
-                       : launch_sample<double>(P, L.threads, L.lds_forward, s),
-                   "batched sample launch");
+    const bool seq = L.scan_mode == 1;
+    int rc = check_hip(f32 ? launch_sample<float>(P, L.threads, seq, L.lds_forward, s)
+                           : launch_sample<double>(P, L.threads, seq, L.lds_forward, s),
+                       "batched sample launch");
     if (rc != LOITS_HIP_OK) return rc;
     return check_hip(launch_finalize(P, s), "batched finalize launch");
   } catch (const std::exception& e) {
@@ -376,6 +377,11 @@ int loits_hip_batched_backward(const loits_hip_batched_args* args, const void* g
       return fail(LOITS_HIP_EINVAL, "batched_backward: grad_events is NULL");
     if (!grad_density) return fail(LOITS_HIP_EINVAL, "batched_backward: grad_density is NULL");
     const loits_hip_layout& L = *args->layout;
+    if (L.lds_backward > kMaxLds)
+      return fail(LOITS_HIP_EUNSUPPORTED,
+                  "batched_backward: curve tables of " + std::to_string(L.max_nodes) +
+                      " nodes and their histograms exceed the LDS budget (" +
+                      std::to_string(L.lds_backward) + " > " + std::to_string(kMaxLds) + " bytes)");
     const BatchedParams P = params_of(args);
     hipStream_t s = static_cast<hipStream_t>(stream);
     const bool f32 = L.dtype == LOITS_HIP_F32;

@@ -135,24 +135,57 @@ __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
 
 __device__ __forceinline__ int popc64(uint64_t v) { return __popcll(v); }
 
-// Inclusive prefix sum across the 64 lanes of a wave (Hillis-Steele over
-// lane shuffles; the compiler lowers the shuffles to DPP/ds_bpermute).
+// Wave64 prefix sums with DPP row shifts and row broadcasts (no LDS traffic):
+// Hillis-Steele within each row of 16 lanes, then row 0 / row 2 totals into
+// rows 1 / 3 (row_bcast:15) and the row-1 total into rows 2-3 (row_bcast:31).
+// Lanes without a source read 0, the identity.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ uint32_t dpp_shift_u32(uint32_t v) {
+  return static_cast<uint32_t>(
+      __builtin_amdgcn_update_dpp(0, static_cast<int>(v), CTRL, ROW_MASK, 0xF, false));
+}
+
+template <int CTRL, int ROW_MASK, typename T>
+__device__ __forceinline__ T dpp_shift(T v) {
+  if constexpr (sizeof(T) == 4) {
+    return __builtin_bit_cast(T, dpp_shift_u32<CTRL, ROW_MASK>(__builtin_bit_cast(uint32_t, v)));
+  } else {
+    const uint64_t b = __builtin_bit_cast(uint64_t, v);
+    const uint64_t lo = dpp_shift_u32<CTRL, ROW_MASK>(static_cast<uint32_t>(b));
+    const uint64_t hi = dpp_shift_u32<CTRL, ROW_MASK>(static_cast<uint32_t>(b >> 32));
+    return __builtin_bit_cast(T, lo | (hi << 32));
+  }
+}
+
 template <typename T>
 __device__ __forceinline__ T wave_inclusive_scan(T v) {
-  const int l = lane_id();
-#pragma unroll
-  for (int d = 1; d < kWave; d <<= 1) {
-    const T o = __shfl_up(v, d, kWave);
-    if (l >= d) v += o;
-  }
+  v = v + dpp_shift<0x111, 0xF>(v);  // row_shr:1
+  v = v + dpp_shift<0x112, 0xF>(v);  // row_shr:2
+  v = v + dpp_shift<0x114, 0xF>(v);  // row_shr:4
+  v = v + dpp_shift<0x118, 0xF>(v);  // row_shr:8
+  v = v + dpp_shift<0x142, 0xA>(v);  // row_bcast:15 -> rows 1, 3
+  v = v + dpp_shift<0x143, 0xC>(v);  // row_bcast:31 -> rows 2, 3
   return v;
 }
 
 template <typename T>
+__device__ __forceinline__ T wave_lane(T v, int lane) {
+  if constexpr (sizeof(T) == 4) {
+    return __builtin_bit_cast(
+        T, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), lane));
+  } else {
+    const uint64_t b = __builtin_bit_cast(uint64_t, v);
+    const uint64_t lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(b), lane));
+    const uint64_t hi =
+        static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(b >> 32), lane));
+    return __builtin_bit_cast(T, lo | (hi << 32));
+  }
+}
+
+// Sum over the 64 lanes (every lane gets it).
+template <typename T>
 __device__ __forceinline__ T wave_sum(T v) {
-#pragma unroll
-  for (int d = kWave / 2; d >= 1; d >>= 1) v += __shfl_xor(v, d, kWave);
-  return v;
+  return wave_lane(wave_inclusive_scan(v), kWave - 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -219,19 +252,17 @@ __host__ __device__ __forceinline__ Segment segment_of(uint64_t counter_base, in
 
 // Table blob of one instance (byte offsets from table_off), all 16-byte aligned:
 //   TableHeader
-//   uint32 srt[n_pad4]          thresholds T_t sorted ascending (the search array;
-//                               equal to T for a monotone CDF)
-//   uint32 guide[G]             level 1: cell g: lo | hi << 16 with lo = #{t : srt_t < g << s},
-//                               hi = #{t : srt_t < (g + 1) << s}, s = 24 - guide_log2,
-//                               or kIndirect | offset into level 2
-//   uint32 guide2[G]            level 2 sub-cells (kSub per redirected cell)
+//   uint32 srt[n_pad4]          thresholds T_t sorted ascending (equal to T for a
+//                               monotone CDF); read only by the rare bisection path
+//   uint32 guide[G]             level 1: one word per cell of 2^s k values,
+//                               s = 24 - guide_log2 (encoding below)
+//   uint32 guide2[G]            level 2: sub-cell words of the redirected crowded cells
 //   Coef   coef[n_pad4]         per-bin interpolation data (bin j uses nodes j, j+1)
 //   double cdf[n_pad4]          C_t (fp64)
-//   float  rlo[n_pad4]          r_j = T_j * 2^-24 - C_j rounded to float (fp32 path)
 struct TableHeader {
   double norm;
   double inv_n;
-  uint32_t flags;   // kMonotone
+  uint32_t flags;   // kMonotone | kSafe
   uint32_t n;
   uint32_t guide_log2;
   uint32_t pad;
@@ -246,24 +277,36 @@ constexpr uint32_t kMonotone = 1u;
 // without a window cut is then always true.
 constexpr uint32_t kSafe = 2u;
 
-// Two-level guide.  Level 1 has G = 2^guide_log2 cells over the 24-bit k
-// range; a direct cell word is lo | hi << 16 (bit 31 clear).  A crowded cell
-// (hi - lo >= 2) may redirect (bit 31 set, low bits = word offset into level
-// 2) to kSub sub-cells of the same form covering its k range.
-constexpr int kSubLog2 = 6;
-constexpr int kSub = 1 << kSubLog2;
-constexpr uint32_t kIndirect = 0x80000000u;
+// Guide words.  A cell covers keys [L, L + S).  With c0 = #{t : srt_t < L},
+// c1 = #{t : srt_t < L + S} and j(c) = clamp(c - 1, 0, n - 2):
+//   direct (bit 31 clear): bits 0-14 j0 = j(c0), bits 15-27 `off`; the bin of
+//     key k in the cell is j0 + (k - L >= off).  Used when every threshold in
+//     the cell sits at one key P with j(c1) - j0 <= 1 (off = P - L), or when
+//     j(c1) == j0 (off = kNoPoint, larger than any in-cell offset).
+//   search (bits 31, 30 set): bits 0-14 c0, bits 15-29 c1 - c0; bisect srt.
+//   indirect (bit 31 set, bit 30 clear, level 1 only): bits 0-23 word index of
+//     2^f level-2 words (direct or search) covering S / 2^f keys each, f in
+//     bits 24-27.  A crowded cell with c thresholds gets 2^f = min(S, pow2(2c))
+//     sub-cells while the G level-2 words last.
+// Cells are at most 2^12 keys wide (guide_log2 >= kMinGuideLog2).
+constexpr int kMinGuideLog2 = 12;
+constexpr int kMaxGuideLog2 = 14;
+constexpr uint32_t kSpecial = 0x80000000u;
+constexpr uint32_t kSearch = 0xC0000000u;
+constexpr uint32_t kNoPoint = 0x1000u;
 
-// fp32 path: u - C_j = (k - T_j) * 2^-24 + r_j with r_j = T_j * 2^-24 - C_j
-// rounded to float; value = x_j + m_j * (u - C_j).
-struct CoefF32 {
+// fp32 path: value = A_j + M_j * (k - T_j) with M_j = m_j 2^-24 and
+// A_j = x_j + m_j (T_j 2^-24 - C_j), both rounded from fp64, which is
+// x_j + m_j (u - C_j) for u = k 2^-24; r_j = T_j 2^-24 - C_j rounded to float
+// gives u - C_j = (k - T_j) 2^-24 + r_j for the VJP.
+struct alignas(16) CoefF32 {
   uint32_t thr;
+  float mk;
+  float a;
   float r;
-  float m;
-  float x;
 };
 // fp64 path: the reference's own operands.
-struct CoefF64 {
+struct alignas(16) CoefF64 {
   double c;
   double m;
   double x;
@@ -286,7 +329,7 @@ struct CoefOf<double> {
 __host__ __device__ __forceinline__ int64_t pad4(int64_t n) { return (n + 3) & ~int64_t(3); }
 
 struct TableLayout {
-  int64_t thr, guide, coef, cdf, rlo, bytes;  // byte offsets within the instance blob
+  int64_t thr, guide, coef, cdf, bytes;  // byte offsets within the instance blob
 };
 
 __host__ __device__ __forceinline__ TableLayout table_layout(int64_t n, int guide_log2,
@@ -296,16 +339,63 @@ __host__ __device__ __forceinline__ TableLayout table_layout(int64_t n, int guid
   t.guide = t.thr + pad4(n) * 4;
   t.coef = t.guide + 2 * (int64_t(1) << guide_log2) * 4;
   t.cdf = t.coef + pad4(n) * coef_size;
-  t.rlo = t.cdf + pad4(n) * 8;
-  t.bytes = t.rlo + pad4(n) * 4;
+  t.bytes = t.cdf + pad4(n) * 8;
   return t;
 }
 
-// Bytes of LDS the sampling kernel stages for one instance (thr, guide, coef).
+// Bytes of LDS the sampling kernel stages for one instance (srt, guide, coef).
 __host__ __device__ __forceinline__ int64_t lds_table_bytes(int64_t n, int guide_log2,
                                                             int64_t coef_size) {
   const TableLayout t = table_layout(n, guide_log2, coef_size);
   return t.cdf - t.thr;
+}
+
+// log2 of the level-2 fan-out of a crowded cell with cnt >= 2 thresholds.
+__host__ __device__ __forceinline__ int guide_fanout_log2(int cnt, int shift) {
+  const int l = 1 + (32 - __builtin_clz(uint32_t(cnt - 1)));  // 1 + ceil(log2 cnt)
+  return l < shift ? l : shift;
+}
+
+__host__ __device__ __forceinline__ int64_t pow2_ceil(int64_t n) {
+  int64_t p = 1;
+  while (p < n) p <<= 1;
+  return p;
+}
+
+// LDS image of one instance's tables while a work-group builds and samples it
+// (byte offsets): srt[pad4(pow2(n))] | guide[2G] | coef[pad4(n)] | cdf[pad4(n)]
+// | crowd[G / 4] (uint2 per redirected cell, used only while building).
+struct LdsLayout {
+  int64_t srt, guide, coef, cdf, crowd, bytes;
+};
+
+__host__ __device__ __forceinline__ LdsLayout lds_build_layout(int64_t n, int guide_log2,
+                                                               int64_t coef_size) {
+  LdsLayout l;
+  const int64_t G = int64_t(1) << guide_log2;
+  l.srt = 0;
+  l.guide = pad4(pow2_ceil(n)) * 4;
+  l.coef = l.guide + 2 * G * 4;
+  l.cdf = l.coef + pad4(n) * coef_size;
+  l.crowd = l.cdf + pad4(n) * 8;
+  l.bytes = l.crowd + (G / 4) * 8;
+  return l;
+}
+
+// Bin of key k from a direct guide word; ko = k - L (the offset in the cell).
+__host__ __device__ __forceinline__ int guide_direct_bin(uint32_t w, uint32_t ko) {
+  return int(w & 0x7FFFu) + (ko >= ((w >> 15) & 0x1FFFu) ? 1 : 0);
+}
+
+// Guide word of a cell [key_lo, key_lo + S) with c0 / c1 thresholds below its
+// ends; srt is the sorted threshold array.
+__host__ __device__ __forceinline__ uint32_t guide_encode(int c0, int c1, const uint32_t* srt,
+                                                          uint32_t key_lo, int n) {
+  const int j0 = c0 - 1 < 0 ? 0 : (c0 - 1 > n - 2 ? n - 2 : c0 - 1);
+  const int j1 = c1 - 1 < 0 ? 0 : (c1 - 1 > n - 2 ? n - 2 : c1 - 1);
+  if (j1 == j0) return uint32_t(j0) | (kNoPoint << 15);
+  if (j1 - j0 == 1 && srt[c0] == srt[c1 - 1]) return uint32_t(j0) | ((srt[c0] - key_lo) << 15);
+  return kSearch | uint32_t(c0) | (uint32_t(c1 - c0) << 15);
 }
 
 }  // namespace qloits
