@@ -41,8 +41,11 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--threads", type=int, default=1024)
     p.add_argument("--chunk", type=int, default=1 << 18)
-    p.add_argument("--guide-shift", type=int, default=1)
+    p.add_argument("--guide-shift", type=int, default=2)
     p.add_argument("--scan-mode", type=int, default=0)
+    p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                   help="only copy a MIB-MiB buffer a few times (the plain-copy kernel that "
+                        "scales rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
     return p.parse_args()
 
 
@@ -79,14 +82,19 @@ def cpu_baseline():
         lib.backward(g, args[0], args[1], args[2], args[3], args[5], st)
         return time.perf_counter() - t0, st["events"].shape[0]
 
-    dt, _ = run(1024)  # calibrate
-    per_event = max(dt / (b * 1024), 1e-9)
-    e = int(min(1 << 16, max(1024, 10.0 / per_event / b)))
-    dt, n = run(e)
-    return dict(value=n / dt, unit="events/s", cores=threads, kind=kind,
-                sample=f"{b} curves x N={N} x {e} events fwd+VJP through the reference "
-                       f"{'openmp core' if kind == 'reference' else 'oracle port'} "
-                       f"(1-D mapping), {dt:.1f} s wall")
+    e = 1 << 14
+    run(e)  # warm-up
+    done = spent = 0.0
+    calls = 0
+    while spent < 10.0:  # repeat the same call until about 10 s of CPU work
+        dt, n = run(e)
+        spent += dt
+        done += n
+        calls += 1
+    return dict(value=done / spent, unit="events/s", cores=threads, kind=kind,
+                sample=f"{calls} x ({b} curves x N={N} x {e} events) fwd+VJP through the "
+                       f"reference {'openmp core' if kind == 'reference' else 'oracle port'} "
+                       f"(1-D mapping, 2-axis call with a dummy q axis), {spent:.1f} s")
 
 
 def main():
@@ -96,6 +104,16 @@ def main():
                                    "(the batched path shards by instances, see DESIGN.md)"}))
         raise SystemExit(2)
     import torch
+
+    if args.copy_calibration:
+        n = args.copy_calibration << 18  # float32 elements
+        src = torch.ones(n, device="cuda", dtype=torch.float32)
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": n * 4, "copies": 5}))
+        return
 
     sys.path.insert(0, str(ROOT))
     from quantom_loits_amd.batched import BatchedLOITS, Tuning
@@ -145,7 +163,7 @@ def main():
     if bwd_ms >= fwd_ms:
         dom = dict(name="vjp_kernel (batched VJP)", ms=bwd_ms, bytes=bwd_b)
     else:
-        dom = dict(name="tables+sample+finalize (batched forward)", ms=fwd_ms, bytes=fwd_b)
+        dom = dict(name="sample_kernel+finalize_kernel (batched forward)", ms=fwd_ms, bytes=fwd_b)
     achieved = dom["bytes"] / (dom["ms"] * 1e-3) / 1e9
     traffic = None
     tfile = ROOT / "profiles" / "traffic.json"

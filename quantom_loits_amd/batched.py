@@ -27,7 +27,7 @@ DTYPE_CODE = {torch.float64: 0, torch.float32: 1}
 @dataclass
 class Tuning:
     threads: int = 1024
-    guide_shift: int = 1
+    guide_shift: int = 2
     chunk_events: int = 1 << 18
     scan_mode: int = 0        # 0 parallel CDF scan, 1 the reference's sequential chain
     persistent: int = 0
