@@ -898,8 +898,33 @@ __device__ void vjp_tail(const BatchedParams& P, const InstanceDesc& in, const T
   }
 }
 
+// Per-bin data of the fixed-point VJP: t / (tau_j d_j) = float(k - T_j) * alpha_j
+// + beta_j lies in [0, 1) for every key of bin j of a monotone curve (tau_j = 1
+// except for the last bin, which also takes the keys above T_{n-1}).
+struct alignas(16) VjpBin {
+  uint32_t thr;
+  float alpha;
+  float beta;
+  float pad;
+};
+
+__device__ __forceinline__ double vjp_tau_d(const double* cdf, int64_t j, int64_t n) {
+  const double d = cdf[j + 1] - cdf[j] + kEpsilon;
+  if (j != n - 2) return d;
+  const double tail = (1.0 - cdf[j]) / d;
+  return (tail > 1.0 ? tail : 1.0) * d;
+}
+
 // VJP: one work-group per chunk, one segment per wave; the chunk that
 // completes an instance runs the tail.
+//
+// float32 curves with monotone thresholds accumulate h0 = sum g and
+// h1' = sum g t / (tau d) per bin as 64-bit integers (ds_add_u64, about half
+// the LDS cost of ds_add_f64): g 2^e rounded to an integer, with 2^e chosen
+// from the largest |g| the work-group sees first so that it maps below 2^24.
+// A value that would reach 2^30 (64x that maximum) or is not finite goes to
+// fp64 atomics in the chunk's global partial sums.  Other curves, and float64,
+// accumulate fp64 directly (ds_add_f64).
 template <typename Real, int MAXPER>
 __global__ void __launch_bounds__(1024) vjp_kernel(BatchedParams P, const Real* grad,
                                                    Real* grad_density) {
@@ -907,38 +932,116 @@ __global__ void __launch_bounds__(1024) vjp_kernel(BatchedParams P, const Real* 
   extern __shared__ __align__(16) uint8_t lds[];
   __shared__ double scratch[16];
   __shared__ int sh_last;
+  __shared__ uint32_t sh_gmax;
   const int c = blockIdx.x;
   const ChunkDesc ch = P.chunks[c];
   const InstanceDesc in = P.inst[ch.instance];
   Tables<Real> T = tables_of<Real>(P.tables, in);
   const int n = static_cast<int>(in.n_nodes);
   const int shift = 24 - in.guide_log2;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const bool fixed = sizeof(Real) == 4 && (T.hdr->flags & kMonotone) != 0;
 
-  // LDS: srt | guide | coef | h0 | h1
+  // LDS: srt | guide | per-bin data (coef, or VjpBin when fixed) | h0 | h1
   const int64_t nb = pad4(n);
-  lds_copy(lds, T.srt, T.L.cdf - T.L.thr);
-  double* h0 = reinterpret_cast<double*>(lds + (T.L.cdf - T.L.thr));
+  lds_copy(lds, T.srt, T.L.coef - T.L.thr);
+  uint8_t* s_bin = lds + (T.L.coef - T.L.thr);
+  double* h0 = reinterpret_cast<double*>(s_bin + nb * int64_t(sizeof(Coef)));
   double* h1 = h0 + nb;
-  for (int64_t i = threadIdx.x; i < 2 * nb; i += blockDim.x) h0[i] = 0.0;
+  uint64_t* a0 = reinterpret_cast<uint64_t*>(h0);
+  uint64_t* a1 = reinterpret_cast<uint64_t*>(h1);
+  double* outl = P.partials + ch.partial_off;  // [2 nb] fp64 overflow sums, then partials
+  if (fixed) {
+    VjpBin* vb = reinterpret_cast<VjpBin*>(s_bin);
+    for (int64_t j = tid; j < n - 1; j += nt) {
+      const double tau_d = vjp_tau_d(T.cdf, j, n);
+      const uint32_t thr = T.coef[j].thr;
+      const double r = double(thr) * 0x1.0p-24 - T.cdf[j];
+      vb[j] = VjpBin{thr, static_cast<float>(0x1.0p-24 / tau_d), static_cast<float>(r / tau_d),
+                     0.0f};
+    }
+  } else {
+    lds_copy(s_bin, T.coef, nb * int64_t(sizeof(Coef)));
+  }
+  for (int64_t i = tid; i < 2 * nb; i += nt) {
+    h0[i] = 0.0;
+    outl[i] = 0.0;
+  }
+  if (tid == 0) sh_gmax = 0u;
   __syncthreads();
   const uint32_t* s_srt = reinterpret_cast<const uint32_t*>(lds);
   const uint32_t* s_guide = reinterpret_cast<const uint32_t*>(lds + (T.L.guide - T.L.thr));
-  const Coef* s_coef = reinterpret_cast<const Coef*>(lds + (T.L.coef - T.L.thr));
+  const Coef* s_coef = reinterpret_cast<const Coef*>(s_bin);
+  const VjpBin* s_vb = reinterpret_cast<const VjpBin*>(s_bin);
 
-  const int nw = blockDim.x >> 6, w = threadIdx.x >> 6, lane = lane_id();
+  const int nw = nt >> 6, w = tid >> 6, lane = lane_id();
   const Segment sg = segment_of(in.counter_base, ch.slot_lo, ch.slot_hi, w, nw);
   const int64_t seg = int64_t(c) * nw + w;
   const int first_drop = P.seg_drop[seg];
   const int64_t row0 = P.seg_row[seg];
+
+  // Fixed-point scale: the largest |g| over the first rows of each segment
+  // holding a non-zero cotangent (bit patterns order |g|; NaN above inf).
+  float scale = 0.0f;
+  int scale_log2 = 0;
+  if (fixed) {
+    const int64_t kept = P.seg_kept[seg];
+    uint32_t m = 0u;
+    for (int64_t r0 = 0; r0 < kept && m == 0u; r0 += 256) {
+      uint32_t lm = 0u;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int64_t r = r0 + 4 * lane + q;
+        if (r < kept) {
+          const uint32_t bits = __builtin_bit_cast(uint32_t, float(grad[row0 + r])) & 0x7FFFFFFFu;
+          lm = bits > lm ? bits : lm;
+        }
+      }
+#pragma unroll
+      for (int d = 32; d >= 1; d >>= 1) {
+        const uint32_t o = uint32_t(__shfl_xor(int(lm), d, kWave));
+        lm = o > lm ? o : lm;
+      }
+      m = lm;
+    }
+    if (lane == 0 && m != 0u) atomicMax(&sh_gmax, m);
+    __syncthreads();
+    const uint32_t gm = sh_gmax;
+    const int ex = int(gm >> 23);  // |g| < 2^(ex - 126)
+    if (ex == 255) {
+      scale = __builtin_nanf("");  // inf / NaN cotangents: every value takes the fp64 path
+    } else {
+      scale_log2 = 24 - (ex - 126);
+      scale_log2 = scale_log2 > 126 ? 126 : scale_log2;
+      scale = __builtin_ldexpf(1.0f, scale_log2);
+    }
+  }
+
   const int64_t per_wave = int64_t(blocks_per_wave(uint64_t(chunk_blocks(in, ch)), nw) / 64);
   const uint64_t* mask = P.keepmask + ch.mask_off + int64_t(w) * per_wave * 4;
-  // kept slots before the first masked iteration: every live slot before it
   const uint32_t nblk = uint32_t(sg.qb - sg.qa);
   const uint32_t off0 = uint32_t(sg.p_lo - 4 * sg.qa);
   const uint32_t span = uint32_t(sg.p_hi - 4 * sg.qa);
   // kept slots before the first masked iteration: every live slot before it
   int64_t running = int64_t(4 * 64 * int64_t(first_drop)) - int64_t(off0);
   running = running < 0 ? 0 : running;
+
+  // one kept slot: bin j, cotangent g, key k
+  auto add_fixed = [&](int jj, float g, float th) {
+    const float gs = g * scale;
+    if (__builtin_expect(__builtin_fabsf(gs) < 0x1.0p30f, 1)) {
+      const int ia = static_cast<int>(__builtin_rintf(gs));
+      const int ib = static_cast<int>(__builtin_rintf(gs * th));
+      atomicAdd(reinterpret_cast<unsigned long long*>(a0 + jj),
+                static_cast<unsigned long long>(static_cast<int64_t>(ia)));
+      atomicAdd(reinterpret_cast<unsigned long long*>(a1 + jj),
+                static_cast<unsigned long long>(static_cast<int64_t>(ib)));
+    } else {
+      atomicAdd(outl + jj, double(g));
+      atomicAdd(outl + nb + jj, double(g) * double(th));
+    }
+  };
+
   int it = 0;
   for (uint32_t i0 = 0; i0 < nblk; i0 += 64, ++it) {
     const uint32_t i = i0 + uint32_t(lane);
@@ -969,45 +1072,79 @@ __global__ void __launch_bounds__(1024) vjp_kernel(BatchedParams P, const Real* 
     }
     int j[4];
     find_bins4(k, s_guide, s_srt, shift, n, j);
-    double t[4], g[4];
+    if (fixed) {
+      float th[4];
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      if constexpr (sizeof(Real) == 4) {
-        const CoefF32& cf = s_coef[j[e]];
-        t[e] = double(static_cast<float>(static_cast<int32_t>(k[e] - cf.thr)) * 0x1.0p-24f + cf.r);
-      } else {
-        t[e] = double(k[e]) * 0x1.0p-24 - s_coef[j[e]].c;
+      for (int e = 0; e < 4; ++e) {
+        const VjpBin vb = s_vb[j[e]];
+        th[e] = __builtin_fmaf(static_cast<float>(static_cast<int32_t>(k[e] - vb.thr)), vb.alpha,
+                               vb.beta);
       }
-    }
-    if (keepbits == 0xF) {
-      if constexpr (sizeof(Real) == 4) {
+      if (keepbits == 0xF) {
+        float g[4];
         if ((row & 3) == 0) {
           const float4 g4 = *reinterpret_cast<const float4*>(grad + row);
           g[0] = g4.x; g[1] = g4.y; g[2] = g4.z; g[3] = g4.w;
         } else {
 #pragma unroll
-          for (int e = 0; e < 4; ++e) g[e] = double(grad[row + e]);
+          for (int e = 0; e < 4; ++e) g[e] = float(grad[row + e]);
+        }
+#pragma unroll
+        for (int e = 0; e < 4; ++e) add_fixed(j[e], g[e], th[e]);
+      } else {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          if (!((keepbits >> e) & 1)) continue;
+          add_fixed(j[e], float(grad[row++]), th[e]);
+        }
+      }
+    } else {
+      double t[4], g[4];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        if constexpr (sizeof(Real) == 4) {
+          const CoefF32& cf = s_coef[j[e]];
+          t[e] = double(static_cast<float>(static_cast<int32_t>(k[e] - cf.thr)) * 0x1.0p-24f +
+                        cf.r);
+        } else {
+          t[e] = double(k[e]) * 0x1.0p-24 - s_coef[j[e]].c;
+        }
+      }
+      if (keepbits == 0xF) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) g[e] = double(grad[row + e]);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          atomicAdd(&h0[j[e]], g[e]);
+          atomicAdd(&h1[j[e]], g[e] * t[e]);
         }
       } else {
 #pragma unroll
-        for (int e = 0; e < 4; ++e) g[e] = double(grad[row + e]);
-      }
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        atomicAdd(&h0[j[e]], g[e]);
-        atomicAdd(&h1[j[e]], g[e] * t[e]);
-      }
-    } else {
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        if (!((keepbits >> e) & 1)) continue;
-        const double ge = double(grad[row++]);
-        atomicAdd(&h0[j[e]], ge);
-        atomicAdd(&h1[j[e]], ge * t[e]);
+        for (int e = 0; e < 4; ++e) {
+          if (!((keepbits >> e) & 1)) continue;
+          const double ge = double(grad[row++]);
+          atomicAdd(&h0[j[e]], ge);
+          atomicAdd(&h1[j[e]], ge * t[e]);
+        }
       }
     }
   }
   __syncthreads();
+
+  if (fixed) {  // integers (and overflow sums) -> h0 = sum g, h1 = sum g t, in place
+    const double inv_scale = __builtin_ldexp(1.0, -scale_log2);
+    for (int64_t i = tid; i < nb; i += nt) {
+      double v0 = 0.0, v1 = 0.0;
+      if (i < n - 1) {
+        v0 = double(static_cast<int64_t>(a0[i])) * inv_scale + outl[i];
+        v1 = (double(static_cast<int64_t>(a1[i])) * inv_scale + outl[nb + i]) *
+             vjp_tau_d(T.cdf, i, n);
+      }
+      h0[i] = v0;
+      h1[i] = v1;
+    }
+    __syncthreads();
+  }
 
   if (in.n_chunks == 1) {
     vjp_tail<Real, MAXPER>(P, in, T, h0, h1, grad_density, scratch);
@@ -1015,20 +1152,20 @@ __global__ void __launch_bounds__(1024) vjp_kernel(BatchedParams P, const Real* 
   }
   // multi-chunk instance: publish partial sums; the last chunk reduces them in order
   double* part = P.partials + ch.partial_off;
-  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+  for (int64_t i = tid; i < n; i += nt) {
     part[i] = h0[i];
     part[nb + i] = h1[i];
   }
   __threadfence();
   __syncthreads();
-  if (threadIdx.x == 0) {
+  if (tid == 0) {
     const uint32_t old = atomicAdd(&P.done[ch.instance], 1u);
     sh_last = old == uint32_t(in.n_chunks - 1);
   }
   __syncthreads();
   if (!sh_last) return;
   __threadfence();
-  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+  for (int64_t i = tid; i < n; i += nt) {
     double a = 0.0, b = 0.0;
     for (int cc = in.first_chunk; cc < in.first_chunk + in.n_chunks; ++cc) {
       const double* pp = P.partials + P.chunks[cc].partial_off;
@@ -1038,7 +1175,7 @@ __global__ void __launch_bounds__(1024) vjp_kernel(BatchedParams P, const Real* 
     h0[i] = a;
     h1[i] = b;
   }
-  if (threadIdx.x == 0) P.done[ch.instance] = 0u;
+  if (tid == 0) P.done[ch.instance] = 0u;
   __syncthreads();
   vjp_tail<Real, MAXPER>(P, in, T, h0, h1, grad_density, scratch);
 }

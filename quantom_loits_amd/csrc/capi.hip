@@ -160,8 +160,8 @@ int make_plan(const loits_hip_instance* instances, int64_t n, int dtype,
       const int64_t per_wave = int64_t(blocks_per_wave(blocks, nwaves) / 64);
       ch.mask_off = mask_words;
       mask_words += per_wave * nwaves * 4;
-      ch.partial_off = nch > 1 ? partial_doubles : 0;
-      if (nch > 1) partial_doubles += 2 * pad4(s.n_nodes);
+      ch.partial_off = partial_doubles;  // VJP overflow sums, then partials
+      partial_doubles += 2 * pad4(s.n_nodes);
       plan->chunks.push_back(ch);
     }
     out += s.events;

@@ -211,7 +211,7 @@ struct ChunkDesc {
   int64_t slot_lo;      // first slot (relative to the instance)
   int64_t slot_hi;      // one past the last slot
   int64_t mask_off;     // first 64-bit word of this chunk's keep mask
-  int64_t partial_off;  // first double of this chunk's partial VJP sums (multi-chunk instances)
+  int64_t partial_off;  // first double of this chunk's 2 pad4(n) VJP overflow / partial sums
   int32_t instance;
   int32_t index;        // position among the instance's chunks
 };

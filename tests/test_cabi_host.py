@@ -77,7 +77,7 @@ def test_batched_layout_sizes_and_tiles(lib):
     assert lay.n_chunks == 3 + 1 + 1  # ceil(10000/4096), empty instance keeps one tile, 1
     assert lay.n_segments == lay.n_chunks * 4  # 256 threads = 4 waves
     assert lay.lds_forward > 0 and lay.lds_backward > lay.lds_forward - 32 * 2048
-    assert lay.partial_doubles == 3 * 2 * 2048  # only the multi-tile instance
+    assert lay.partial_doubles == lay.n_chunks * 2 * 2048  # 2 pad4(n) per chunk
     blob = (C.c_uint8 * lay.desc_bytes)()
     assert lib.loits_hip_batched_encode(inst, 3, C.byref(tun), C.byref(lay), blob) == 0
     # a layout for different tuning is rejected by encode
