@@ -179,10 +179,11 @@ typedef struct loits_hip_instance {
 
 typedef struct loits_hip_tuning {
   int32_t threads;          /* work-group size: 256, 512 or 1024 */
-  int32_t guide_shift;      /* guide cells = next_pow2(n_nodes) << guide_shift (0..3) */
+  int32_t guide_shift;      /* guide cells = next_pow2(n_nodes) << guide_shift (0..3), clamped
+                               to [2^12, 2^14] and lowered until the tables fit in LDS */
   int64_t chunk_events;     /* events per work-group (rounded up to 4*threads) */
   int32_t scan_mode;        /* 0 = parallel CDF scan, 1 = the reference's sequential chain */
-  int32_t persistent;       /* 1 = persistent grid pulling tiles from a counter */
+  int32_t persistent;       /* reserved, must be 0 */
 } loits_hip_tuning;
 
 typedef struct loits_hip_layout {
@@ -192,7 +193,7 @@ typedef struct loits_hip_layout {
   int64_t scratch_bytes;    /* per-call scratch (counters, partial VJP sums) */
   int64_t lds_forward;      /* dynamic LDS of the sampling kernel */
   int64_t lds_backward;     /* dynamic LDS of the VJP kernel */
-  int64_t lds_tables;       /* dynamic LDS of the tables kernel (threshold sort) */
+  int64_t lds_tables;       /* dynamic LDS of the standalone table build (VJP recompute) */
   int64_t n_segments;       /* chunks * waves per work-group */
   int64_t mask_words;       /* 64-bit keep-mask words in scratch */
   int64_t partial_doubles;  /* partial VJP sums in scratch (multi-chunk instances) */

@@ -92,7 +92,7 @@ class BatchedLOITS:
         L = self.layout
         return dict(
             tables=torch.empty(int(L["table_bytes"]), dtype=torch.uint8, device=d),
-            scratch=torch.empty(int(L["scratch_bytes"]), dtype=torch.uint8, device=d),
+            scratch=torch.zeros(int(L["scratch_bytes"]), dtype=torch.uint8, device=d),
             events=torch.empty(max(self.capacity, 1), dtype=self.dtype, device=d),
             kept=torch.empty(self.n_instances, dtype=torch.int64, device=d),
             total=torch.empty(1, dtype=torch.int64, device=d),

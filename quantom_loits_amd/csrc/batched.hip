@@ -770,38 +770,47 @@ __global__ void __launch_bounds__(1024) sample_kernel(BatchedParams P) {
 }
 
 // ---------------------------------------------------------------------------
-// Finalize: one work-group.  seg_row = exclusive scan of seg_kept; kept per
-// instance; total; dense flag (every segment kept all of its slots).
+// Finalize: one work-group.  seg_row = exclusive scan of seg_kept (four
+// segments per thread per tile); kept per instance; total; dense flag (every
+// segment kept all of its slots).
 __global__ void __launch_bounds__(1024) finalize_kernel(BatchedParams P) {
-  __shared__ double scratch[16];
+  __shared__ int64_t scratch[16];
   __shared__ int sh_sparse;
-  if (threadIdx.x == 0) sh_sparse = 0;
-  __syncthreads();
-  double carry = 0.0;  // counts < 2^53: exact in double
+  const int tid = threadIdx.x, nt = blockDim.x;
+  if (tid == 0) sh_sparse = 0;
+  const int64_t nseg = P.n_segments;
+  int64_t carry = 0;
   bool sparse = false;
-  for (int64_t base = 0; base < P.n_segments; base += blockDim.x) {
-    const int64_t sgi = base + threadIdx.x;
-    double v = 0.0;
-    if (sgi < P.n_segments) {
-      v = double(P.seg_kept[sgi]);
-      sparse = sparse || P.seg_kept[sgi] != P.seg_cap[sgi];
+  for (int64_t base = 0; base < nseg; base += 4 * int64_t(nt)) {
+    const int64_t s0 = base + 4 * int64_t(tid);
+    int64_t v[4], local = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int64_t sgi = s0 + q;
+      v[q] = sgi < nseg ? P.seg_kept[sgi] : 0;
+      sparse = sparse || (sgi < nseg && v[q] != P.seg_cap[sgi]);
+      local += v[q];
     }
-    double tot;
-    const double ex = block_exclusive_scan(v, scratch, &tot);
-    if (sgi < P.n_segments) P.seg_row[sgi] = static_cast<int64_t>(carry + ex);
+    int64_t tot;
+    int64_t run = carry + block_exclusive_scan(local, scratch, &tot);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (s0 + q < nseg) P.seg_row[s0 + q] = run;
+      run += v[q];
+    }
     carry += tot;
   }
   if (sparse) atomicOr(&sh_sparse, 1);
+  __threadfence();
   __syncthreads();
-  for (int64_t i = threadIdx.x; i < P.n_instances; i += blockDim.x) {
+  for (int64_t i = tid; i < P.n_instances; i += nt) {
     const InstanceDesc in = P.inst[i];
-    int64_t k = 0;
-    const int64_t s0 = int64_t(in.first_chunk) * P.waves, s1 = s0 + int64_t(in.n_chunks) * P.waves;
-    for (int64_t s = s0; s < s1; ++s) k += P.seg_kept[s];
-    P.kept[i] = k;
+    const int64_t s0 = int64_t(in.first_chunk) * P.waves;
+    const int64_t s1 = s0 + int64_t(in.n_chunks) * P.waves;
+    P.kept[i] = (s1 < nseg ? P.seg_row[s1] : carry) - P.seg_row[s0];
   }
-  if (threadIdx.x == 0) {
-    P.total[0] = static_cast<int64_t>(carry);
+  if (tid == 0) {
+    P.total[0] = carry;
     P.flags[0] = sh_sparse ? 0 : 1;  // dense
   }
 }
@@ -1156,15 +1165,16 @@ __global__ void __launch_bounds__(1024) vjp_kernel(BatchedParams P, const Real* 
     part[i] = h0[i];
     part[nb + i] = h1[i];
   }
-  __threadfence();
+  // one device-scope fence per work-group (each is an L2 write-back)
   __syncthreads();
   if (tid == 0) {
+    __threadfence();
     const uint32_t old = atomicAdd(&P.done[ch.instance], 1u);
     sh_last = old == uint32_t(in.n_chunks - 1);
+    if (sh_last) __threadfence();
   }
   __syncthreads();
   if (!sh_last) return;
-  __threadfence();
   for (int64_t i = tid; i < n; i += nt) {
     double a = 0.0, b = 0.0;
     for (int cc = in.first_chunk; cc < in.first_chunk + in.n_chunks; ++cc) {

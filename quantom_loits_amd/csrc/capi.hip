@@ -87,6 +87,8 @@ int make_plan(const loits_hip_instance* instances, int64_t n, int dtype,
     return fail(LOITS_HIP_EINVAL, "batched: threads must be 256, 512 or 1024");
   if (tuning->guide_shift < 0 || tuning->guide_shift > 3)
     return fail(LOITS_HIP_EINVAL, "batched: guide_shift must be in [0, 3]");
+  if (tuning->persistent != 0)
+    return fail(LOITS_HIP_EINVAL, "batched: persistent is reserved and must be 0");
   const int64_t per_iter = int64_t(4) * threads;
   int64_t chunk_events = tuning->chunk_events;
   if (chunk_events <= 0 || chunk_events > (int64_t(1) << 30)) chunk_events = int64_t(1) << 30;
