@@ -287,8 +287,11 @@ __device__ void build_tables(const BatchedParams& P, int inst, const InstanceDes
       if (!__builtin_isfinite(m)) bits &= ~2u;
       if constexpr (sizeof(Real) == 4) {
         const double r = double(thr) * 0x1.0p-24 - c;
-        L.coef[t] = CoefF32{thr, static_cast<float>(m * 0x1.0p-24),
-                            static_cast<float>(__builtin_fma(m, r, xt)), static_cast<float>(r)};
+        const CoefF32View cv = coef_f32_view(L.coef, pad4(n));
+        cv.ma[t] = make_float2(static_cast<float>(m * 0x1.0p-24),
+                               static_cast<float>(__builtin_fma(m, r, xt)));
+        cv.thr[t] = thr;
+        cv.r[t] = static_cast<float>(r);
       } else {
         L.coef[t] = CoefF64{c, m, xt, thr, 0u};
       }
@@ -341,62 +344,39 @@ __device__ void build_tables(const BatchedParams& P, int inst, const InstanceDes
   __syncthreads();
   const int per = int(G / nt);  // G >= 4096 and nt <= 1024 are powers of two
   const int64_t g0 = int64_t(tid) * per;
-  uint32_t local = 0u;
-  for (int i = 0; i < per; i += 4) {
-    const uint4 v = *reinterpret_cast<const uint4*>(hist + g0 + i);
-    local += v.x + v.y + v.z + v.w;
-  }
-  uint32_t in_cells;
-  const uint32_t run0 =
-      block_exclusive_scan(local, reinterpret_cast<uint32_t*>(sh.scratch), &in_cells);
-  // A cell whose thresholds do not sit at one key changing the bin by at most
-  // one needs a search word (guide_encode); h < 2 never does.
   const int nn = int(n);
-  auto crowded = [&](uint32_t c0, uint32_t h) -> bool {
-    if (h < 2u) return false;
-    const int j0 = int(c0) - 1 < 0 ? 0 : (int(c0) - 1 > nn - 2 ? nn - 2 : int(c0) - 1);
-    const int c1m = int(c0 + h) - 1;
-    const int j1 = c1m > nn - 2 ? nn - 2 : c1m;
-    if (j1 == j0) return false;
-    return !(j1 - j0 == 1 && L.srt[c0] == L.srt[c1m]);
-  };
-
-  // 5. level-1 words.  Crowded cells get 2^f level-2 sub-cells each, in cell
-  //    order, while the G level-2 words last.
-  //    Most cells hold at most one threshold and take the branch-free direct
-  //    word; cells with several run guide_encode and the crowding logic.
-  auto jof = [&](uint32_t c) -> int {
-    const int j = int(c) - 1;
-    return j < 0 ? 0 : (j > nn - 2 ? nn - 2 : j);
-  };
-  uint32_t crowd_n = 0u, need = 0u, run = run0;
+  // Cells holding two or more thresholds are "crowded": they get 2^f level-2
+  // sub-cells each, in cell order, while the G level-2 words last (any cell
+  // may take the level-2 or search form; direct words are an optimisation).
+  // One scan of (crowded cells | thresholds << 16 | level-2 words << 32).
+  uint64_t local = 0u;
   for (int i = 0; i < per; i += 4) {
     const uint4 v = *reinterpret_cast<const uint4*>(hist + g0 + i);
-    if ((v.x | v.y | v.z | v.w) > 1u) {  // some cell holds two or more
-      const uint32_t hv[4] = {v.x, v.y, v.z, v.w};
-      uint32_t c = run;
+    const uint32_t hv[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        if (crowded(c, hv[q])) {
-          ++crowd_n;
-          need += 1u << guide_fanout_log2(int(hv[q]), shift);
-        }
-        c += hv[q];
-      }
+    for (int q = 0; q < 4; ++q) {
+      local += uint64_t(hv[q]) << 16;
+      if (hv[q] >= 2u)
+        local += 1u + (uint64_t(1u << guide_fanout_log2(int(hv[q]), shift)) << 32);
     }
-    run += v.x + v.y + v.z + v.w;
   }
   uint64_t tot64;
-  const uint64_t ex = block_exclusive_scan((uint64_t(need) << 32) | crowd_n,
-                                           reinterpret_cast<uint64_t*>(sh.scratch), &tot64);
-  uint32_t rank = uint32_t(ex), woff = uint32_t(ex >> 32);
+  const uint64_t ex =
+      block_exclusive_scan(local, reinterpret_cast<uint64_t*>(sh.scratch), &tot64);
+  const uint32_t run0 = uint32_t(ex >> 16) & 0xFFFFu;
+  uint32_t rank = uint32_t(ex) & 0xFFFFu, woff = uint32_t(ex >> 32);
   // Redirected cells are the crowded cells before the first one that would
   // overrun level 2; that cell's thread (or thread 0 when all fit) records
   // their number and the level-2 words they use.
   if (tid == 0 && (tot64 >> 32) <= uint64_t(G)) {
-    sh.assigned = int(uint32_t(tot64));
+    sh.assigned = int(uint32_t(tot64) & 0xFFFFu);
     sh.used = int(tot64 >> 32);
   }
+  auto jof = [&](uint32_t c) -> int {
+    const int j = int(c) - 1;
+    return j < 0 ? 0 : (j > nn - 2 ? nn - 2 : j);
+  };
+  uint32_t run;
   run = run0;
   for (int i = 0; i < per; i += 4) {
     const uint4 v = *reinterpret_cast<const uint4*>(hist + g0 + i);
@@ -418,23 +398,19 @@ __device__ void build_tables(const BatchedParams& P, int inst, const InstanceDes
       for (int q = 0; q < 4; ++q) {
         const uint32_t c0 = c, h = hv[q], g = uint32_t(g0 + i + q);
         if (h >= 2u) {
-          if (!crowded(c0, h)) {
-            wv[q] = guide_encode(int(c0), int(c0 + h), L.srt, g << shift, nn);
+          const int fl = guide_fanout_log2(int(h), shift);
+          if (woff + (1u << fl) <= uint32_t(G)) {
+            L.crowd[rank] = make_uint2(c0 | ((c0 + h) << 16), woff | (g << 15));
+            wv[q] = kSpecial | (uint32_t(fl) << 24) | uint32_t(G + woff);
           } else {
-            const int fl = guide_fanout_log2(int(h), shift);
-            if (woff + (1u << fl) <= uint32_t(G)) {
-              L.crowd[rank] = make_uint2(c0 | ((c0 + h) << 16), woff | (g << 15));
-              wv[q] = kSpecial | (uint32_t(fl) << 24) | uint32_t(G + woff);
-            } else {
-              if (woff <= uint32_t(G)) {  // the first cell that does not fit
-                sh.assigned = int(rank);
-                sh.used = int(woff);
-              }
-              wv[q] = kSearch | c0 | (h << 15);
+            if (woff <= uint32_t(G)) {  // the first cell that does not fit
+              sh.assigned = int(rank);
+              sh.used = int(woff);
             }
-            woff += 1u << fl;
-            ++rank;
+            wv[q] = kSearch | c0 | (h << 15);
           }
+          woff += 1u << fl;
+          ++rank;
         }
         c += h;
       }
@@ -527,44 +503,51 @@ __global__ void __launch_bounds__(1024) tables_kernel(BatchedParams P) {
 // Interval search: j = clamp(#{t : srt_t <= k} - 1, 0, n - 2).  One guide word
 // settles almost every key; keys in redirected cells read one level-2 word,
 // and the rare crowded (sub-)cells bisect the sorted thresholds.
-__device__ __forceinline__ void find_bins4(const uint32_t (&k)[4], const uint32_t* guide,
-                                           const uint32_t* srt, int shift, int n,
-                                           int (&j)[4]) {
+template <int NK>
+__device__ __forceinline__ void find_bins(const uint32_t (&k)[NK], const uint32_t* guide,
+                                          const uint32_t* srt, int shift, int n, int (&j)[NK]) {
   const uint32_t cmask = (1u << shift) - 1u;
-  uint32_t w[4];
+  uint32_t w[NK];
 #pragma unroll
-  for (int e = 0; e < 4; ++e) w[e] = guide[k[e] >> shift];
+  for (int e = 0; e < NK; ++e) w[e] = guide[k[e] >> shift];
+  uint32_t any = 0u;
 #pragma unroll
-  for (int e = 0; e < 4; ++e) j[e] = guide_direct_bin(w[e], k[e] & cmask);
-  if (__builtin_expect(((w[0] | w[1] | w[2] | w[3]) & kSpecial) != 0, 0)) {
+  for (int e = 0; e < NK; ++e) {
+    j[e] = guide_direct_bin(w[e], k[e] & cmask);
+    any |= w[e];
+  }
+  if (__builtin_expect((any & kSpecial) != 0, 0)) {
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
+    for (int e = 0; e < NK; ++e) {
       if ((w[e] & kSearch) == kSpecial) {  // indirect
         const int ss = shift - int((w[e] >> 24) & 0xFu);
         w[e] = guide[(w[e] & 0xFFFFFFu) + ((k[e] & cmask) >> ss)];
         j[e] = guide_direct_bin(w[e], k[e] & ((1u << ss) - 1u));
       }
     }
-    int lo[4], len[4];
+    int lo[NK], len[NK], pending = 0;
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
+    for (int e = 0; e < NK; ++e) {
       const bool search = (w[e] & kSearch) == kSearch;
       lo[e] = int(w[e] & 0x7FFFu);
       len[e] = search ? int((w[e] >> 15) & 0x7FFFu) : 0;
+      pending |= len[e];
     }
-    if ((len[0] | len[1] | len[2] | len[3]) != 0) {
-      while ((len[0] | len[1] | len[2] | len[3]) != 0) {
+    if (pending != 0) {
+      while (pending != 0) {
+        pending = 0;
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
+        for (int e = 0; e < NK; ++e) {
           const int half = len[e] >> 1;
           const int idx = lo[e] + half;  // < lo + len <= n while len > 0
           const bool le = len[e] > 0 && srt[len[e] > 0 ? idx : 0] <= k[e];
           lo[e] = le ? idx + 1 : lo[e];
           len[e] = len[e] > 0 ? (le ? len[e] - half - 1 : half) : 0;
+          pending |= len[e];
         }
       }
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
+      for (int e = 0; e < NK; ++e) {
         if ((w[e] & kSearch) == kSearch) {
           const int jj = lo[e] - 1;
           j[e] = jj < 0 ? 0 : (jj > n - 2 ? n - 2 : jj);
@@ -572,6 +555,12 @@ __device__ __forceinline__ void find_bins4(const uint32_t (&k)[4], const uint32_
       }
     }
   }
+}
+
+__device__ __forceinline__ void find_bins4(const uint32_t (&k)[4], const uint32_t* guide,
+                                           const uint32_t* srt, int shift, int n,
+                                           int (&j)[4]) {
+  find_bins<4>(k, guide, srt, shift, n, j);
 }
 
 template <bool WINDOW>
@@ -596,31 +585,36 @@ __device__ __noinline__ double interp_reference(const double* cdf, const Real* x
 // formula, u - C_j = (k - T_j) 2^-24 + r_j; a value whose keep decision could
 // differ from the fp64 decision (|v| < 1e-30 or > 1e30, non-finite, or within
 // 2e-6 |v| of a window edge) is re-evaluated in fp64 from the global tables.
-template <typename Real, bool WINDOW>
-__device__ __forceinline__ int slot_values(const uint32_t (&k)[4], const int (&j)[4],
+// tarr: the bins' own thresholds T_j (the sorted array when it is the same).
+template <typename Real, bool WINDOW, int NK>
+__device__ __forceinline__ int slot_values(const uint32_t (&k)[NK], const int (&j)[NK],
                                            const typename CoefOf<Real>::type* coef,
-                                           const double* cdf_g, const Real* x_g,
-                                           const BatchedParams& P, bool safe_curve,
-                                           Real (&v)[4]) {
-  using Coef = typename CoefOf<Real>::type;
-  Coef cf[4];
-#pragma unroll
-  for (int e = 0; e < 4; ++e) cf[e] = coef[j[e]];
+                                           const uint32_t* tarr, const double* cdf_g,
+                                           const Real* x_g, const BatchedParams& P,
+                                           bool safe_curve, Real (&v)[NK]) {
   int keep = 0;
   if constexpr (sizeof(Real) == 4) {
+    const float2* ma = reinterpret_cast<const float2*>(coef);
+    float2 cm[NK];
+    uint32_t ct[NK];
+#pragma unroll
+    for (int e = 0; e < NK; ++e) {
+      cm[e] = ma[j[e]];
+      ct[e] = tarr[j[e]];
+    }
     if (!WINDOW && safe_curve) {  // every value of this curve is kept (kSafe)
 #pragma unroll
-      for (int e = 0; e < 4; ++e)
-        v[e] = __builtin_fmaf(cf[e].mk, static_cast<float>(static_cast<int32_t>(k[e] - cf[e].thr)),
-                              cf[e].a);
-      return 0xF;
+      for (int e = 0; e < NK; ++e)
+        v[e] = __builtin_fmaf(cm[e].x, static_cast<float>(static_cast<int32_t>(k[e] - ct[e])),
+                              cm[e].y);
+      return (1 << NK) - 1;
     }
     const float lo = static_cast<float>(P.win_lo), hi = static_cast<float>(P.win_hi);
     int unsafe = 0;
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      v[e] = __builtin_fmaf(cf[e].mk, static_cast<float>(static_cast<int32_t>(k[e] - cf[e].thr)),
-                            cf[e].a);
+    for (int e = 0; e < NK; ++e) {
+      v[e] = __builtin_fmaf(cm[e].x, static_cast<float>(static_cast<int32_t>(k[e] - ct[e])),
+                            cm[e].y);
       const float a = __builtin_fabsf(v[e]);
       bool safe = a >= 1e-30f && a <= 1e30f;
       bool in = true;
@@ -634,7 +628,7 @@ __device__ __forceinline__ int slot_values(const uint32_t (&k)[4], const int (&j
     }
     if (__builtin_expect(unsafe != 0, 0)) {
 #pragma unroll 1
-      for (int e = 0; e < 4; ++e) {
+      for (int e = 0; e < NK; ++e) {
         if ((unsafe >> e) & 1) {
           const double v64 = interp_reference(cdf_g, x_g, j[e], k[e]);
           v[e] = static_cast<float>(v64);
@@ -643,8 +637,11 @@ __device__ __forceinline__ int slot_values(const uint32_t (&k)[4], const int (&j
       }
     }
   } else {
+    CoefF64 cf[NK];
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
+    for (int e = 0; e < NK; ++e) cf[e] = coef[j[e]];
+#pragma unroll
+    for (int e = 0; e < NK; ++e) {
       v[e] = __builtin_fma(cf[e].m, double(k[e]) * 0x1.0p-24 - cf[e].c, cf[e].x);
       keep |= keep_value<WINDOW>(v[e], P) ? (1 << e) : 0;
     }
@@ -698,21 +695,19 @@ __global__ void __launch_bounds__(1024) sample_kernel(BatchedParams P) {
   const int64_t per_wave = int64_t(blocks_per_wave(uint64_t(chunk_blocks(in, ch)), nw) / 64);
   uint64_t* mask = P.keepmask + ch.mask_off + int64_t(w) * per_wave * 4;
   const bool safe_curve = (sh.flags & kSafe) != 0;
+  const uint32_t* s_thr = (sh.flags & kMonotone) != 0 || sizeof(Real) != 4
+                              ? s_srt
+                              : coef_f32_view(L.coef, pad4(n)).thr;
   // 32-bit slot positions relative to counter 4 qa: live slots are [off0, span)
   const uint32_t nblk = uint32_t(sg.qb - sg.qa);
   const uint32_t off0 = uint32_t(sg.p_lo - 4 * sg.qa);
   const uint32_t span = uint32_t(sg.p_hi - 4 * sg.qa);
   int64_t running = 0;
   int first_drop = int(sg.iters);  // iterations before this one kept every slot
-  int it = 0;
-  for (uint32_t i0 = 0; i0 < nblk; i0 += 64, ++it) {
+
+  // Keep bits, rows and stores of one iteration (four slots per lane).
+  auto emit = [&](int it, uint32_t i0, const Real* v, const int* j, int keep) {
     const uint32_t i = i0 + uint32_t(lane);
-    const Words4 wd = philox4x32_10_uniform(sg.qa + i, in.stream, P.seed);
-    const uint32_t k[4] = {wd.w0 >> 8, wd.w1 >> 8, wd.w2 >> 8, wd.w3 >> 8};
-    int j[4];
-    find_bins4(k, s_guide, s_srt, shift, n, j);
-    Real v[4];
-    const int keep = slot_values<Real, WINDOW>(k, j, s_coef, L.cdf, x_g, P, safe_curve, v);
     const bool full_iter = (i0 > 0 || off0 == 0) && 4 * (i0 + 64) <= span;
     int live = 0xF;
     if (!full_iter) {
@@ -737,8 +732,8 @@ __global__ void __launch_bounds__(1024) sample_kernel(BatchedParams P) {
       if (first_drop == int(sg.iters)) first_drop = it;
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        const uint64_t b = ballot((keepbits >> e) & 1);
-        if (lane == e) mask[int64_t(it) * 4 + e] = b;
+        const uint64_t bb = ballot((keepbits >> e) & 1);
+        if (lane == e) mask[int64_t(it) * 4 + e] = bb;
       }
       row = running + wave_scan_counts(__popc(keepbits), &total);
     }
@@ -761,6 +756,23 @@ __global__ void __launch_bounds__(1024) sample_kernel(BatchedParams P) {
       }
     }
     running += total;
+  };
+
+  // Two iterations per trip: their Philox, search and interpolation chains are
+  // independent, which hides LDS latency at four waves per SIMD.
+  int it = 0;
+  for (uint32_t i0 = 0; i0 < nblk; i0 += 128, it += 2) {
+    const Words4 wa = philox4x32_10_uniform(sg.qa + i0 + uint32_t(lane), in.stream, P.seed);
+    const Words4 wb = philox4x32_10_uniform(sg.qa + i0 + 64 + uint32_t(lane), in.stream, P.seed);
+    const uint32_t k[8] = {wa.w0 >> 8, wa.w1 >> 8, wa.w2 >> 8, wa.w3 >> 8,
+                           wb.w0 >> 8, wb.w1 >> 8, wb.w2 >> 8, wb.w3 >> 8};
+    int j[8];
+    find_bins<8>(k, s_guide, s_srt, shift, n, j);
+    Real v[8];
+    const int keep =
+        slot_values<Real, WINDOW, 8>(k, j, s_coef, s_thr, L.cdf, x_g, P, safe_curve, v);
+    emit(it, i0, v, j, keep & 0xF);
+    if (i0 + 64 < nblk) emit(it + 1, i0 + 64, v + 4, j + 4, keep >> 4);
   }
   if (lane == 0) {
     P.seg_kept[seg] = running;
@@ -801,8 +813,7 @@ __global__ void __launch_bounds__(1024) finalize_kernel(BatchedParams P) {
     carry += tot;
   }
   if (sparse) atomicOr(&sh_sparse, 1);
-  __threadfence();
-  __syncthreads();
+  __syncthreads();  // seg_row written above is visible to the whole work-group
   for (int64_t i = tid; i < P.n_instances; i += nt) {
     const InstanceDesc in = P.inst[i];
     const int64_t s0 = int64_t(in.first_chunk) * P.waves;
@@ -907,15 +918,10 @@ __device__ void vjp_tail(const BatchedParams& P, const InstanceDesc& in, const T
   }
 }
 
-// Per-bin data of the fixed-point VJP: t / (tau_j d_j) = float(k - T_j) * alpha_j
-// + beta_j lies in [0, 1) for every key of bin j of a monotone curve (tau_j = 1
-// except for the last bin, which also takes the keys above T_{n-1}).
-struct alignas(16) VjpBin {
-  uint32_t thr;
-  float alpha;
-  float beta;
-  float pad;
-};
+// Per-bin data of the fixed-point VJP, {alpha_j, beta_j}: t / (tau_j d_j) =
+// float(k - T_j) * alpha_j + beta_j lies in [0, 1) for every key of bin j of a
+// monotone curve (tau_j = 1 except for the last bin, which also takes the keys
+// above T_{n-1}); T_j is the sorted threshold array itself.
 
 __device__ __forceinline__ double vjp_tau_d(const double* cdf, int64_t j, int64_t n) {
   const double d = cdf[j + 1] - cdf[j] + kEpsilon;
@@ -961,13 +967,11 @@ __global__ void __launch_bounds__(1024) vjp_kernel(BatchedParams P, const Real* 
   uint64_t* a1 = reinterpret_cast<uint64_t*>(h1);
   double* outl = P.partials + ch.partial_off;  // [2 nb] fp64 overflow sums, then partials
   if (fixed) {
-    VjpBin* vb = reinterpret_cast<VjpBin*>(s_bin);
+    float2* vb = reinterpret_cast<float2*>(s_bin);
     for (int64_t j = tid; j < n - 1; j += nt) {
       const double tau_d = vjp_tau_d(T.cdf, j, n);
-      const uint32_t thr = T.coef[j].thr;
-      const double r = double(thr) * 0x1.0p-24 - T.cdf[j];
-      vb[j] = VjpBin{thr, static_cast<float>(0x1.0p-24 / tau_d), static_cast<float>(r / tau_d),
-                     0.0f};
+      const double r = double(T.srt[j]) * 0x1.0p-24 - T.cdf[j];
+      vb[j] = make_float2(static_cast<float>(0x1.0p-24 / tau_d), static_cast<float>(r / tau_d));
     }
   } else {
     lds_copy(s_bin, T.coef, nb * int64_t(sizeof(Coef)));
@@ -981,7 +985,7 @@ __global__ void __launch_bounds__(1024) vjp_kernel(BatchedParams P, const Real* 
   const uint32_t* s_srt = reinterpret_cast<const uint32_t*>(lds);
   const uint32_t* s_guide = reinterpret_cast<const uint32_t*>(lds + (T.L.guide - T.L.thr));
   const Coef* s_coef = reinterpret_cast<const Coef*>(s_bin);
-  const VjpBin* s_vb = reinterpret_cast<const VjpBin*>(s_bin);
+  const float2* s_vb = reinterpret_cast<const float2*>(s_bin);
 
   const int nw = nt >> 6, w = tid >> 6, lane = lane_id();
   const Segment sg = segment_of(in.counter_base, ch.slot_lo, ch.slot_hi, w, nw);
@@ -1051,14 +1055,11 @@ __global__ void __launch_bounds__(1024) vjp_kernel(BatchedParams P, const Real* 
     }
   };
 
-  int it = 0;
-  for (uint32_t i0 = 0; i0 < nblk; i0 += 64, ++it) {
+  // keep bits and first packed row of a lane's four slots in iteration `it`
+  auto rows_of = [&](int it, uint32_t i0, int& keepbits, int64_t& row) {
     const uint32_t i = i0 + uint32_t(lane);
-    const Words4 wd = philox4x32_10_uniform(sg.qa + i, in.stream, P.seed);
-    const uint32_t k[4] = {wd.w0 >> 8, wd.w1 >> 8, wd.w2 >> 8, wd.w3 >> 8};
     const bool full_iter = (i0 > 0 || off0 == 0) && 4 * (i0 + 64) <= span;
-    int keepbits = 0xF;
-    int64_t row;
+    keepbits = 0xF;
     if (it < first_drop) {  // every live slot of these iterations was kept
       if (!full_iter) {
         keepbits = 0;
@@ -1079,61 +1080,78 @@ __global__ void __launch_bounds__(1024) vjp_kernel(BatchedParams P, const Real* 
       row = row0 + running + wave_scan_counts(__popc(keepbits), &total);
       running += total;
     }
-    int j[4];
-    find_bins4(k, s_guide, s_srt, shift, n, j);
+  };
+
+  // Two iterations per trip (independent Philox / search chains).
+  int it = 0;
+  for (uint32_t i0 = 0; i0 < nblk; i0 += 128, it += 2) {
+    const bool has_b = i0 + 64 < nblk;
+    const Words4 wa = philox4x32_10_uniform(sg.qa + i0 + uint32_t(lane), in.stream, P.seed);
+    const Words4 wb = philox4x32_10_uniform(sg.qa + i0 + 64 + uint32_t(lane), in.stream, P.seed);
+    const uint32_t k[8] = {wa.w0 >> 8, wa.w1 >> 8, wa.w2 >> 8, wa.w3 >> 8,
+                           wb.w0 >> 8, wb.w1 >> 8, wb.w2 >> 8, wb.w3 >> 8};
+    int kb[2];
+    int64_t rw[2];
+    rows_of(it, i0, kb[0], rw[0]);
+    kb[1] = 0;
+    rw[1] = 0;
+    if (has_b) rows_of(it + 1, i0 + 64, kb[1], rw[1]);
+    int j[8];
+    find_bins<8>(k, s_guide, s_srt, shift, n, j);
     if (fixed) {
-      float th[4];
+      float th[8];
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const VjpBin vb = s_vb[j[e]];
-        th[e] = __builtin_fmaf(static_cast<float>(static_cast<int32_t>(k[e] - vb.thr)), vb.alpha,
-                               vb.beta);
+      for (int e = 0; e < 8; ++e) {
+        const float2 ab = s_vb[j[e]];
+        th[e] = __builtin_fmaf(static_cast<float>(static_cast<int32_t>(k[e] - s_srt[j[e]])), ab.x,
+                               ab.y);
       }
-      if (keepbits == 0xF) {
-        float g[4];
-        if ((row & 3) == 0) {
-          const float4 g4 = *reinterpret_cast<const float4*>(grad + row);
-          g[0] = g4.x; g[1] = g4.y; g[2] = g4.z; g[3] = g4.w;
+      float g[8];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        if (kb[h] == 0xF && (rw[h] & 3) == 0) {
+          const float4 g4 = *reinterpret_cast<const float4*>(grad + rw[h]);
+          g[4 * h] = g4.x; g[4 * h + 1] = g4.y; g[4 * h + 2] = g4.z; g[4 * h + 3] = g4.w;
+        } else {
+          int64_t r = rw[h];
+#pragma unroll
+          for (int e = 0; e < 4; ++e) g[4 * h + e] = ((kb[h] >> e) & 1) ? float(grad[r++]) : 0.0f;
+        }
+      }
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        if (kb[h] == 0xF) {
+#pragma unroll
+          for (int e = 0; e < 4; ++e) add_fixed(j[4 * h + e], g[4 * h + e], th[4 * h + e]);
         } else {
 #pragma unroll
-          for (int e = 0; e < 4; ++e) g[e] = float(grad[row + e]);
-        }
-#pragma unroll
-        for (int e = 0; e < 4; ++e) add_fixed(j[e], g[e], th[e]);
-      } else {
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          if (!((keepbits >> e) & 1)) continue;
-          add_fixed(j[e], float(grad[row++]), th[e]);
+          for (int e = 0; e < 4; ++e)
+            if ((kb[h] >> e) & 1) add_fixed(j[4 * h + e], g[4 * h + e], th[4 * h + e]);
         }
       }
     } else {
-      double t[4], g[4];
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        if constexpr (sizeof(Real) == 4) {
-          const CoefF32& cf = s_coef[j[e]];
-          t[e] = double(static_cast<float>(static_cast<int32_t>(k[e] - cf.thr)) * 0x1.0p-24f +
-                        cf.r);
-        } else {
-          t[e] = double(k[e]) * 0x1.0p-24 - s_coef[j[e]].c;
-        }
-      }
-      if (keepbits == 0xF) {
-#pragma unroll
-        for (int e = 0; e < 4; ++e) g[e] = double(grad[row + e]);
+      for (int h = 0; h < 2; ++h) {
+        double t[4];
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-          atomicAdd(&h0[j[e]], g[e]);
-          atomicAdd(&h1[j[e]], g[e] * t[e]);
+          const int je = j[4 * h + e];
+          if constexpr (sizeof(Real) == 4) {
+            const CoefF32View cv = coef_f32_view(const_cast<uint8_t*>(s_bin), nb);
+            t[e] = double(static_cast<float>(static_cast<int32_t>(k[4 * h + e] - cv.thr[je])) *
+                              0x1.0p-24f +
+                          cv.r[je]);
+          } else {
+            t[e] = double(k[4 * h + e]) * 0x1.0p-24 - s_coef[je].c;
+          }
         }
-      } else {
+        int64_t r = rw[h];
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-          if (!((keepbits >> e) & 1)) continue;
-          const double ge = double(grad[row++]);
-          atomicAdd(&h0[j[e]], ge);
-          atomicAdd(&h1[j[e]], ge * t[e]);
+          if (!((kb[h] >> e) & 1)) continue;
+          const double ge = double(grad[r++]);
+          atomicAdd(&h0[j[4 * h + e]], ge);
+          atomicAdd(&h1[j[4 * h + e]], ge * t[e]);
         }
       }
     }

@@ -298,13 +298,25 @@ constexpr uint32_t kNoPoint = 0x1000u;
 // fp32 path: value = A_j + M_j * (k - T_j) with M_j = m_j 2^-24 and
 // A_j = x_j + m_j (T_j 2^-24 - C_j), both rounded from fp64, which is
 // x_j + m_j (u - C_j) for u = k 2^-24; r_j = T_j 2^-24 - C_j rounded to float
-// gives u - C_j = (k - T_j) 2^-24 + r_j for the VJP.
-struct alignas(16) CoefF32 {
-  uint32_t thr;
-  float mk;
-  float a;
-  float r;
+// gives u - C_j = (k - T_j) 2^-24 + r_j for the VJP.  Stored as a structure
+// of arrays over nb = pad4(n) bins (a random 8-byte LDS read costs what a
+// 4-byte one does, a 12-byte one three times that):
+//   float2 ma[nb] = {M_j, A_j} | uint32 thr[nb] = T_j | float r[nb] = r_j
+struct alignas(16) CoefF32 {  // one bin's share of the arrays (sizing only)
+  uint32_t w[4];
 };
+struct CoefF32View {
+  float2* ma;
+  uint32_t* thr;
+  float* r;
+};
+__host__ __device__ __forceinline__ CoefF32View coef_f32_view(void* base, int64_t nb) {
+  CoefF32View v;
+  v.ma = static_cast<float2*>(base);
+  v.thr = reinterpret_cast<uint32_t*>(v.ma + nb);
+  v.r = reinterpret_cast<float*>(v.thr + nb);
+  return v;
+}
 // fp64 path: the reference's own operands.
 struct alignas(16) CoefF64 {
   double c;
