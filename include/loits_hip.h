@@ -199,6 +199,9 @@ typedef struct loits_hip_layout {
   int64_t partial_doubles;  /* partial VJP sums in scratch (multi-chunk instances) */
   int32_t dtype, threads, scan_mode, persistent;
   int32_t grid_forward, grid_backward;
+  int32_t global_instances; /* instances whose per-bin tables stay in global memory
+                               (too large for LDS; built by a pre-pass, slower) */
+  int32_t reserved;
 } loits_hip_layout;
 
 /* Host only: validate `instances` and compute sizes for `dtype`. */

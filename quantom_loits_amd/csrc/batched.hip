@@ -152,14 +152,30 @@ __device__ __forceinline__ LdsTables<Real> lds_tables_of(uint8_t* lds, const Ins
   return t;
 }
 
-template <typename Real, bool SEQUENTIAL>
+// GLOBAL: srt, coef and cdf are built in place in the global blob (instances
+// whose LDS image does not fit); LDS then holds the crowded-cell list and the
+// cell counts only.
+template <typename Real, bool SEQUENTIAL, bool GLOBAL = false>
 __device__ void build_tables(const BatchedParams& P, int inst, const InstanceDesc& in,
                              uint8_t* lds, bool writer, BuildShared& sh) {
   using Coef = typename CoefOf<Real>::type;
   const int64_t n = in.n_nodes;
   const Real* x = static_cast<const Real*>(P.x) + in.node_off;
   const Real* s = static_cast<const Real*>(P.density) + in.node_off;
-  const LdsTables<Real> L = lds_tables_of<Real>(lds, in);
+  LdsTables<Real> L;
+  uint32_t* hist;  // per-cell threshold counts (step 4)
+  if constexpr (GLOBAL) {
+    const Tables<Real> T = tables_of<Real>(P.tables, in);
+    L.srt = T.srt;
+    L.guide = T.guide;
+    L.coef = T.coef;
+    L.cdf = T.cdf;
+    L.crowd = reinterpret_cast<uint2*>(lds);
+    hist = reinterpret_cast<uint32_t*>(lds + ((int64_t(1) << in.guide_log2) / 4) * 8);
+  } else {
+    L = lds_tables_of<Real>(lds, in);
+    hist = L.guide + (int64_t(1) << in.guide_log2);  // the level-2 region, free until step 6
+  }
   const double acc = in.accept ? 1.0 : 0.0;
   const int tid = threadIdx.x, nt = blockDim.x;
   if (tid == 0) {
@@ -334,7 +350,6 @@ __device__ void build_tables(const BatchedParams& P, int inst, const InstanceDes
   //    of the counts gives c0 = #{t : srt_t < g << shift} for its first cell.
   const int64_t G = int64_t(1) << in.guide_log2;
   const int shift = 24 - in.guide_log2;
-  uint32_t* hist = L.guide + G;
   for (int64_t g = tid; g < G; g += nt) hist[g] = 0u;
   __syncthreads();
   for (int64_t t = tid; t < n; t += nt) {
@@ -473,10 +488,12 @@ __device__ void build_tables(const BatchedParams& P, int inst, const InstanceDes
   // 7. publish
   if (writer) {
     Tables<Real> T = tables_of<Real>(P.tables, in);
-    lds_copy(T.srt, L.srt, pad4(n) * 4);
-    lds_copy(T.guide, L.guide, 2 * G * 4);
-    lds_copy(T.coef, L.coef, pad4(n) * int64_t(sizeof(Coef)));
-    lds_copy(T.cdf, L.cdf, pad4(n) * 8);
+    if constexpr (!GLOBAL) {
+      lds_copy(T.srt, L.srt, pad4(n) * 4);
+      lds_copy(T.guide, L.guide, 2 * G * 4);
+      lds_copy(T.coef, L.coef, pad4(n) * int64_t(sizeof(Coef)));
+      lds_copy(T.cdf, L.cdf, pad4(n) * 8);
+    }
     if (P.cdf_out)
       for (int64_t t = tid; t < n; t += nt) P.cdf_out[in.node_off + t] = L.cdf[t];
     if (tid == 0) {
@@ -491,12 +508,18 @@ __device__ void build_tables(const BatchedParams& P, int inst, const InstanceDes
   }
 }
 
-// Standalone build (the VJP's recompute path): one work-group per instance.
+// Standalone build: one work-group per instance.  Before sampling it builds
+// the global-table instances (only_global); for the VJP's recompute path it
+// rebuilds every instance's blob.
 template <typename Real, bool SEQUENTIAL>
-__global__ void __launch_bounds__(1024) tables_kernel(BatchedParams P) {
+__global__ void __launch_bounds__(1024) tables_kernel(BatchedParams P, int only_global) {
   extern __shared__ __align__(16) uint8_t lds[];
   __shared__ BuildShared sh;
-  build_tables<Real, SEQUENTIAL>(P, blockIdx.x, P.inst[blockIdx.x], lds, true, sh);
+  const InstanceDesc in = P.inst[blockIdx.x];
+  if (in.global_tables)
+    build_tables<Real, SEQUENTIAL, true>(P, blockIdx.x, in, lds, true, sh);
+  else if (!only_global)
+    build_tables<Real, SEQUENTIAL, false>(P, blockIdx.x, in, lds, true, sh);
 }
 
 // ---------------------------------------------------------------------------
@@ -670,20 +693,18 @@ __device__ __forceinline__ int64_t chunk_blocks(const InstanceDesc& in, const Ch
 // of a segment so far was kept, rows are positional and no keep mask is
 // written; the first iteration with a dropped slot is recorded, and masks are
 // written from that iteration on.
-template <typename Real, bool WINDOW, bool SEQUENTIAL>
-__global__ void __launch_bounds__(1024) sample_kernel(BatchedParams P) {
-  extern __shared__ __align__(16) uint8_t lds[];
-  __shared__ BuildShared sh;
-  const int c = blockIdx.x;
-  const ChunkDesc ch = P.chunks[c];
-  const InstanceDesc in = P.inst[ch.instance];
-  build_tables<Real, SEQUENTIAL>(P, ch.instance, in, lds, ch.index == 0, sh);
-  const LdsTables<Real> L = lds_tables_of<Real>(lds, in);
+// One chunk's sampling loop over tables in LDS or, for global-table instances,
+// a guide in LDS and the per-bin arrays in global memory (forceinline keeps
+// each call's address spaces known to the compiler).
+template <typename Real, bool WINDOW>
+__device__ __forceinline__ void sample_segments(const BatchedParams& P, int c, const ChunkDesc& ch,
+                                                const InstanceDesc& in, const uint32_t* s_guide,
+                                                const uint32_t* s_srt,
+                                                const typename CoefOf<Real>::type* s_coef,
+                                                const uint32_t* s_thr, const double* cdf,
+                                                bool safe_curve) {
   const int n = static_cast<int>(in.n_nodes);
   const int shift = 24 - in.guide_log2;
-  const uint32_t* s_srt = L.srt;
-  const uint32_t* s_guide = L.guide;
-  const auto* s_coef = L.coef;
   const Real* x_g = static_cast<const Real*>(P.x) + in.node_off;
 
   const int nw = blockDim.x >> 6, w = threadIdx.x >> 6, lane = lane_id();
@@ -694,10 +715,6 @@ __global__ void __launch_bounds__(1024) sample_kernel(BatchedParams P) {
   int32_t* bins_out = P.bins ? P.bins + base_row : nullptr;
   const int64_t per_wave = int64_t(blocks_per_wave(uint64_t(chunk_blocks(in, ch)), nw) / 64);
   uint64_t* mask = P.keepmask + ch.mask_off + int64_t(w) * per_wave * 4;
-  const bool safe_curve = (sh.flags & kSafe) != 0;
-  const uint32_t* s_thr = (sh.flags & kMonotone) != 0 || sizeof(Real) != 4
-                              ? s_srt
-                              : coef_f32_view(L.coef, pad4(n)).thr;
   // 32-bit slot positions relative to counter 4 qa: live slots are [off0, span)
   const uint32_t nblk = uint32_t(sg.qb - sg.qa);
   const uint32_t off0 = uint32_t(sg.p_lo - 4 * sg.qa);
@@ -770,7 +787,7 @@ __global__ void __launch_bounds__(1024) sample_kernel(BatchedParams P) {
     find_bins<8>(k, s_guide, s_srt, shift, n, j);
     Real v[8];
     const int keep =
-        slot_values<Real, WINDOW, 8>(k, j, s_coef, s_thr, L.cdf, x_g, P, safe_curve, v);
+        slot_values<Real, WINDOW, 8>(k, j, s_coef, s_thr, cdf, x_g, P, safe_curve, v);
     emit(it, i0, v, j, keep & 0xF);
     if (i0 + 64 < nblk) emit(it + 1, i0 + 64, v + 4, j + 4, keep >> 4);
   }
@@ -778,6 +795,35 @@ __global__ void __launch_bounds__(1024) sample_kernel(BatchedParams P) {
     P.seg_kept[seg] = running;
     P.seg_cap[seg] = int64_t(sg.p_hi - sg.p_lo);
     P.seg_drop[seg] = first_drop;
+  }
+}
+
+template <typename Real, bool WINDOW, bool SEQUENTIAL>
+__global__ void __launch_bounds__(1024) sample_kernel(BatchedParams P) {
+  extern __shared__ __align__(16) uint8_t lds[];
+  __shared__ BuildShared sh;
+  const int c = blockIdx.x;
+  const ChunkDesc ch = P.chunks[c];
+  const InstanceDesc in = P.inst[ch.instance];
+  const int64_t nb = pad4(in.n_nodes);
+  if (in.global_tables) {  // built beforehand by tables_kernel; stage the guide
+    const Tables<Real> T = tables_of<Real>(P.tables, in);
+    lds_copy(lds, T.guide, (int64_t(2) << in.guide_log2) * 4);
+    if (threadIdx.x == 0) sh.flags = T.hdr->flags;
+    __syncthreads();
+    const uint32_t* thr = (sh.flags & kMonotone) != 0 || sizeof(Real) != 4
+                              ? T.srt
+                              : coef_f32_view(T.coef, nb).thr;
+    sample_segments<Real, WINDOW>(P, c, ch, in, reinterpret_cast<const uint32_t*>(lds), T.srt,
+                                  T.coef, thr, T.cdf, (sh.flags & kSafe) != 0);
+  } else {
+    build_tables<Real, SEQUENTIAL>(P, ch.instance, in, lds, ch.index == 0, sh);
+    const LdsTables<Real> L = lds_tables_of<Real>(lds, in);
+    const uint32_t* thr = (sh.flags & kMonotone) != 0 || sizeof(Real) != 4
+                              ? L.srt
+                              : coef_f32_view(L.coef, nb).thr;
+    sample_segments<Real, WINDOW>(P, c, ch, in, L.guide, L.srt, L.coef, thr, L.cdf,
+                                  (sh.flags & kSafe) != 0);
   }
 }
 
@@ -940,11 +986,19 @@ __device__ __forceinline__ double vjp_tau_d(const double* cdf, int64_t j, int64_
 // A value that would reach 2^30 (64x that maximum) or is not finite goes to
 // fp64 atomics in the chunk's global partial sums.  Other curves, and float64,
 // accumulate fp64 directly (ds_add_f64).
-template <typename Real, int MAXPER>
-__global__ void __launch_bounds__(1024) vjp_kernel(BatchedParams P, const Real* grad,
-                                                   Real* grad_density) {
+template <bool GLOBAL>
+__device__ __forceinline__ void add_f64(double* p, double v) {
+  if constexpr (GLOBAL) unsafeAtomicAdd(p, v);
+  else atomicAdd(p, v);
+}
+
+// GLOBAL (global-table instances): only the guide is staged in LDS; srt and
+// coef are read from the blob and the fp64 histograms are the chunk's partial
+// sums in global memory.
+template <typename Real, int MAXPER, bool GLOBAL>
+__device__ __forceinline__ void vjp_body(const BatchedParams& P, const Real* grad,
+                                         Real* grad_density, uint8_t* lds) {
   using Coef = typename CoefOf<Real>::type;
-  extern __shared__ __align__(16) uint8_t lds[];
   __shared__ double scratch[16];
   __shared__ int sh_last;
   __shared__ uint32_t sh_gmax;
@@ -955,17 +1009,32 @@ __global__ void __launch_bounds__(1024) vjp_kernel(BatchedParams P, const Real* 
   const int n = static_cast<int>(in.n_nodes);
   const int shift = 24 - in.guide_log2;
   const int tid = threadIdx.x, nt = blockDim.x;
-  const bool fixed = sizeof(Real) == 4 && (T.hdr->flags & kMonotone) != 0;
+  const bool fixed = !GLOBAL && sizeof(Real) == 4 && (T.hdr->flags & kMonotone) != 0;
 
-  // LDS: srt | guide | per-bin data (coef, or VjpBin when fixed) | h0 | h1
+  // LDS: srt | guide | per-bin data (coef, or {alpha, beta} when fixed) | h0 | h1
+  // (GLOBAL: guide only)
   const int64_t nb = pad4(n);
-  lds_copy(lds, T.srt, T.L.coef - T.L.thr);
-  uint8_t* s_bin = lds + (T.L.coef - T.L.thr);
-  double* h0 = reinterpret_cast<double*>(s_bin + nb * int64_t(sizeof(Coef)));
+  double* outl = P.partials + ch.partial_off;  // [2 nb] fp64 overflow sums, then partials
+  uint8_t* s_bin;
+  double* h0;
+  const uint32_t* s_srt;
+  const uint32_t* s_guide;
+  if constexpr (GLOBAL) {
+    lds_copy(lds, T.guide, (int64_t(2) << in.guide_log2) * 4);
+    s_bin = reinterpret_cast<uint8_t*>(T.coef);
+    h0 = outl;
+    s_srt = T.srt;
+    s_guide = reinterpret_cast<const uint32_t*>(lds);
+  } else {
+    lds_copy(lds, T.srt, T.L.coef - T.L.thr);
+    s_bin = lds + (T.L.coef - T.L.thr);
+    h0 = reinterpret_cast<double*>(s_bin + nb * int64_t(sizeof(Coef)));
+    s_srt = reinterpret_cast<const uint32_t*>(lds);
+    s_guide = reinterpret_cast<const uint32_t*>(lds + (T.L.guide - T.L.thr));
+  }
   double* h1 = h0 + nb;
   uint64_t* a0 = reinterpret_cast<uint64_t*>(h0);
   uint64_t* a1 = reinterpret_cast<uint64_t*>(h1);
-  double* outl = P.partials + ch.partial_off;  // [2 nb] fp64 overflow sums, then partials
   if (fixed) {
     float2* vb = reinterpret_cast<float2*>(s_bin);
     for (int64_t j = tid; j < n - 1; j += nt) {
@@ -973,7 +1042,7 @@ __global__ void __launch_bounds__(1024) vjp_kernel(BatchedParams P, const Real* 
       const double r = double(T.srt[j]) * 0x1.0p-24 - T.cdf[j];
       vb[j] = make_float2(static_cast<float>(0x1.0p-24 / tau_d), static_cast<float>(r / tau_d));
     }
-  } else {
+  } else if (!GLOBAL) {
     lds_copy(s_bin, T.coef, nb * int64_t(sizeof(Coef)));
   }
   for (int64_t i = tid; i < 2 * nb; i += nt) {
@@ -982,8 +1051,6 @@ __global__ void __launch_bounds__(1024) vjp_kernel(BatchedParams P, const Real* 
   }
   if (tid == 0) sh_gmax = 0u;
   __syncthreads();
-  const uint32_t* s_srt = reinterpret_cast<const uint32_t*>(lds);
-  const uint32_t* s_guide = reinterpret_cast<const uint32_t*>(lds + (T.L.guide - T.L.thr));
   const Coef* s_coef = reinterpret_cast<const Coef*>(s_bin);
   const float2* s_vb = reinterpret_cast<const float2*>(s_bin);
 
@@ -1150,13 +1217,17 @@ __global__ void __launch_bounds__(1024) vjp_kernel(BatchedParams P, const Real* 
         for (int e = 0; e < 4; ++e) {
           if (!((kb[h] >> e) & 1)) continue;
           const double ge = double(grad[r++]);
-          atomicAdd(&h0[j[4 * h + e]], ge);
-          atomicAdd(&h1[j[4 * h + e]], ge * t[e]);
+          add_f64<GLOBAL>(&h0[j[4 * h + e]], ge);
+          add_f64<GLOBAL>(&h1[j[4 * h + e]], ge * t[e]);
         }
       }
     }
   }
   __syncthreads();
+  if (GLOBAL) {  // the histograms were summed by device atomics: drop stale L1 lines
+    if (tid == 0) __threadfence();
+    __syncthreads();
+  }
 
   if (fixed) {  // integers (and overflow sums) -> h0 = sum g, h1 = sum g t, in place
     const double inv_scale = __builtin_ldexp(1.0, -scale_log2);
@@ -1178,10 +1249,12 @@ __global__ void __launch_bounds__(1024) vjp_kernel(BatchedParams P, const Real* 
     return;
   }
   // multi-chunk instance: publish partial sums; the last chunk reduces them in order
-  double* part = P.partials + ch.partial_off;
-  for (int64_t i = tid; i < n; i += nt) {
-    part[i] = h0[i];
-    part[nb + i] = h1[i];
+  if (!GLOBAL) {  // (GLOBAL: they are already there)
+    double* part = P.partials + ch.partial_off;
+    for (int64_t i = tid; i < n; i += nt) {
+      part[i] = h0[i];
+      part[nb + i] = h1[i];
+    }
   }
   // one device-scope fence per work-group (each is an L2 write-back)
   __syncthreads();
@@ -1193,7 +1266,7 @@ __global__ void __launch_bounds__(1024) vjp_kernel(BatchedParams P, const Real* 
   }
   __syncthreads();
   if (!sh_last) return;
-  for (int64_t i = tid; i < n; i += nt) {
+  for (int64_t i = tid; i < n; i += nt) {  // each i read and written by one thread
     double a = 0.0, b = 0.0;
     for (int cc = in.first_chunk; cc < in.first_chunk + in.n_chunks; ++cc) {
       const double* pp = P.partials + P.chunks[cc].partial_off;
@@ -1208,16 +1281,27 @@ __global__ void __launch_bounds__(1024) vjp_kernel(BatchedParams P, const Real* 
   vjp_tail<Real, MAXPER>(P, in, T, h0, h1, grad_density, scratch);
 }
 
+template <typename Real, int MAXPER>
+__global__ void __launch_bounds__(1024) vjp_kernel(BatchedParams P, const Real* grad,
+                                                   Real* grad_density) {
+  extern __shared__ __align__(16) uint8_t lds[];
+  if (P.inst[P.chunks[blockIdx.x].instance].global_tables)
+    vjp_body<Real, MAXPER, true>(P, grad, grad_density, lds);
+  else
+    vjp_body<Real, MAXPER, false>(P, grad, grad_density, lds);
+}
+
 // ---------------------------------------------------------------------------
 // Host-side launchers (called from capi.hip).
 template <typename Real>
 hipError_t launch_tables(const BatchedParams& P, int threads, bool sequential, int64_t lds,
-                         hipStream_t stream) {
+                         bool only_global, hipStream_t stream) {
   auto kern = sequential ? tables_kernel<Real, true> : tables_kernel<Real, false>;
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(kern, dim3(P.n_instances), dim3(threads), size_t(lds), stream, P);
+  hipLaunchKernelGGL(kern, dim3(P.n_instances), dim3(threads), size_t(lds), stream, P,
+                     only_global ? 1 : 0);
   return hipGetLastError();
 }
 
@@ -1270,8 +1354,10 @@ hipError_t launch_vjp(const BatchedParams& P, int threads, int64_t lds, int maxp
   return hipErrorInvalidValue;
 }
 
-template hipError_t launch_tables<float>(const BatchedParams&, int, bool, int64_t, hipStream_t);
-template hipError_t launch_tables<double>(const BatchedParams&, int, bool, int64_t, hipStream_t);
+template hipError_t launch_tables<float>(const BatchedParams&, int, bool, int64_t, bool,
+                                         hipStream_t);
+template hipError_t launch_tables<double>(const BatchedParams&, int, bool, int64_t, bool,
+                                          hipStream_t);
 template hipError_t launch_sample<float>(const BatchedParams&, int, bool, int64_t, hipStream_t);
 template hipError_t launch_sample<double>(const BatchedParams&, int, bool, int64_t, hipStream_t);
 template hipError_t launch_pack<float>(const BatchedParams&, int, void*, int32_t*, hipStream_t);

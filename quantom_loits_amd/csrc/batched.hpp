@@ -42,7 +42,7 @@ struct BatchedParams {
 
 template <typename Real>
 hipError_t launch_tables(const BatchedParams& P, int threads, bool sequential, int64_t lds,
-                         hipStream_t stream);
+                         bool only_global, hipStream_t stream);
 template <typename Real>
 hipError_t launch_sample(const BatchedParams& P, int threads, bool sequential, int64_t lds,
                          hipStream_t stream);

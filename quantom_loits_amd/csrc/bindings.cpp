@@ -304,6 +304,7 @@ class BatchedPlan {
     d["n_segments"] = layout_.n_segments;
     d["mask_words"] = layout_.mask_words;
     d["partial_doubles"] = layout_.partial_doubles;
+    d["global_instances"] = layout_.global_instances;
     d["dtype"] = layout_.dtype;
     d["threads"] = layout_.threads;
     d["scan_mode"] = layout_.scan_mode;

@@ -105,6 +105,7 @@ int make_plan(const loits_hip_instance* instances, int64_t n, int dtype,
   L.persistent = tuning->persistent;
   int64_t out = 0, table_off = 0, mask_words = 0, partial_doubles = 0;
   int64_t lds_f = 0, lds_b = 0, lds_t = 0;
+  int32_t n_global = 0;
   plan->inst.resize(size_t(n));
   plan->chunks.clear();
   for (int64_t i = 0; i < n; ++i) {
@@ -128,21 +129,33 @@ int make_plan(const loits_hip_instance* instances, int64_t n, int dtype,
     d.counter_base = s.counter_base;
     d.stream = s.stream;
     d.accept = s.accept ? 1 : 0;
-    // finest guide that leaves the forward's LDS build and the VJP's tables and
-    // histograms within LDS
-    d.guide_log2 = std::min(kMaxGuideLog2,
-                            std::max(kMinGuideLog2, ceil_log2(s.n_nodes) + tuning->guide_shift));
-    while (d.guide_log2 > kMinGuideLog2 &&
-           std::max(lds_build_layout(s.n_nodes, d.guide_log2, coef_size).bytes,
-                    lds_table_bytes(s.n_nodes, d.guide_log2, coef_size) + pad4(s.n_nodes) * 16) >
-               kMaxLds)
-      --d.guide_log2;
+    // Finest guide that leaves the forward's LDS build and the VJP's tables and
+    // histograms within LDS; when none does, the instance keeps its per-bin
+    // tables in global memory and only the guide is staged in LDS.
+    const int g_want = std::min(kMaxGuideLog2,
+                                std::max(kMinGuideLog2, ceil_log2(s.n_nodes) + tuning->guide_shift));
+    auto lds_mode_bytes = [&](int g) {
+      return std::max(lds_build_layout(s.n_nodes, g, coef_size).bytes,
+                      lds_table_bytes(s.n_nodes, g, coef_size) + pad4(s.n_nodes) * 16);
+    };
+    d.guide_log2 = g_want;
+    while (d.guide_log2 > kMinGuideLog2 && lds_mode_bytes(d.guide_log2) > kMaxLds) --d.guide_log2;
+    d.global_tables = lds_mode_bytes(d.guide_log2) > kMaxLds ? 1 : 0;
+    if (d.global_tables) d.guide_log2 = g_want;
     d.table_off = table_off;
     const TableLayout tl = table_layout(s.n_nodes, d.guide_log2, coef_size);
     table_off += align16(tl.bytes);
-    lds_f = std::max(lds_f, lds_build_layout(s.n_nodes, d.guide_log2, coef_size).bytes);
-    lds_t = lds_f;
-    lds_b = std::max(lds_b, (tl.cdf - tl.thr) + pad4(s.n_nodes) * 16);
+    const int64_t guide_bytes = (int64_t(2) << d.guide_log2) * 4;
+    if (d.global_tables) {
+      ++n_global;
+      lds_f = std::max(lds_f, guide_bytes);
+      lds_b = std::max(lds_b, guide_bytes);
+      lds_t = std::max(lds_t, lds_global_build_bytes(d.guide_log2));
+    } else {
+      lds_f = std::max(lds_f, lds_build_layout(s.n_nodes, d.guide_log2, coef_size).bytes);
+      lds_t = std::max(lds_t, lds_build_layout(s.n_nodes, d.guide_log2, coef_size).bytes);
+      lds_b = std::max(lds_b, (tl.cdf - tl.thr) + pad4(s.n_nodes) * 16);
+    }
     L.max_nodes = std::max(L.max_nodes, s.n_nodes);
     L.total_nodes = std::max(L.total_nodes, s.node_off + s.n_nodes);
     const int64_t nch = std::max<int64_t>(1, (s.events + chunk_events - 1) / chunk_events);
@@ -191,6 +204,7 @@ int make_plan(const loits_hip_instance* instances, int64_t n, int dtype,
                     align16(8 * mask_words) + align16(8 * partial_doubles);
   L.grid_forward = int32_t(L.n_chunks);
   L.grid_backward = int32_t(L.n_chunks);
+  L.global_instances = n_global;
   return LOITS_HIP_OK;
 }
 
@@ -343,6 +357,14 @@ int loits_hip_batched_forward(const loits_hip_batched_args* args, void* stream) 
     hipStream_t s = static_cast<hipStream_t>(stream);
     const bool f32 = L.dtype == LOITS_HIP_F32;
     const bool seq = L.scan_mode == 1;
+    if (L.global_instances > 0) {  // global-table instances are built first
+      const int tthreads = std::min(L.threads, 1024);
+      const int rt = check_hip(
+          f32 ? launch_tables<float>(P, tthreads, seq, L.lds_tables, true, s)
+              : launch_tables<double>(P, tthreads, seq, L.lds_tables, true, s),
+          "batched tables launch");
+      if (rt != LOITS_HIP_OK) return rt;
+    }
     int rc = check_hip(f32 ? launch_sample<float>(P, L.threads, seq, L.lds_forward, s)
                            : launch_sample<double>(P, L.threads, seq, L.lds_forward, s),
                        "batched sample launch");
@@ -392,8 +414,8 @@ int loits_hip_batched_backward(const loits_hip_batched_args* args, const void* g
       Q.norm_out = nullptr;
       Q.cdf_out = nullptr;
       const int tthreads = std::min(L.threads, 1024);
-      const int rc = check_hip(f32 ? launch_tables<float>(Q, tthreads, L.scan_mode == 1, L.lds_tables, s)
-                                   : launch_tables<double>(Q, tthreads, L.scan_mode == 1, L.lds_tables, s),
+      const int rc = check_hip(f32 ? launch_tables<float>(Q, tthreads, L.scan_mode == 1, L.lds_tables, false, s)
+                                   : launch_tables<double>(Q, tthreads, L.scan_mode == 1, L.lds_tables, false, s),
                                "batched tables launch");
       if (rc != LOITS_HIP_OK) return rc;
     }

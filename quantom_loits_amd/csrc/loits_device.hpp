@@ -203,6 +203,8 @@ struct InstanceDesc {
   int32_t guide_log2;    // guide cells = 1 << guide_log2
   int32_t first_chunk;   // chunks of an instance are consecutive
   int32_t n_chunks;
+  int32_t global_tables; // 1: per-bin tables too large for LDS stay in global memory
+  int32_t reserved;
 };
 
 // A chunk is the work of one work-group; each of its waves owns a contiguous
@@ -252,7 +254,7 @@ __host__ __device__ __forceinline__ Segment segment_of(uint64_t counter_base, in
 
 // Table blob of one instance (byte offsets from table_off), all 16-byte aligned:
 //   TableHeader
-//   uint32 srt[n_pad4]          thresholds T_t sorted ascending (equal to T for a
+//   uint32 srt[pad4(pow2(n))]   thresholds T_t sorted ascending (equal to T for a
 //                               monotone CDF); read only by the rare bisection path
 //   uint32 guide[G]             level 1: one word per cell of 2^s k values,
 //                               s = 24 - guide_log2 (encoding below)
@@ -340,6 +342,13 @@ struct CoefOf<double> {
 
 __host__ __device__ __forceinline__ int64_t pad4(int64_t n) { return (n + 3) & ~int64_t(3); }
 
+__host__ __device__ __forceinline__ int64_t pow2_ceil(int64_t n) {
+  int64_t p = 1;
+  while (p < n) p <<= 1;
+  return p;
+}
+
+
 struct TableLayout {
   int64_t thr, guide, coef, cdf, bytes;  // byte offsets within the instance blob
 };
@@ -348,7 +357,7 @@ __host__ __device__ __forceinline__ TableLayout table_layout(int64_t n, int guid
                                                              int64_t coef_size) {
   TableLayout t;
   t.thr = sizeof(TableHeader);
-  t.guide = t.thr + pad4(n) * 4;
+  t.guide = t.thr + pad4(pow2_ceil(n)) * 4;  // room for an in-place bitonic sort
   t.coef = t.guide + 2 * (int64_t(1) << guide_log2) * 4;
   t.cdf = t.coef + pad4(n) * coef_size;
   t.bytes = t.cdf + pad4(n) * 8;
@@ -366,12 +375,6 @@ __host__ __device__ __forceinline__ int64_t lds_table_bytes(int64_t n, int guide
 __host__ __device__ __forceinline__ int guide_fanout_log2(int cnt, int shift) {
   const int l = 1 + (32 - __builtin_clz(uint32_t(cnt - 1)));  // 1 + ceil(log2 cnt)
   return l < shift ? l : shift;
-}
-
-__host__ __device__ __forceinline__ int64_t pow2_ceil(int64_t n) {
-  int64_t p = 1;
-  while (p < n) p <<= 1;
-  return p;
 }
 
 // LDS image of one instance's tables while a work-group builds and samples it
@@ -392,6 +395,14 @@ __host__ __device__ __forceinline__ LdsLayout lds_build_layout(int64_t n, int gu
   l.crowd = l.cdf + pad4(n) * 8;
   l.bytes = l.crowd + (G / 4) * 8;
   return l;
+}
+
+// Instances whose LDS image does not fit keep srt, coef and cdf in the global
+// blob ("global tables"); a work-group then stages only the guide in LDS.
+// Standalone build: crowded-cell list (G / 4 uint2) | cell counts (G uint32).
+__host__ __device__ __forceinline__ int64_t lds_global_build_bytes(int guide_log2) {
+  const int64_t G = int64_t(1) << guide_log2;
+  return (G / 4) * 8 + G * 4;
 }
 
 // Bin of key k from a direct guide word; ko = k - L (the offset in the cell).

@@ -50,7 +50,8 @@ class Layout(C.Structure):
         "n_instances", "n_chunks", "capacity", "total_nodes", "max_nodes", "desc_bytes",
         "table_bytes", "scratch_bytes", "lds_forward", "lds_backward", "lds_tables", "n_segments",
         "mask_words", "partial_doubles")] + [(n, C.c_int32) for n in (
-            "dtype", "threads", "scan_mode", "persistent", "grid_forward", "grid_backward")]
+            "dtype", "threads", "scan_mode", "persistent", "grid_forward", "grid_backward",
+            "global_instances", "reserved")]
 
 
 def make_instances(nodes, events):
@@ -91,7 +92,7 @@ def test_batched_layout_sizes_and_tiles(lib):
     (dict(nodes=[1]), "n_nodes"),
     (dict(out_shift=5), "out_off"),
     (dict(nodes=[40000]), "n_nodes <= 32767"),
-    (dict(nodes=[20000]), "LDS budget"),
+    (dict(nodes=[20000], threads=256), "n_nodes / threads"),
 ])
 def test_batched_layout_rejects_bad_arguments(lib, bad, msg):
     nodes = bad.get("nodes", [64])
@@ -99,6 +100,8 @@ def test_batched_layout_rejects_bad_arguments(lib, bad, msg):
     if "out_shift" in bad:
         inst[0].out_off = bad["out_shift"]
     tun = Tuning(bad.get("threads", 256), 1, 4096, 0, 0)
+    if "nodes" in bad and bad["nodes"] == [20000]:
+        tun = Tuning(256, 1, 4096, 0, 0)
     lay = Layout()
     rc = lib.loits_hip_batched_layout(inst, len(nodes), 1, C.byref(tun), C.byref(lay))
     assert rc in (1, 3) and msg in last_error(lib)
@@ -170,3 +173,13 @@ def test_hip_backend_rejects_other_epsilon():
 
     with pytest.raises(ValueError, match="epsilon 1e-5"):
         HipLOITS(device="cuda", epsilon=1e-4)
+
+
+def test_large_curves_use_global_tables(lib):
+    """Curves whose tables do not fit in LDS keep them in global memory."""
+    inst = make_instances([2048, 6000, 16384], [1000, 1000, 1000])
+    tun = Tuning(1024, 2, 4096, 0, 0)
+    lay = Layout()
+    assert lib.loits_hip_batched_layout(inst, 3, 1, C.byref(tun), C.byref(lay)) == 0
+    assert lay.global_instances == 2
+    assert lay.lds_forward <= 160 * 1024 and lay.lds_backward <= 160 * 1024

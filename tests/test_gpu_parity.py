@@ -231,6 +231,53 @@ def test_batched_ragged_window_matches_oracle(oracle_port, hip_device):
             assert err <= (1e-10 if dtype == torch.float64 else 1e-4), (i, err)
 
 
+def test_batched_large_curves_global_tables_match_oracle(oracle_port, hip_device):
+    """Curves too large for LDS (global tables) mixed with LDS-resident ones,
+    several chunks per curve, a window cut; both precisions."""
+    from quantom_loits_amd.batched import BatchedLOITS, Tuning
+    from quantom_loits_amd.workloads import ragged_family
+
+    w = ragged_family(6, n_range=(2500, 16384), e_range=(20000, 150000), seed=11,
+                      dtype=np.float64)
+    n = len(w["n_nodes"])
+    rng = np.random.default_rng(12)
+    base = rng.integers(0, 2 ** 40, n).astype(np.uint64)
+    streams = rng.integers(0, 2 ** 33, n).astype(np.uint64)
+    accept = np.ones(n, np.uint8)
+    window = (0.05, 0.95)
+    for dtype, scan in ((torch.float64, 1), (torch.float32, 0)):
+        print("large curves", dtype, w["n_nodes"].tolist(), w["events"].tolist(), flush=True)
+        s = BatchedLOITS(w["n_nodes"], w["events"], base, streams, accept, w["node_off"],
+                         dtype=dtype, window=window,
+                         tuning=Tuning(threads=1024, chunk_events=65536, scan_mode=scan))
+        assert s.layout["global_instances"] > 0
+        x = torch.from_numpy(w["x"]).to(dtype).cuda()
+        d = torch.from_numpy(w["density"]).to(dtype).cuda()
+        x64, d64 = x.double().cpu().numpy(), d.double().cpu().numpy()
+        ref_v, ref_b, ref_k, _, _ = oracle_port.batched_forward(
+            x64, d64, w["node_off"], w["n_nodes"], w["events"], base, streams, 7,
+            accept=accept, window=window)
+        values, kept, st, ex = s.run_forward(x, d, 7, want_bins=True)
+        assert kept.cpu().numpy().tolist() == ref_k.tolist()
+        assert np.array_equal(ex["bins"].cpu().numpy(), ref_b)
+        v = values.cpu().numpy().astype(np.float64)
+        if dtype == torch.float64:
+            assert np.array_equal(v, ref_v)
+        else:
+            assert np.max(np.abs(v - ref_v) / np.abs(ref_v)) <= 1e-5
+        gv = np.random.default_rng(13).standard_normal(len(ref_v))
+        ref_g = oracle_port.batched_backward(x64, d64, w["node_off"], w["n_nodes"], w["events"],
+                                             base, streams, 7, gv, accept=accept, window=window)
+        gd = torch.empty_like(d)
+        s.backward_raw(x, d, 7, st, torch.from_numpy(gv).to(dtype).cuda(), gd)
+        got = gd.double().cpu().numpy()
+        for i in range(n):
+            sl = slice(int(w["node_off"][i]), int(w["node_off"][i] + w["n_nodes"][i]))
+            scale = np.abs(ref_g[sl]).max()
+            err = np.abs(got[sl] - ref_g[sl]).max() / scale
+            assert err <= (1e-9 if dtype == torch.float64 else 1e-4), (i, err)
+
+
 def test_public_autograd_path(hip_device):
     from quantom_loits_amd import LOITS
     from quantom_loits_amd.workloads import reference_1d_inputs, uniform_family
