@@ -43,6 +43,10 @@ def parse():
     p.add_argument("--chunk", type=int, default=1 << 18)
     p.add_argument("--guide-shift", type=int, default=2)
     p.add_argument("--scan-mode", type=int, default=0)
+    p.add_argument("--autotune", action="store_true",
+                   help="pick threads / chunk / guide shift with the launch-shape autotuner")
+    p.add_argument("--graph", action="store_true",
+                   help="also time the step (forward, pack, VJP) replayed as one HIP graph")
     p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                    help="only copy a MIB-MiB buffer a few times (the plain-copy kernel that "
                         "scales rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
@@ -125,6 +129,10 @@ def main():
     d = torch.from_numpy(d_np.reshape(-1)).to(dev)
     tun = Tuning(threads=args.threads, guide_shift=args.guide_shift, chunk_events=args.chunk,
                  scan_mode=args.scan_mode)
+    if args.autotune:
+        from quantom_loits_amd.autotune import Autotuner
+        tun = Autotuner().tuning_for(np.full(B, N), np.full(B, E), torch.float32,
+                                     args.scan_mode)
     s = BatchedLOITS.uniform(B, N, E, SEQUENCE, dtype=torch.float32, device=dev, tuning=tun)
     st = s.new_state()
     gen = torch.Generator(device=dev).manual_seed(1)
@@ -172,6 +180,21 @@ def main():
             traffic = json.loads(tfile.read_text()).get(dom["name"].split()[0])
         except (ValueError, OSError):
             traffic = None
+
+    graph = None
+    if args.graph:
+        g = s.capture(x, d, SEED)
+        g.grad.copy_(grad)
+        for _ in range(args.warmup):
+            g.replay()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        for _ in range(args.steps):
+            g.replay()
+        torch.cuda.synchronize()
+        graph_ms = (time.perf_counter() - t2) * 1e3 / args.steps
+        graph = dict(ms_per_step=graph_ms, events_per_s=B * E / (graph_ms * 1e-3))
+        del g
 
     plain = None
     if not args.no_plain:
@@ -227,6 +250,7 @@ def main():
                  "algorithmic_bytes": fwd_b + bwd_b, "achieved_gbs": step_gbs,
                  "frac": step_gbs / HBM_PEAK_GBS},
         "plain": plain,
+        "graph": graph,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

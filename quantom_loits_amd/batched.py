@@ -63,6 +63,11 @@ class BatchedLOITS:
             raise TypeError("dtype must be torch.float32 or torch.float64")
         self.dtype = dtype
         self.device = torch.device(device)
+        if isinstance(tuning, str):
+            if tuning != "auto":
+                raise ValueError("tuning must be a Tuning, None or 'auto'")
+            from .autotune import auto_tuning
+            tuning = auto_tuning(n_nodes, events, dtype)
         self.tuning = tuning or Tuning()
         self.window = window
         ext = load_extension()
@@ -163,6 +168,83 @@ class BatchedLOITS:
     def __call__(self, x, density, seed):
         """Differentiable w.r.t. density: returns the packed kept values."""
         return _BatchedFunction.apply(density, x, self, int(seed))
+
+    def capture(self, x, density, seed, recompute_tables=False):
+        """Capture forward, pack and VJP on fixed buffers into one HIP graph."""
+        return BatchedGraph(self, x, density, seed, recompute_tables)
+
+
+class BatchedGraph:
+    """One training step of the batched sampler as a replayable HIP graph.
+
+    The graph holds, in launch order: the forward (tables, sampling, row
+    offsets), the pack (a no-op inside the kernel when nothing was dropped) and
+    the VJP.  Every buffer is fixed at capture; between replays the caller
+    writes new nodes, densities and cotangents in place (``x``, ``density``,
+    ``grad``: one cotangent per row of the packed output, capacity rows) and
+    may move the Philox streams with ``set_streams``.  The seed is a launch
+    argument and stays fixed.  No kernel in the graph needs memory cleared
+    beforehand (the build resets the completion counters and the VJP clears its
+    own partials), so the graph holds no memset nodes.
+    """
+
+    def __init__(self, sampler, x, density, seed, recompute_tables=False):
+        sampler._check_inputs(x, density)
+        self.sampler = sampler
+        self.seed = int(seed)
+        d = sampler.device
+        cap = max(sampler.capacity, 1)
+        self.x = x
+        self.density = density
+        self.grad = torch.zeros(cap, dtype=sampler.dtype, device=d)
+        self.grad_density = torch.zeros_like(density)
+        self.packed = torch.zeros(cap, dtype=sampler.dtype, device=d)
+        self.desc = sampler.desc.clone()
+        self.st = sampler.new_state()
+        self.recompute_tables = bool(recompute_tables)
+        side = torch.cuda.Stream(device=d)
+        side.wait_stream(torch.cuda.current_stream(d))
+        with torch.cuda.stream(side):
+            self._launch()  # eager run: validates the arguments before capture
+        torch.cuda.current_stream(d).wait_stream(side)
+        torch.cuda.synchronize(d)
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            self._launch()
+
+    def _launch(self):
+        s = self.sampler
+        use, lo, hi = s._window()
+        seed = self.seed & (2 ** 64 - 1)
+        st = self.st
+        s.plan.forward(self.desc, self.x, self.density, seed, use, lo, hi, st["tables"],
+                       st["scratch"], st["events"], st["kept"], st["total"], None, None, None,
+                       None)
+        s.plan.pack(self.desc, self.x, self.density, st["tables"], st["scratch"], st["events"],
+                    st["kept"], st["total"], self.packed, None, None, None)
+        s.plan.backward(self.desc, self.x, self.density, seed, use, lo, hi, st["tables"],
+                        st["scratch"], st["events"], st["kept"], st["total"], self.grad,
+                        self.grad_density, self.recompute_tables, None)
+
+    def set_streams(self, streams):
+        """Point every instance at new Philox streams (host encode + one copy)."""
+        streams = torch.as_tensor(np.asarray(streams, dtype=np.uint64).view(np.int64))
+        blob = self.sampler.plan.descriptors_for_streams(streams)
+        self.desc.copy_(blob)
+
+    def replay(self):
+        self.graph.replay()
+
+    def values(self):
+        """Packed kept values of the last replay (reads two scalars on the host)."""
+        st = self.st
+        total = int(st["total"].item())
+        src = st["events"] if self.sampler.dense_flag(st) else self.packed
+        return src[:total]
+
+    @property
+    def kept(self):
+        return self.st["kept"]
 
 
 class _BatchedFunction(torch.autograd.Function):

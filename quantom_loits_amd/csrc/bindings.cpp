@@ -313,6 +313,24 @@ class BatchedPlan {
 
   at::Tensor descriptors() const { return desc_host_; }
 
+  // The descriptor blob with each instance's Philox stream replaced; the
+  // layout does not depend on the streams, so the blob has the same size and
+  // can be copied over a captured graph's device descriptors between replays.
+  at::Tensor descriptors_for_streams(at::Tensor streams) const {
+    streams = streams.contiguous();
+    if (streams.device().is_cuda() || streams.scalar_type() != at::kLong || streams.dim() != 1 ||
+        streams.size(0) != int64_t(inst_.size()))
+      throw std::invalid_argument("streams must be a CPU int64 tensor [n_instances]");
+    std::vector<loits_hip_instance> inst = inst_;
+    const int64_t* src = streams.data_ptr<int64_t>();
+    for (size_t i = 0; i < inst.size(); ++i) inst[i].stream = uint64_t(src[i]);
+    at::Tensor out = at::empty({layout_.desc_bytes}, at::TensorOptions().dtype(at::kByte));
+    check(loits_hip_batched_encode(inst.data(), int64_t(inst.size()), &tuning_, &layout_,
+                                   out.data_ptr()),
+          "loits_hip_batched_encode");
+    return out;
+  }
+
   // All tensors are device tensors the caller allocated with the layout's sizes.
   void forward(at::Tensor desc, at::Tensor x, at::Tensor density, uint64_t seed, bool use_window,
                double lo, double hi, at::Tensor tables, at::Tensor scratch, at::Tensor events,
@@ -413,6 +431,7 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
       .def(py::init<at::Tensor, int64_t, int64_t, int64_t, int64_t, int64_t, int64_t>())
       .def("layout", &BatchedPlan::layout)
       .def("descriptors", &BatchedPlan::descriptors)
+      .def("descriptors_for_streams", &BatchedPlan::descriptors_for_streams)
       .def("forward", &BatchedPlan::forward)
       .def("pack", &BatchedPlan::pack)
       .def("backward", &BatchedPlan::backward);

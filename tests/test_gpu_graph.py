@@ -1,0 +1,100 @@
+"""Batched forward + pack + VJP captured into a HIP graph.
+
+Each replay runs on new densities, cotangents and Philox streams written into
+the captured buffers, with unrelated eager launches and host synchronisations
+between replays, and is checked against an eager run of the same step (and, on
+the first and last replay, against the oracle).
+"""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def _grad_err(got, ref, node_off, n_nodes):
+    worst = 0.0
+    for o, n in zip(node_off, n_nodes):
+        sl = slice(int(o), int(o + n))
+        scale = np.abs(ref[sl]).max()
+        if scale == 0:
+            assert np.abs(got[sl]).max() == 0
+            continue
+        worst = max(worst, float(np.abs(got[sl] - ref[sl]).max() / scale))
+    return worst
+
+
+@pytest.mark.parametrize("dtype,window", [(torch.float32, (0.05, 0.95)),
+                                          (torch.float32, None),
+                                          (torch.float64, (0.1, 0.9))])
+def test_graph_replays_match_eager_and_oracle(oracle_port, hip_device, dtype, window):
+    from quantom_loits_amd.batched import BatchedLOITS, Tuning
+    from quantom_loits_amd.workloads import ragged_family
+
+    w = ragged_family(10, n_range=(16, 3000), e_range=(1, 30000), seed=21, dtype=np.float64)
+    n = len(w["n_nodes"])
+    rng = np.random.default_rng(22)
+    base = rng.integers(0, 2 ** 40, n).astype(np.uint64)
+    accept = np.ones(n, np.uint8)
+    if window is not None:  # (no window, all accepted: the dense path, no pack)
+        accept[2] = 0
+    tuning = Tuning(threads=512, chunk_events=8192, scan_mode=1 if dtype == torch.float64 else 0)
+    s = BatchedLOITS(w["n_nodes"], w["events"], base, np.zeros(n, np.uint64), accept,
+                     w["node_off"], dtype=dtype, window=window, tuning=tuning)
+    x = torch.from_numpy(w["x"]).to(dtype).cuda()
+    d = torch.from_numpy(w["density"]).to(dtype).cuda()
+    seed = 1234
+    g = s.capture(x.clone(), d.clone(), seed)
+    other = BatchedLOITS.uniform(4, 64, 5000, 3, dtype=dtype, tuning=Tuning(threads=256))
+    ox = torch.linspace(0, 1, 256, dtype=dtype, device="cuda")
+    od = torch.rand(256, dtype=dtype, device="cuda") + 0.1
+
+    replays = 4
+    for r in range(replays):
+        print("graph replay", dtype, window, r, flush=True)
+        rr = np.random.default_rng(100 + r)
+        dens = w["density"] * rr.uniform(0.5, 2.0, w["density"].shape)
+        streams = rr.integers(0, 2 ** 33, n).astype(np.uint64)
+        g.density.copy_(torch.from_numpy(dens).to(dtype))
+        g.set_streams(streams)
+        grad = torch.from_numpy(rr.standard_normal(max(s.capacity, 1))).to(dtype).cuda()
+        g.grad.copy_(grad)
+        g.replay()
+        # unrelated eager work and host syncs between replays
+        other.run_forward(ox, od, 77 + r)
+        torch.cuda.synchronize()
+        gv = g.values().clone()
+        gk = g.kept.clone()
+        gd = g.grad_density.clone()
+
+        e = BatchedLOITS(w["n_nodes"], w["events"], base, streams, accept, w["node_off"],
+                         dtype=dtype, window=window, tuning=tuning)
+        ev, ek, est, _ = e.run_forward(g.x, g.density, seed)
+        egd = torch.empty_like(g.density)
+        e.backward_raw(g.x, g.density, seed, est, grad[: max(ev.numel(), 1)].contiguous(), egd)
+        torch.cuda.synchronize()
+        assert torch.equal(gk, ek)
+        assert torch.equal(gv, ev)
+        err = _grad_err(gd.double().cpu().numpy(), egd.double().cpu().numpy(), w["node_off"],
+                        w["n_nodes"])
+        # fp64 atomics may sum in another order; float32 output can round differently
+        assert err <= (1e-12 if dtype == torch.float64 else 1e-6), (r, err)
+
+        if r in (0, replays - 1):
+            x64 = g.x.double().cpu().numpy()
+            d64 = g.density.double().cpu().numpy()
+            ref_v, _, ref_k, _, _ = oracle_port.batched_forward(
+                x64, d64, w["node_off"], w["n_nodes"], w["events"], base, streams, seed,
+                accept=accept, window=window)
+            assert gk.cpu().numpy().tolist() == ref_k.tolist()
+            v = gv.double().cpu().numpy()
+            if dtype == torch.float64:
+                assert np.array_equal(v, ref_v)
+            else:
+                assert np.max(np.abs(v - ref_v) / np.abs(ref_v)) <= 1e-5
+            gvec = grad[: len(ref_v)].double().cpu().numpy()
+            ref_g = oracle_port.batched_backward(x64, d64, w["node_off"], w["n_nodes"],
+                                                 w["events"], base, streams, seed, gvec,
+                                                 accept=accept, window=window)
+            err = _grad_err(gd.double().cpu().numpy(), ref_g, w["node_off"], w["n_nodes"])
+            assert err <= (1e-9 if dtype == torch.float64 else 1e-4), (r, err)
