@@ -14,6 +14,7 @@ reference call on the same B curves with weights 1 and ``n_events = E``.
 """
 from __future__ import annotations
 
+from contextlib import nullcontext
 from dataclasses import dataclass
 
 import numpy as np
@@ -37,7 +38,8 @@ class BatchedLOITS:
     """Plan and launch the batched forward / pack / VJP for fixed instance sizes."""
 
     def __init__(self, n_nodes, events, counter_base=None, streams=None, accept=None,
-                 node_off=None, *, dtype=torch.float32, device="cuda", tuning=None, window=None):
+                 node_off=None, *, dtype=torch.float32, device="cuda", tuning=None, window=None,
+                 profile_regions=False):
         n_nodes = np.asarray(n_nodes, dtype=np.int64).reshape(-1)
         events = np.asarray(events, dtype=np.int64).reshape(-1)
         n = n_nodes.shape[0]
@@ -82,6 +84,8 @@ class BatchedLOITS:
         self.events = events
         self.n_nodes = n_nodes
         self.node_off = node_off
+        self.profile_regions = bool(profile_regions)
+        self.event_log = None  # list of (kind, start, end) CUDA events when set
 
     @classmethod
     def uniform(cls, B, N, E, sequence=0, **kw):
@@ -133,7 +137,8 @@ class BatchedLOITS:
     def run_forward(self, x, density, seed, want_bins=False, want_tables=False):
         """Forward with one host sync; returns (values, kept, state, extras)."""
         self._check_inputs(x, density)
-        st = self.new_state()
+        with self.region("loits::forward::allocation"):
+            st = self.new_state()
         extras = {}
         bins = torch.empty(max(self.capacity, 1), dtype=torch.int32, device=self.device) \
             if want_bins else None
@@ -142,7 +147,13 @@ class BatchedLOITS:
             norm = torch.empty(self.n_instances, dtype=torch.float64, device=self.device)
             cdf = torch.empty(max(self.total_nodes, 1), dtype=torch.float64, device=self.device)
             extras.update(norm=norm, cdf=cdf)
-        self.forward_raw(x, density, seed, st, norm, cdf, bins)
+        # one fused launch group: density, CDF, Philox, search and interpolation
+        with self.region("loits::forward::interpolation_x"):
+            self.forward_raw(x, density, seed, st, norm, cdf, bins)
+        with self.region("loits::forward::stream_compaction"):
+            return self._compact(x, density, st, bins, extras)
+
+    def _compact(self, x, density, st, bins, extras):
         total = int(st["total"].item())
         if self.dense_flag(st):
             values = st["events"][:total]
@@ -167,7 +178,39 @@ class BatchedLOITS:
 
     def __call__(self, x, density, seed):
         """Differentiable w.r.t. density: returns the packed kept values."""
+        return self.sample(x, density, seed)[0]
+
+    def sample(self, x, density, seed):
+        """Differentiable w.r.t. density: (packed kept values, kept per instance)."""
         return _BatchedFunction.apply(density, x, self, int(seed))
+
+    def set_streams(self, streams):
+        """Move every instance to new Philox streams (one host encode and copy).
+
+        The reference post-increments its sequence per call and draws stream
+        2*sequence for x (cpp/backend.py:34-101); a training loop does the
+        same here with ``set_streams(np.full(B, 2 * sequence))``.
+        """
+        streams = torch.as_tensor(np.asarray(streams, dtype=np.uint64).reshape(-1).view(np.int64))
+        self.desc.copy_(self.plan.descriptors_for_streams(streams))
+
+    # -- observability ------------------------------------------------------
+    def region(self, name):
+        """A ``loits::*`` profiler range, synchronised at both ends when profiling.
+
+        Kernels run asynchronously, so a range that did not wait for the device
+        would only time the launches; like the reference's region hooks
+        (pytorch/profiler.py:9-61) the device is synchronised on entry and exit.
+        """
+        if not self.profile_regions:
+            return nullcontext()
+        return _SyncedRegion(name, self.device)
+
+    def _record(self, kind):
+        """CUDA-event pair around one launch group when ``event_log`` is a list."""
+        if self.event_log is None:
+            return nullcontext()
+        return _EventPair(self.event_log, kind)
 
     def capture(self, x, density, seed, recompute_tables=False):
         """Capture forward, pack and VJP on fixed buffers into one HIP graph."""
@@ -247,25 +290,66 @@ class BatchedGraph:
         return self.st["kept"]
 
 
+class _SyncedRegion:
+    def __init__(self, name, device):
+        self.name = name
+        self.device = device
+        self.ctx = None
+
+    def __enter__(self):
+        torch.cuda.synchronize(self.device)
+        self.ctx = torch.profiler.record_function(self.name)
+        self.ctx.__enter__()
+        return self
+
+    def __exit__(self, *exc):
+        torch.cuda.synchronize(self.device)
+        self.ctx.__exit__(*exc)
+        return False
+
+
+class _EventPair:
+    def __init__(self, log, kind):
+        self.log = log
+        self.kind = kind
+
+    def __enter__(self):
+        self.start = torch.cuda.Event(enable_timing=True)
+        self.end = torch.cuda.Event(enable_timing=True)
+        self.start.record()
+        return self
+
+    def __exit__(self, *exc):
+        self.end.record()
+        self.log.append((self.kind, self.start, self.end))
+        return False
+
+
 class _BatchedFunction(torch.autograd.Function):
     @staticmethod
     def forward(ctx, density, x, sampler, seed):
         x = x.contiguous()
         density = density.contiguous()
-        values, kept, st, _ = sampler.run_forward(x, density, seed)
+        with sampler.region("loits::forward"), sampler._record("forward"):
+            values, kept, st, _ = sampler.run_forward(x, density, seed)
         ctx.sampler = sampler
         ctx.seed = seed
         ctx.st = st
         ctx.save_for_backward(x, density)
-        return values
+        ctx.mark_non_differentiable(kept)
+        return values, kept
 
     @staticmethod
-    def backward(ctx, grad):
+    def backward(ctx, grad, _grad_kept):
         x, density = ctx.saved_tensors
         sampler = ctx.sampler
-        grad = grad.contiguous().to(sampler.dtype)
-        if grad.numel() == 0:
-            grad = torch.zeros(1, dtype=sampler.dtype, device=sampler.device)
-        gd = torch.empty_like(density)
-        sampler.backward_raw(x, density, ctx.seed, ctx.st, grad, gd)
+        with sampler.region("loits::backward"), sampler._record("backward"):
+            with sampler.region("loits::backward::interpolation_x"):
+                if grad is None:
+                    grad = torch.zeros(1, dtype=sampler.dtype, device=sampler.device)
+                grad = grad.contiguous().to(sampler.dtype)
+                if grad.numel() == 0:
+                    grad = torch.zeros(1, dtype=sampler.dtype, device=sampler.device)
+                gd = torch.empty_like(density)
+                sampler.backward_raw(x, density, ctx.seed, ctx.st, grad, gd)
         return gd, None, None, None
