@@ -47,6 +47,9 @@ def parse():
                    help="pick threads / chunk / guide shift with the launch-shape autotuner")
     p.add_argument("--graph", action="store_true",
                    help="also time the step (forward, pack, VJP) replayed as one HIP graph")
+    p.add_argument("--no-extras", action="store_true",
+                   help="skip the other workloads (forward at B=1024 N=4096, recomputed "
+                        "tables, the 4096-curve ragged batch)")
     p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                    help="only copy a MIB-MiB buffer a few times (the plain-copy kernel that "
                         "scales rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
@@ -60,6 +63,88 @@ def algorithmic_bytes(kept, nodes, s=4):
     fwd = nodes * (s + s) + kept * s
     bwd = kept * s + nodes * (s + s + s)
     return fwd, bwd
+
+
+def time_ms(fn, reps, warmup=2):
+    import torch
+
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / reps
+
+
+def extra_workloads(dev, steps):
+    """Informational timings of the other workloads (not the headline value)."""
+    import torch
+    from quantom_loits_amd.batched import BatchedLOITS, Tuning
+    from quantom_loits_amd.workloads import ragged_family, uniform_family
+
+    out = {}
+    # forward only, B=1024 curves of N=4096 nodes, 2^16 events each
+    b1, n1, e1 = 1024, 4096, 1 << 16
+    x1, d1 = uniform_family(b1, n1, seed=1, dtype=np.float32)
+    x1 = torch.from_numpy(x1.reshape(-1)).to(dev)
+    d1 = torch.from_numpy(d1.reshape(-1)).to(dev)
+    s1 = BatchedLOITS.uniform(b1, n1, e1, SEQUENCE, dtype=torch.float32, device=dev,
+                              tuning=Tuning(chunk_events=1 << 16))
+    st1 = s1.new_state()
+    ms = time_ms(lambda: s1.forward_raw(x1, d1, SEED, st1), steps)
+    out["forward_B1024_N4096_E2p16"] = dict(ms=ms, events_per_s=b1 * e1 / (ms * 1e-3),
+                                            global_table_curves=s1.layout["global_instances"])
+    del s1, st1, x1, d1
+    # VJP that rebuilds the curve tables instead of reading the forward's
+    xb, db = uniform_family(B, N, seed=0, dtype=np.float32)
+    xb = torch.from_numpy(xb.reshape(-1)).to(dev)
+    db = torch.from_numpy(db.reshape(-1)).to(dev)
+    sb = BatchedLOITS.uniform(B, N, E, SEQUENCE, dtype=torch.float32, device=dev)
+    stb = sb.new_state()
+    gb = torch.randn(B * E, device=dev, dtype=torch.float32)
+    gdb = torch.empty_like(db)
+    sb.forward_raw(xb, db, SEED, stb)
+    stored = time_ms(lambda: sb.backward_raw(xb, db, SEED, stb, gb, gdb), steps)
+    rebuilt = time_ms(lambda: sb.backward_raw(xb, db, SEED, stb, gb, gdb, True), steps)
+    out["vjp_B256_N2048_E2p18"] = dict(stored_tables_ms=stored, recomputed_tables_ms=rebuilt)
+    del sb, stb, gb, gdb
+    # 4096 ragged curves, N log-uniform in [128, 16384], E in [1024, 2^20], window cut
+    w = ragged_family(4096, n_range=(128, 16384), e_range=(1024, 1 << 20), seed=5)
+    nn = len(w["n_nodes"])
+    xr = torch.from_numpy(w["x"]).to(dev)
+    dr = torch.from_numpy(w["density"]).to(dev)
+    base = np.concatenate([[0], np.cumsum(w["events"])[:-1]]).astype(np.uint64)
+    sr = BatchedLOITS(w["n_nodes"], w["events"], base, None, None, w["node_off"],
+                      dtype=torch.float32, device=dev, window=(0.05, 0.95))
+    g = sr.capture(xr, dr, SEED)
+    g.grad.normal_()
+    fused = time_ms(g.replay, max(3, steps // 4))
+    kept = int(g.kept.sum())
+    del g
+    singles = []
+    for i in range(nn):
+        si = BatchedLOITS(w["n_nodes"][i:i + 1], w["events"][i:i + 1], base[i:i + 1], None, None,
+                          w["node_off"][i:i + 1], dtype=torch.float32, device=dev,
+                          window=(0.05, 0.95), tuning=Tuning(chunk_events=1 << 16))
+        singles.append((si, si.new_state(),
+                        torch.empty(max(si.capacity, 1), device=dev, dtype=torch.float32)))
+    gr = torch.randn(max(int(w["events"].max()), 1), device=dev, dtype=torch.float32)
+    gdr = torch.empty_like(dr)
+
+    def loop():
+        for si, sti, packed in singles:
+            si.forward_raw(xr, dr, SEED, sti)
+            si.pack_raw(xr, dr, sti, packed)
+            si.backward_raw(xr, dr, SEED, sti, gr, gdr)
+
+    looped = time_ms(loop, 2, warmup=1)
+    ev = int(w["events"].sum())
+    out["ragged_4096_window"] = dict(curves=nn, events=ev, kept=kept, fused_graph_ms=fused,
+                                     fused_events_per_s=ev / (fused * 1e-3),
+                                     per_curve_loop_ms=looped, speedup=looped / fused)
+    return out
 
 
 def cpu_baseline():
@@ -222,6 +307,7 @@ def main():
         plain = dict(ms_per_step=plain_ms, events_per_s=B * E / (plain_ms * 1e-3),
                      speedup=plain_ms / ms_per_step)
 
+    extras = None if args.no_extras else extra_workloads(dev, args.steps)
     cpu = None if args.no_cpu else cpu_baseline()
     out = {
         "metric": "events/s fwd+VJP, B=256 N=2048, fused batched vs plain; % HBM peak",
@@ -251,6 +337,7 @@ def main():
                  "frac": step_gbs / HBM_PEAK_GBS},
         "plain": plain,
         "graph": graph,
+        "other_workloads": extras,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -200,8 +200,13 @@ typedef struct loits_hip_layout {
   int32_t dtype, threads, scan_mode, persistent;
   int32_t grid_forward, grid_backward;
   int32_t global_instances; /* instances whose per-bin tables stay in global memory
-                               (too large for LDS; built by a pre-pass, slower) */
-  int32_t reserved;
+                               (too large for the forward's LDS build; built by a
+                               pre-pass, slower) */
+  int32_t range_instances;  /* instances whose VJP splits the bins into ranges of at
+                               most 4096 (tables and histograms too large for LDS);
+                               grid_backward counts one work item per chunk and range */
+  int64_t backward_plain;   /* work items of the instances without ranges (the first
+                               launch of the VJP; the ranged items follow) */
 } loits_hip_layout;
 
 /* Host only: validate `instances` and compute sizes for `dtype`. */

@@ -29,7 +29,14 @@
 #include "batched.hpp"
 #include "loits_device.hpp"
 
+#ifndef QLOITS_TRIP
+#define QLOITS_TRIP 2
+#endif
+
 namespace qloits {
+
+// Philox blocks per lane per trip of the sampling and VJP loops.
+constexpr int kTrip = QLOITS_TRIP;
 
 // ---------------------------------------------------------------------------
 // Block-wide fp64 helpers (work-group of up to 1024 threads = 16 waves).
@@ -775,21 +782,29 @@ __device__ __forceinline__ void sample_segments(const BatchedParams& P, int c, c
     running += total;
   };
 
-  // Two iterations per trip: their Philox, search and interpolation chains are
-  // independent, which hides LDS latency at four waves per SIMD.
+  // kTrip iterations per trip: their Philox, search and interpolation chains
+  // are independent, which hides LDS latency at four waves per SIMD.
   int it = 0;
-  for (uint32_t i0 = 0; i0 < nblk; i0 += 128, it += 2) {
-    const Words4 wa = philox4x32_10_uniform(sg.qa + i0 + uint32_t(lane), in.stream, P.seed);
-    const Words4 wb = philox4x32_10_uniform(sg.qa + i0 + 64 + uint32_t(lane), in.stream, P.seed);
-    const uint32_t k[8] = {wa.w0 >> 8, wa.w1 >> 8, wa.w2 >> 8, wa.w3 >> 8,
-                           wb.w0 >> 8, wb.w1 >> 8, wb.w2 >> 8, wb.w3 >> 8};
-    int j[8];
-    find_bins<8>(k, s_guide, s_srt, shift, n, j);
-    Real v[8];
+  for (uint32_t i0 = 0; i0 < nblk; i0 += 64 * kTrip, it += kTrip) {
+    uint32_t k[4 * kTrip];
+#pragma unroll
+    for (int h = 0; h < kTrip; ++h) {
+      const Words4 wd =
+          philox4x32_10_uniform(sg.qa + i0 + 64 * h + uint32_t(lane), in.stream, P.seed);
+      k[4 * h] = wd.w0 >> 8;
+      k[4 * h + 1] = wd.w1 >> 8;
+      k[4 * h + 2] = wd.w2 >> 8;
+      k[4 * h + 3] = wd.w3 >> 8;
+    }
+    int j[4 * kTrip];
+    find_bins<4 * kTrip>(k, s_guide, s_srt, shift, n, j);
+    Real v[4 * kTrip];
     const int keep =
-        slot_values<Real, WINDOW, 8>(k, j, s_coef, s_thr, cdf, x_g, P, safe_curve, v);
-    emit(it, i0, v, j, keep & 0xF);
-    if (i0 + 64 < nblk) emit(it + 1, i0 + 64, v + 4, j + 4, keep >> 4);
+        slot_values<Real, WINDOW, 4 * kTrip>(k, j, s_coef, s_thr, cdf, x_g, P, safe_curve, v);
+#pragma unroll
+    for (int h = 0; h < kTrip; ++h)
+      if (h == 0 || i0 + 64 * h < nblk)
+        emit(it + h, i0 + 64 * h, v + 4 * h, j + 4 * h, (keep >> (4 * h)) & 0xF);
   }
   if (lane == 0) {
     P.seg_kept[seg] = running;
@@ -986,6 +1001,11 @@ __device__ __forceinline__ double vjp_tau_d(const double* cdf, int64_t j, int64_
 // A value that would reach 2^30 (64x that maximum) or is not finite goes to
 // fp64 atomics in the chunk's global partial sums.  Other curves, and float64,
 // accumulate fp64 directly (ds_add_f64).
+// A load that skips the work-group's L1: values summed by device atomics at L2.
+__device__ __forceinline__ double load_l2(const double* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 template <bool GLOBAL>
 __device__ __forceinline__ void add_f64(double* p, double v) {
   if constexpr (GLOBAL) unsafeAtomicAdd(p, v);
@@ -997,12 +1017,11 @@ __device__ __forceinline__ void add_f64(double* p, double v) {
 // sums in global memory.
 template <typename Real, int MAXPER, bool GLOBAL>
 __device__ __forceinline__ void vjp_body(const BatchedParams& P, const Real* grad,
-                                         Real* grad_density, uint8_t* lds) {
+                                         Real* grad_density, uint8_t* lds, int c) {
   using Coef = typename CoefOf<Real>::type;
   __shared__ double scratch[16];
   __shared__ int sh_last;
   __shared__ uint32_t sh_gmax;
-  const int c = blockIdx.x;
   const ChunkDesc ch = P.chunks[c];
   const InstanceDesc in = P.inst[ch.instance];
   Tables<Real> T = tables_of<Real>(P.tables, in);
@@ -1149,33 +1168,40 @@ __device__ __forceinline__ void vjp_body(const BatchedParams& P, const Real* gra
     }
   };
 
-  // Two iterations per trip (independent Philox / search chains).
+  // kTrip iterations per trip (independent Philox / search chains).
   int it = 0;
-  for (uint32_t i0 = 0; i0 < nblk; i0 += 128, it += 2) {
-    const bool has_b = i0 + 64 < nblk;
-    const Words4 wa = philox4x32_10_uniform(sg.qa + i0 + uint32_t(lane), in.stream, P.seed);
-    const Words4 wb = philox4x32_10_uniform(sg.qa + i0 + 64 + uint32_t(lane), in.stream, P.seed);
-    const uint32_t k[8] = {wa.w0 >> 8, wa.w1 >> 8, wa.w2 >> 8, wa.w3 >> 8,
-                           wb.w0 >> 8, wb.w1 >> 8, wb.w2 >> 8, wb.w3 >> 8};
-    int kb[2];
-    int64_t rw[2];
-    rows_of(it, i0, kb[0], rw[0]);
-    kb[1] = 0;
-    rw[1] = 0;
-    if (has_b) rows_of(it + 1, i0 + 64, kb[1], rw[1]);
-    int j[8];
-    find_bins<8>(k, s_guide, s_srt, shift, n, j);
-    if (fixed) {
-      float th[8];
+  for (uint32_t i0 = 0; i0 < nblk; i0 += 64 * kTrip, it += kTrip) {
+    uint32_t k[4 * kTrip];
 #pragma unroll
-      for (int e = 0; e < 8; ++e) {
+    for (int h = 0; h < kTrip; ++h) {
+      const Words4 wd =
+          philox4x32_10_uniform(sg.qa + i0 + 64 * h + uint32_t(lane), in.stream, P.seed);
+      k[4 * h] = wd.w0 >> 8;
+      k[4 * h + 1] = wd.w1 >> 8;
+      k[4 * h + 2] = wd.w2 >> 8;
+      k[4 * h + 3] = wd.w3 >> 8;
+    }
+    int kb[kTrip];
+    int64_t rw[kTrip];
+#pragma unroll
+    for (int h = 0; h < kTrip; ++h) {
+      kb[h] = 0;
+      rw[h] = 0;
+      if (h == 0 || i0 + 64 * h < nblk) rows_of(it + h, i0 + 64 * h, kb[h], rw[h]);
+    }
+    int j[4 * kTrip];
+    find_bins<4 * kTrip>(k, s_guide, s_srt, shift, n, j);
+    if (fixed) {
+      float th[4 * kTrip];
+#pragma unroll
+      for (int e = 0; e < 4 * kTrip; ++e) {
         const float2 ab = s_vb[j[e]];
         th[e] = __builtin_fmaf(static_cast<float>(static_cast<int32_t>(k[e] - s_srt[j[e]])), ab.x,
                                ab.y);
       }
-      float g[8];
+      float g[4 * kTrip];
 #pragma unroll
-      for (int h = 0; h < 2; ++h) {
+      for (int h = 0; h < kTrip; ++h) {
         if (kb[h] == 0xF && (rw[h] & 3) == 0) {
           const float4 g4 = *reinterpret_cast<const float4*>(grad + rw[h]);
           g[4 * h] = g4.x; g[4 * h + 1] = g4.y; g[4 * h + 2] = g4.z; g[4 * h + 3] = g4.w;
@@ -1186,7 +1212,7 @@ __device__ __forceinline__ void vjp_body(const BatchedParams& P, const Real* gra
         }
       }
 #pragma unroll
-      for (int h = 0; h < 2; ++h) {
+      for (int h = 0; h < kTrip; ++h) {
         if (kb[h] == 0xF) {
 #pragma unroll
           for (int e = 0; e < 4; ++e) add_fixed(j[4 * h + e], g[4 * h + e], th[4 * h + e]);
@@ -1198,7 +1224,7 @@ __device__ __forceinline__ void vjp_body(const BatchedParams& P, const Real* gra
       }
     } else {
 #pragma unroll
-      for (int h = 0; h < 2; ++h) {
+      for (int h = 0; h < kTrip; ++h) {
         double t[4];
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
@@ -1234,8 +1260,8 @@ __device__ __forceinline__ void vjp_body(const BatchedParams& P, const Real* gra
     for (int64_t i = tid; i < nb; i += nt) {
       double v0 = 0.0, v1 = 0.0;
       if (i < n - 1) {
-        v0 = double(static_cast<int64_t>(a0[i])) * inv_scale + outl[i];
-        v1 = (double(static_cast<int64_t>(a1[i])) * inv_scale + outl[nb + i]) *
+        v0 = double(static_cast<int64_t>(a0[i])) * inv_scale + load_l2(outl + i);
+        v1 = (double(static_cast<int64_t>(a1[i])) * inv_scale + load_l2(outl + nb + i)) *
              vjp_tau_d(T.cdf, i, n);
       }
       h0[i] = v0;
@@ -1281,14 +1307,341 @@ __device__ __forceinline__ void vjp_body(const BatchedParams& P, const Real* gra
   vjp_tail<Real, MAXPER>(P, in, T, h0, h1, grad_density, scratch);
 }
 
+// Exclusive prefix maximum over the work-group (wmax: 16 words of LDS).
+__device__ uint32_t block_exclusive_max(uint32_t v, uint32_t* wmax) {
+  const int lane = lane_id(), w = threadIdx.x >> 6;
+  uint32_t inc = v;
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    const uint32_t o = uint32_t(__shfl_up(int(inc), d, kWave));
+    if (lane >= d) inc = o > inc ? o : inc;
+  }
+  uint32_t ex = uint32_t(__shfl_up(int(inc), 1, kWave));
+  if (lane == 0) ex = 0u;
+  if (lane == kWave - 1) wmax[w] = inc;
+  __syncthreads();
+  for (int i = 0; i < w; ++i) ex = wmax[i] > ex ? wmax[i] : ex;
+  __syncthreads();
+  return ex;
+}
+
+// Bin-range VJP of a monotone curve whose VJP tables and histograms do not fit
+// LDS.  Work item (chunk c, range q) owns the bins [lo, hi), i.e. the keys k in
+// [K_lo, K_hi) (K_lo = T_lo, or 0 for the first range; K_hi = T_hi, or 2^24 for
+// the last): it regenerates the chunk's keys, skips the keys outside its
+// interval and sums the others into LDS histograms of its bins.  The bin of a
+// key is the last t with T_t <= k, clamped to n - 2 (cpp/loits_core.cpp:12-18
+// for a sorted CDF): a cell table over the interval gives the bins at the
+// start of the key's cell and of the next cell, and a bisection between them
+// (usually zero or one step) finishes it.  Partial sums go to the chunk's
+// partial slots of these bins; the last item of the instance reduces them.
+template <typename Real, int MAXPER>
+__device__ void vjp_range_body(const BatchedParams& P, const Real* grad, Real* grad_density,
+                               uint8_t* lds, int c, int q) {
+  __shared__ double scratch[16];
+  __shared__ int sh_last;
+  __shared__ uint32_t sh_gmax;
+  __shared__ uint32_t sh_wmax[16];
+  constexpr bool fixed = sizeof(Real) == 4;
+  const ChunkDesc ch = P.chunks[c];
+  const InstanceDesc in = P.inst[ch.instance];
+  Tables<Real> T = tables_of<Real>(P.tables, in);
+  const int n = static_cast<int>(in.n_nodes);
+  const int R = in.vjp_ranges;
+  const int W = vjp_range_width(n, R);
+  const int lo = q * W;
+  const int hi = lo + W < n - 1 ? lo + W : n - 1;
+  const int nbins = hi - lo;  // >= 1
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int64_t nb = pad4(n);
+  double* outl = P.partials + ch.partial_off;  // [2 nb]: overflow sums, then partial sums
+
+  const uint32_t k_lo = lo == 0 ? 0u : T.srt[lo];
+  const uint32_t k_hi = hi == n - 1 ? (1u << 24) : T.srt[hi];
+  const uint32_t span = k_hi > k_lo ? k_hi - k_lo : 0u;
+  const int span_log2 = span <= 1u ? 0 : 32 - __builtin_clz(span - 1u);
+  const int cs = span_log2 > kRangeCellsLog2 ? span_log2 - kRangeCellsLog2 : 0;  // cell 2^cs keys
+  const int ncell = span == 0u ? 1 : int(((span - 1u) >> cs) + 1u);             // <= kRangeCells
+
+  uint32_t* cell_bin = reinterpret_cast<uint32_t*>(lds);  // [kRangeCells + 4]
+  uint32_t* thr = cell_bin + (kRangeCells + 4);           // [W + 4]: T_lo .. T_hi
+  uint8_t* bd = reinterpret_cast<uint8_t*>(thr + (W + 4));  // [W] per-bin data
+  uint64_t* a0 = reinterpret_cast<uint64_t*>(bd + int64_t(8) * W);
+  uint64_t* a1 = a0 + W;
+  double* d0 = reinterpret_cast<double*>(a0);
+  double* d1 = reinterpret_cast<double*>(a1);
+
+  for (int t = tid; t <= nbins; t += nt) thr[t] = T.srt[lo + t];
+  for (int i = tid; i < nbins; i += nt) {
+    const int j = lo + i;
+    if constexpr (fixed) {  // t / tau_d = float(k - T_j) alpha + beta
+      const double tau_d = vjp_tau_d(T.cdf, j, n);
+      const double r = double(T.srt[j]) * 0x1.0p-24 - T.cdf[j];
+      reinterpret_cast<float2*>(bd)[i] =
+          make_float2(static_cast<float>(0x1.0p-24 / tau_d), static_cast<float>(r / tau_d));
+    } else {  // t = u - C_j
+      reinterpret_cast<double*>(bd)[i] = T.cdf[j];
+    }
+  }
+  for (int i = tid; i < 2 * W; i += nt) a0[i] = 0u;
+  for (int i = lo + tid; i < hi; i += nt) {
+    outl[i] = 0.0;
+    outl[nb + i] = 0.0;
+  }
+  if (hi == n - 1)
+    for (int64_t i = hi + tid; i < nb; i += nt) {
+      outl[i] = 0.0;
+      outl[nb + i] = 0.0;
+    }
+  for (int i = tid; i <= ncell; i += nt) cell_bin[i] = 0u;
+  if (tid == 0) sh_gmax = 0u;
+  __syncthreads();
+  // Bin t (relative) starts the cells whose first key lies in [T_t, T_next);
+  // those cells are disjoint across bins, so each run's first cell gets one
+  // mark and a prefix maximum fills the rest.
+  for (int t = tid; t <= nbins; t += nt) {
+    const uint32_t a = thr[t];
+    const uint32_t b = lo + t + 1 < n ? T.srt[lo + t + 1] : (1u << 24);
+    const uint32_t ca = a <= k_lo ? 0u : uint32_t((uint64_t(a - k_lo) + (1u << cs) - 1u) >> cs);
+    const uint32_t cb = b <= k_lo ? 0u : uint32_t((uint64_t(b - k_lo) + (1u << cs) - 1u) >> cs);
+    if (ca < cb && ca < uint32_t(ncell)) cell_bin[ca] = uint32_t(t);
+  }
+  __syncthreads();
+  {
+    const int per = (ncell + nt - 1) / nt;
+    const int c0 = tid * per;
+    uint32_t m = 0u;
+    for (int i = 0; i < per; ++i)
+      if (c0 + i < ncell) m = cell_bin[c0 + i] > m ? cell_bin[c0 + i] : m;
+    uint32_t run = block_exclusive_max(m, sh_wmax);
+    for (int i = 0; i < per; ++i) {
+      if (c0 + i >= ncell) break;
+      run = cell_bin[c0 + i] > run ? cell_bin[c0 + i] : run;
+      cell_bin[c0 + i] = run;
+    }
+    if (tid == 0) cell_bin[ncell] = uint32_t(nbins);
+  }
+
+  const int nw = nt >> 6, w = tid >> 6, lane = lane_id();
+  const Segment sg = segment_of(in.counter_base, ch.slot_lo, ch.slot_hi, w, nw);
+  const int64_t seg = int64_t(c) * nw + w;
+  const int first_drop = P.seg_drop[seg];
+  const int64_t row0 = P.seg_row[seg];
+
+  // Fixed-point scale from the first rows of each segment holding a non-zero
+  // cotangent (as in the LDS-resident VJP).
+  float scale = 0.0f;
+  int scale_log2 = 0;
+  if constexpr (fixed) {
+    const int64_t kept = P.seg_kept[seg];
+    uint32_t m = 0u;
+    for (int64_t r0 = 0; r0 < kept && m == 0u; r0 += 256) {
+      uint32_t lm = 0u;
+#pragma unroll
+      for (int qq = 0; qq < 4; ++qq) {
+        const int64_t r = r0 + 4 * lane + qq;
+        if (r < kept) {
+          const uint32_t bits = __builtin_bit_cast(uint32_t, float(grad[row0 + r])) & 0x7FFFFFFFu;
+          lm = bits > lm ? bits : lm;
+        }
+      }
+#pragma unroll
+      for (int d = 32; d >= 1; d >>= 1) {
+        const uint32_t o = uint32_t(__shfl_xor(int(lm), d, kWave));
+        lm = o > lm ? o : lm;
+      }
+      m = lm;
+    }
+    if (lane == 0 && m != 0u) atomicMax(&sh_gmax, m);
+  }
+  __syncthreads();
+  if constexpr (fixed) {
+    const uint32_t gm = sh_gmax;
+    const int ex = int(gm >> 23);
+    if (ex == 255) {
+      scale = __builtin_nanf("");
+    } else {
+      scale_log2 = 24 - (ex - 126);
+      scale_log2 = scale_log2 > 126 ? 126 : scale_log2;
+      scale = __builtin_ldexpf(1.0f, scale_log2);
+    }
+  }
+
+  const int64_t per_wave = int64_t(blocks_per_wave(uint64_t(chunk_blocks(in, ch)), nw) / 64);
+  const uint64_t* mask = P.keepmask + ch.mask_off + int64_t(w) * per_wave * 4;
+  const uint32_t nblk = uint32_t(sg.qb - sg.qa);
+  const uint32_t off0 = uint32_t(sg.p_lo - 4 * sg.qa);
+  const uint32_t span_s = uint32_t(sg.p_hi - 4 * sg.qa);
+  int64_t running = int64_t(4 * 64 * int64_t(first_drop)) - int64_t(off0);
+  running = running < 0 ? 0 : running;
+  const int jmax = n - 2 - lo;  // relative clamp (only the last range reaches it)
+  const float2* s_vb = reinterpret_cast<const float2*>(bd);
+  const double* s_c = reinterpret_cast<const double*>(bd);
+
+  auto rows_of = [&](int it, uint32_t i0, int& keepbits, int64_t& row) {
+    const uint32_t i = i0 + uint32_t(lane);
+    const bool full_iter = (i0 > 0 || off0 == 0) && 4 * (i0 + 64) <= span_s;
+    keepbits = 0xF;
+    if (it < first_drop) {
+      if (!full_iter) {
+        keepbits = 0;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const uint32_t r = 4 * i + e;
+          keepbits |= (i < nblk && r >= off0 && r < span_s) ? (1 << e) : 0;
+        }
+      }
+      const int32_t r0 = int32_t(4 * i) - int32_t(off0);
+      row = row0 + (r0 < 0 ? 0 : r0);
+    } else {
+      const uint64_t* mw = mask + int64_t(it) * 4;
+      keepbits = 0;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) keepbits |= int((mw[e] >> lane) & 1ull) << e;
+      int total;
+      row = row0 + running + wave_scan_counts(__popc(keepbits), &total);
+      running += total;
+    }
+  };
+
+  int it = 0;
+  for (uint32_t i0 = 0; i0 < nblk; i0 += 64 * kTrip, it += kTrip) {
+    uint32_t k[4 * kTrip];
+#pragma unroll
+    for (int h = 0; h < kTrip; ++h) {
+      const Words4 wd =
+          philox4x32_10_uniform(sg.qa + i0 + 64 * h + uint32_t(lane), in.stream, P.seed);
+      k[4 * h] = wd.w0 >> 8;
+      k[4 * h + 1] = wd.w1 >> 8;
+      k[4 * h + 2] = wd.w2 >> 8;
+      k[4 * h + 3] = wd.w3 >> 8;
+    }
+    int kb[kTrip];
+    int64_t rw[kTrip];
+#pragma unroll
+    for (int h = 0; h < kTrip; ++h) {
+      kb[h] = 0;
+      rw[h] = 0;
+      if (h == 0 || i0 + 64 * h < nblk) rows_of(it + h, i0 + 64 * h, kb[h], rw[h]);
+    }
+#pragma unroll
+    for (int h = 0; h < kTrip; ++h) {
+      int64_t r = rw[h];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        if (!((kb[h] >> e) & 1)) continue;
+        const int64_t row = r++;
+        const uint32_t kk = k[4 * h + e];
+        if (kk < k_lo || kk >= k_hi) continue;
+        const uint32_t cell = (kk - k_lo) >> cs;
+        int a = int(cell_bin[cell]), b = int(cell_bin[cell + 1]);
+        while (a < b) {  // last t in [a, b] with T_t <= k (T_a <= k holds)
+          const int mid = (a + b + 1) >> 1;
+          if (thr[mid] <= kk) a = mid; else b = mid - 1;
+        }
+        const int jr = a > jmax ? jmax : a;
+        const Real gr = grad[row];
+        if constexpr (fixed) {
+          const float g = float(gr);
+          const float2 ab = s_vb[jr];
+          const float th = __builtin_fmaf(
+              static_cast<float>(static_cast<int32_t>(kk - thr[jr])), ab.x, ab.y);
+          const float gs = g * scale;
+          if (__builtin_expect(__builtin_fabsf(gs) < 0x1.0p30f, 1)) {
+            const int ia = static_cast<int>(__builtin_rintf(gs));
+            const int ib = static_cast<int>(__builtin_rintf(gs * th));
+            atomicAdd(reinterpret_cast<unsigned long long*>(a0 + jr),
+                      static_cast<unsigned long long>(static_cast<int64_t>(ia)));
+            atomicAdd(reinterpret_cast<unsigned long long*>(a1 + jr),
+                      static_cast<unsigned long long>(static_cast<int64_t>(ib)));
+          } else {
+            unsafeAtomicAdd(outl + lo + jr, double(g));
+            unsafeAtomicAdd(outl + nb + lo + jr, double(g) * double(th));
+          }
+        } else {
+          const double ge = double(gr);
+          const double t = double(kk) * 0x1.0p-24 - s_c[jr];
+          atomicAdd(d0 + jr, ge);
+          atomicAdd(d1 + jr, ge * t);
+        }
+      }
+    }
+  }
+  __syncthreads();
+
+  // histograms -> the chunk's partial sums of bins [lo, hi)
+  {
+    const double inv_scale = __builtin_ldexp(1.0, -scale_log2);
+    for (int i = tid; i < nbins; i += nt) {
+      const int j = lo + i;
+      double v0, v1;
+      if constexpr (fixed) {
+        v0 = double(static_cast<int64_t>(a0[i])) * inv_scale + load_l2(outl + j);
+        v1 = (double(static_cast<int64_t>(a1[i])) * inv_scale + load_l2(outl + nb + j)) *
+             vjp_tau_d(T.cdf, j, n);
+      } else {
+        v0 = d0[i];
+        v1 = d1[i];
+      }
+      outl[j] = v0;
+      outl[nb + j] = v1;
+    }
+  }
+  // one device-scope fence per work-group; the last of the instance's
+  // n_chunks * R items sums the chunks' partials in chunk order
+  __syncthreads();
+  const uint32_t arrivals = uint32_t(in.n_chunks) * uint32_t(R);
+  if (tid == 0) {
+    __threadfence();
+    const uint32_t old = atomicAdd(&P.done[ch.instance], 1u);
+    sh_last = old == arrivals - 1u;
+    if (sh_last) __threadfence();
+  }
+  __syncthreads();
+  if (!sh_last) return;
+  double* h0 = outl;
+  double* h1 = outl + nb;
+  for (int64_t i = tid; i < n; i += nt) {  // each i read and written by one thread
+    double sa = 0.0, sb = 0.0;
+    for (int cc = in.first_chunk; cc < in.first_chunk + in.n_chunks; ++cc) {
+      const double* pp = P.partials + P.chunks[cc].partial_off;
+      sa += load_l2(pp + i);
+      sb += load_l2(pp + nb + i);
+    }
+    h0[i] = sa;
+    h1[i] = sb;
+  }
+  if (tid == 0) P.done[ch.instance] = 0u;
+  __syncthreads();
+  vjp_tail<Real, MAXPER>(P, in, T, h0, h1, grad_density, scratch);
+}
+
+// VJP of the instances without bin ranges: one work-group per chunk.
 template <typename Real, int MAXPER>
 __global__ void __launch_bounds__(1024) vjp_kernel(BatchedParams P, const Real* grad,
                                                    Real* grad_density) {
   extern __shared__ __align__(16) uint8_t lds[];
-  if (P.inst[P.chunks[blockIdx.x].instance].global_tables)
-    vjp_body<Real, MAXPER, true>(P, grad, grad_density, lds);
+  const int c = P.vjp_items[blockIdx.x].x;
+  if (P.inst[P.chunks[c].instance].global_tables)
+    vjp_body<Real, MAXPER, true>(P, grad, grad_density, lds, c);
   else
-    vjp_body<Real, MAXPER, false>(P, grad, grad_density, lds);
+    vjp_body<Real, MAXPER, false>(P, grad, grad_density, lds, c);
+}
+
+// VJP of the instances with bin ranges: one work-group per (chunk, range).
+// A curve whose CDF turns out not to be monotone has no key interval per
+// range; its range-0 items take the whole chunk with the global-table VJP.
+template <typename Real, int MAXPER>
+__global__ void __launch_bounds__(1024) vjp_range_kernel(BatchedParams P, const Real* grad,
+                                                         Real* grad_density) {
+  extern __shared__ __align__(16) uint8_t lds[];
+  const int2 item = P.vjp_items[P.n_vjp_plain + blockIdx.x];
+  const InstanceDesc& in = P.inst[P.chunks[item.x].instance];
+  const TableHeader* hdr = reinterpret_cast<const TableHeader*>(P.tables + in.table_off);
+  if (hdr->flags & kMonotone)
+    vjp_range_body<Real, MAXPER>(P, grad, grad_density, lds, item.x, item.y);
+  else if (item.y == 0)
+    vjp_body<Real, MAXPER, true>(P, grad, grad_density, lds, item.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -1335,15 +1688,24 @@ hipError_t launch_pack(const BatchedParams& P, int threads, void* packed, int32_
 template <typename Real>
 hipError_t launch_vjp(const BatchedParams& P, int threads, int64_t lds, int maxper,
                       const void* grad, void* grad_density, hipStream_t stream) {
+  // plain items, then bin-range items (independent instances: either order)
 #define QL_VJP_CASE(MP)                                                                      \
   if (maxper <= MP) {                                                                        \
-    auto kern = vjp_kernel<Real, MP>;                                                        \
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),                 \
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)); \
-    if (e != hipSuccess) return e;                                                           \
-    hipLaunchKernelGGL(kern, dim3(P.n_chunks), dim3(threads), size_t(lds), stream, P,       \
-                       static_cast<const Real*>(grad), static_cast<Real*>(grad_density));    \
-    return hipGetLastError();                                                                \
+    const int64_t grids[2] = {P.n_vjp_plain, P.n_vjp_items - P.n_vjp_plain};                 \
+    for (int kind = 0; kind < 2; ++kind) {                                                   \
+      if (grids[kind] == 0) continue;                                                        \
+      auto kern = kind == 0 ? vjp_kernel<Real, MP> : vjp_range_kernel<Real, MP>;             \
+      hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),               \
+                                         hipFuncAttributeMaxDynamicSharedMemorySize,        \
+                                         int(lds));                                          \
+      if (e != hipSuccess) return e;                                                         \
+      hipLaunchKernelGGL(kern, dim3(unsigned(grids[kind])), dim3(threads), size_t(lds),      \
+                         stream, P, static_cast<const Real*>(grad),                          \
+                         static_cast<Real*>(grad_density));                                  \
+      e = hipGetLastError();                                                                 \
+      if (e != hipSuccess) return e;                                                         \
+    }                                                                                        \
+    return hipSuccess;                                                                       \
   }
   QL_VJP_CASE(2)
   QL_VJP_CASE(4)

@@ -12,7 +12,8 @@ namespace qloits {
 struct BatchedParams {
   const InstanceDesc* inst;
   const ChunkDesc* chunks;
-  int64_t n_instances, n_chunks;
+  const int2* vjp_items;  // VJP work items {chunk, bin range}: plain items, then ranged
+  int64_t n_instances, n_chunks, n_vjp_items, n_vjp_plain;
   const void* x;
   const void* density;
   uint8_t* tables;

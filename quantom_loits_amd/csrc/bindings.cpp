@@ -305,6 +305,9 @@ class BatchedPlan {
     d["mask_words"] = layout_.mask_words;
     d["partial_doubles"] = layout_.partial_doubles;
     d["global_instances"] = layout_.global_instances;
+    d["range_instances"] = layout_.range_instances;
+    d["grid_backward"] = layout_.grid_backward;
+    d["backward_plain"] = layout_.backward_plain;
     d["dtype"] = layout_.dtype;
     d["threads"] = layout_.threads;
     d["scan_mode"] = layout_.scan_mode;

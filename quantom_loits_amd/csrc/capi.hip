@@ -70,9 +70,15 @@ int ceil_log2(int64_t v) {
 
 int64_t align16(int64_t v) { return (v + 15) & ~int64_t(15); }
 
+struct VjpItem {
+  int32_t chunk, range;
+};
+
 struct Plan {
   std::vector<InstanceDesc> inst;
   std::vector<ChunkDesc> chunks;
+  std::vector<VjpItem> items;  // VJP work items: one per chunk, then one per (chunk, range)
+  int64_t n_plain = 0;
   loits_hip_layout layout;
 };
 
@@ -105,9 +111,11 @@ int make_plan(const loits_hip_instance* instances, int64_t n, int dtype,
   L.persistent = tuning->persistent;
   int64_t out = 0, table_off = 0, mask_words = 0, partial_doubles = 0;
   int64_t lds_f = 0, lds_b = 0, lds_t = 0;
-  int32_t n_global = 0;
+  int32_t n_global = 0, n_ranged = 0;
+  std::vector<VjpItem> ranged;  // appended after the plain items
   plan->inst.resize(size_t(n));
   plan->chunks.clear();
+  plan->items.clear();
   for (int64_t i = 0; i < n; ++i) {
     const loits_hip_instance& s = instances[i];
     if (s.n_nodes < 2 || s.n_nodes > 32767)
@@ -129,18 +137,26 @@ int make_plan(const loits_hip_instance* instances, int64_t n, int dtype,
     d.counter_base = s.counter_base;
     d.stream = s.stream;
     d.accept = s.accept ? 1 : 0;
-    // Finest guide that leaves the forward's LDS build and the VJP's tables and
-    // histograms within LDS; when none does, the instance keeps its per-bin
-    // tables in global memory and only the guide is staged in LDS.
+    // Forward: the finest guide whose LDS build fits; when none does, the
+    // instance keeps its per-bin tables in global memory (built by a pre-pass)
+    // and only the guide is staged in LDS.  VJP: tables and histograms in LDS
+    // at the finest guide that fits both kernels; otherwise bin ranges.
     const int g_want = std::min(kMaxGuideLog2,
                                 std::max(kMinGuideLog2, ceil_log2(s.n_nodes) + tuning->guide_shift));
-    auto lds_mode_bytes = [&](int g) {
-      return std::max(lds_build_layout(s.n_nodes, g, coef_size).bytes,
-                      lds_table_bytes(s.n_nodes, g, coef_size) + pad4(s.n_nodes) * 16);
+    auto build_bytes = [&](int g) { return lds_build_layout(s.n_nodes, g, coef_size).bytes; };
+    auto vjp_bytes = [&](int g) {
+      return lds_table_bytes(s.n_nodes, g, coef_size) + pad4(s.n_nodes) * 16;
     };
     d.guide_log2 = g_want;
-    while (d.guide_log2 > kMinGuideLog2 && lds_mode_bytes(d.guide_log2) > kMaxLds) --d.guide_log2;
-    d.global_tables = lds_mode_bytes(d.guide_log2) > kMaxLds ? 1 : 0;
+    while (d.guide_log2 > kMinGuideLog2 &&
+           std::max(build_bytes(d.guide_log2), vjp_bytes(d.guide_log2)) > kMaxLds)
+      --d.guide_log2;
+    if (vjp_bytes(d.guide_log2) > kMaxLds) {  // the VJP takes bin ranges; fit the build only
+      d.guide_log2 = g_want;
+      while (d.guide_log2 > kMinGuideLog2 && build_bytes(d.guide_log2) > kMaxLds) --d.guide_log2;
+      d.vjp_ranges = vjp_range_count(s.n_nodes);
+    }
+    d.global_tables = build_bytes(d.guide_log2) > kMaxLds ? 1 : 0;
     if (d.global_tables) d.guide_log2 = g_want;
     d.table_off = table_off;
     const TableLayout tl = table_layout(s.n_nodes, d.guide_log2, coef_size);
@@ -149,11 +165,18 @@ int make_plan(const loits_hip_instance* instances, int64_t n, int dtype,
     if (d.global_tables) {
       ++n_global;
       lds_f = std::max(lds_f, guide_bytes);
-      lds_b = std::max(lds_b, guide_bytes);
       lds_t = std::max(lds_t, lds_global_build_bytes(d.guide_log2));
     } else {
-      lds_f = std::max(lds_f, lds_build_layout(s.n_nodes, d.guide_log2, coef_size).bytes);
-      lds_t = std::max(lds_t, lds_build_layout(s.n_nodes, d.guide_log2, coef_size).bytes);
+      lds_f = std::max(lds_f, build_bytes(d.guide_log2));
+      lds_t = std::max(lds_t, build_bytes(d.guide_log2));
+    }
+    if (d.vjp_ranges > 0) {
+      // (a curve that turns out not to be monotone takes the global-table VJP,
+      // which stages the guide)
+      ++n_ranged;
+      lds_b = std::max(lds_b, std::max(guide_bytes,
+                                       lds_range_bytes(vjp_range_width(s.n_nodes, d.vjp_ranges))));
+    } else {
       lds_b = std::max(lds_b, (tl.cdf - tl.thr) + pad4(s.n_nodes) * 16);
     }
     L.max_nodes = std::max(L.max_nodes, s.n_nodes);
@@ -177,14 +200,24 @@ int make_plan(const loits_hip_instance* instances, int64_t n, int dtype,
       mask_words += per_wave * nwaves * 4;
       ch.partial_off = partial_doubles;  // VJP overflow sums, then partials
       partial_doubles += 2 * pad4(s.n_nodes);
+      if (d.vjp_ranges == 0)
+        plan->items.push_back(VjpItem{int32_t(plan->chunks.size()), 0});
+      else
+        for (int r = 0; r < d.vjp_ranges; ++r)
+          ranged.push_back(VjpItem{int32_t(plan->chunks.size()), r});
       plan->chunks.push_back(ch);
     }
     out += s.events;
   }
   L.capacity = out;
   L.n_chunks = int64_t(plan->chunks.size());
+  plan->n_plain = int64_t(plan->items.size());
+  plan->items.insert(plan->items.end(), ranged.begin(), ranged.end());
+  if (plan->items.size() > size_t(INT32_MAX))
+    return fail(LOITS_HIP_EUNSUPPORTED, "batched: too many VJP work items");
   L.desc_bytes = align16(n * int64_t(sizeof(InstanceDesc))) +
-                 L.n_chunks * int64_t(sizeof(ChunkDesc));
+                 align16(L.n_chunks * int64_t(sizeof(ChunkDesc))) +
+                 int64_t(plan->items.size()) * int64_t(sizeof(VjpItem));
   L.table_bytes = std::max<int64_t>(table_off, 16);
   L.lds_forward = lds_f;
   L.lds_backward = lds_b;
@@ -203,8 +236,10 @@ int make_plan(const loits_hip_instance* instances, int64_t n, int dtype,
   L.scratch_bytes = 64 + align16(8 * L.n_segments) * 3 + align16(4 * L.n_segments) + align16(4 * n) +
                     align16(8 * mask_words) + align16(8 * partial_doubles);
   L.grid_forward = int32_t(L.n_chunks);
-  L.grid_backward = int32_t(L.n_chunks);
+  L.grid_backward = int32_t(plan->items.size());
+  L.backward_plain = plan->n_plain;
   L.global_instances = n_global;
+  L.range_instances = n_ranged;
   return LOITS_HIP_OK;
 }
 
@@ -213,10 +248,14 @@ BatchedParams params_of(const loits_hip_batched_args* a) {
   BatchedParams P{};
   const uint8_t* desc = static_cast<const uint8_t*>(a->desc);
   P.inst = reinterpret_cast<const InstanceDesc*>(desc);
-  P.chunks = reinterpret_cast<const ChunkDesc*>(
-      desc + align16(L.n_instances * int64_t(sizeof(InstanceDesc))));
+  const int64_t chunks_at = align16(L.n_instances * int64_t(sizeof(InstanceDesc)));
+  P.chunks = reinterpret_cast<const ChunkDesc*>(desc + chunks_at);
+  P.vjp_items = reinterpret_cast<const int2*>(
+      desc + chunks_at + align16(L.n_chunks * int64_t(sizeof(ChunkDesc))));
   P.n_instances = L.n_instances;
   P.n_chunks = L.n_chunks;
+  P.n_vjp_items = L.grid_backward;
+  P.n_vjp_plain = L.backward_plain;
   P.x = a->x;
   P.density = a->density;
   P.tables = static_cast<uint8_t*>(a->tables);
@@ -340,8 +379,10 @@ int loits_hip_batched_encode(const loits_hip_instance* instances, int64_t n,
     uint8_t* out = static_cast<uint8_t*>(host_desc);
     std::memset(out, 0, size_t(layout->desc_bytes));
     std::memcpy(out, plan.inst.data(), plan.inst.size() * sizeof(InstanceDesc));
-    std::memcpy(out + align16(n * int64_t(sizeof(InstanceDesc))), plan.chunks.data(),
-                plan.chunks.size() * sizeof(ChunkDesc));
+    const int64_t chunks_at = align16(n * int64_t(sizeof(InstanceDesc)));
+    std::memcpy(out + chunks_at, plan.chunks.data(), plan.chunks.size() * sizeof(ChunkDesc));
+    std::memcpy(out + chunks_at + align16(int64_t(plan.chunks.size() * sizeof(ChunkDesc))),
+                plan.items.data(), plan.items.size() * sizeof(VjpItem));
     return LOITS_HIP_OK;
   } catch (const std::exception& e) {
     return fail(LOITS_HIP_ERUNTIME, std::string("batched_encode: ") + e.what());

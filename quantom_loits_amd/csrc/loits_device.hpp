@@ -204,7 +204,7 @@ struct InstanceDesc {
   int32_t first_chunk;   // chunks of an instance are consecutive
   int32_t n_chunks;
   int32_t global_tables; // 1: per-bin tables too large for LDS stay in global memory
-  int32_t reserved;
+  int32_t vjp_ranges;    // 0: VJP tables and histograms in LDS; R > 0: R bin ranges
 };
 
 // A chunk is the work of one work-group; each of its waves owns a contiguous
@@ -403,6 +403,28 @@ __host__ __device__ __forceinline__ LdsLayout lds_build_layout(int64_t n, int gu
 __host__ __device__ __forceinline__ int64_t lds_global_build_bytes(int guide_log2) {
   const int64_t G = int64_t(1) << guide_log2;
   return (G / 4) * 8 + G * 4;
+}
+
+// Bin-range VJP (curves whose VJP tables and histograms do not fit LDS): the
+// bins of a curve are split into R ranges of at most kRangeBins bins; one work
+// item (chunk, range) sums the keys of its range over a cell table of at most
+// kRangeCells cells.  LDS: cell_bin[kRangeCells + 4] | thr[W + 4] | per-bin
+// data[W] (8 B) | two 64-bit histograms[W].
+constexpr int kRangeBins = 4096;
+constexpr int kRangeCellsLog2 = 13;
+constexpr int kRangeCells = 1 << kRangeCellsLog2;
+
+__host__ __device__ __forceinline__ int vjp_range_count(int64_t n) {
+  return int((n - 1 + kRangeBins - 1) / kRangeBins);
+}
+
+// Bins per range (a multiple of 4); every range is non-empty.
+__host__ __device__ __forceinline__ int vjp_range_width(int64_t n, int ranges) {
+  return int(pad4((n - 1 + ranges - 1) / ranges));
+}
+
+__host__ __device__ __forceinline__ int64_t lds_range_bytes(int width) {
+  return int64_t(kRangeCells + 4) * 4 + int64_t(width + 4) * 4 + int64_t(width) * 24;
 }
 
 // Bin of key k from a direct guide word; ko = k - L (the offset in the cell).

@@ -51,7 +51,7 @@ class Layout(C.Structure):
         "table_bytes", "scratch_bytes", "lds_forward", "lds_backward", "lds_tables", "n_segments",
         "mask_words", "partial_doubles")] + [(n, C.c_int32) for n in (
             "dtype", "threads", "scan_mode", "persistent", "grid_forward", "grid_backward",
-            "global_instances", "reserved")]
+            "global_instances", "range_instances")] + [("backward_plain", C.c_int64)]
 
 
 def make_instances(nodes, events):
@@ -183,3 +183,18 @@ def test_large_curves_use_global_tables(lib):
     assert lib.loits_hip_batched_layout(inst, 3, 1, C.byref(tun), C.byref(lay)) == 0
     assert lay.global_instances == 2
     assert lay.lds_forward <= 160 * 1024 and lay.lds_backward <= 160 * 1024
+    # VJP bin ranges of at most 4096 bins: 6000 nodes -> 2, 16384 -> 4 work items
+    assert lay.range_instances == 2 and lay.grid_backward == 1 + 2 + 4
+    assert lay.backward_plain == 1
+
+
+def test_vjp_bin_ranges_without_global_tables(lib):
+    """4096 nodes: the forward builds its tables in LDS, the VJP takes one bin range;
+    2500 nodes keep both in LDS."""
+    inst = make_instances([4096, 2500], [300000, 10])
+    tun = Tuning(1024, 2, 1 << 17, 0, 0)
+    lay = Layout()
+    assert lib.loits_hip_batched_layout(inst, 2, 1, C.byref(tun), C.byref(lay)) == 0
+    assert lay.global_instances == 0 and lay.range_instances == 1
+    assert lay.n_chunks == 3 + 1 and lay.grid_backward == 4 and lay.backward_plain == 1
+    assert lay.lds_backward <= 160 * 1024 - 2048

@@ -381,3 +381,131 @@ def test_batched_spiky_densities_exercise_guide_overflow(oracle_port, hip_device
     ref = ref_g.reshape(B, N)
     err = np.abs(got - ref).max(axis=1) / np.abs(ref).max(axis=1)
     assert err.max() < 1e-9
+
+
+def test_batched_vjp_matches_float64_central_differences(hip_device):
+    """d/d density_i of L = sum_r w_r v_r by central differences (h = 1e-6) on
+    16 random nodes per curve, float64, no window (kept rows do not move).
+
+    The value is x_j + dx_j (u - C_j) / (C_{j+1} - C_j + eps): at a bin
+    boundary it jumps by dx_j eps / (d_j + eps), so a difference quotient
+    across a bin change measures that jump, not the derivative.  A node whose
+    perturbation moves any event to another bin is retried with h / 10 (twice)
+    and otherwise left out; at most 2 of the 48 nodes may be left out.
+    """
+    from quantom_loits_amd.batched import BatchedLOITS, Tuning
+    from quantom_loits_amd.workloads import uniform_family
+
+    B, N, E, seed = 3, 256, 20000, 5
+    x, d = uniform_family(B, N, seed=8, dtype=np.float64)
+    s = BatchedLOITS.uniform(B, N, E, 1, dtype=torch.float64,
+                             tuning=Tuning(threads=256, chunk_events=4096, scan_mode=1))
+    xt = torch.from_numpy(x.reshape(-1)).cuda()
+    dt = torch.from_numpy(d.reshape(-1)).cuda()
+    w = torch.from_numpy(np.random.default_rng(9).standard_normal(B * E)).cuda()
+
+    def loss(dens):
+        v, _, _, ex = s.run_forward(xt, dens, seed, want_bins=True)
+        return float((v * w).sum()), ex["bins"]
+
+    values, kept, st, ex0 = s.run_forward(xt, dt, seed, want_bins=True)
+    bins0 = ex0["bins"].clone()
+    gd = torch.empty_like(dt)
+    s.backward_raw(xt, dt, seed, st, w, gd)
+    got = gd.cpu().numpy()
+    rng = np.random.default_rng(10)
+    skipped = 0
+    for b in range(B):
+        nodes = rng.choice(N, 16, replace=False) + b * N
+        scale = np.abs(got[b * N:(b + 1) * N]).max()
+        worst = 0.0
+        for i in nodes:
+            for h in (1e-6, 1e-7, 1e-8):
+                dp = dt.clone()
+                dp[i] += h
+                dm = dt.clone()
+                dm[i] -= h
+                lp, bp = loss(dp)
+                lm, bm = loss(dm)
+                if torch.equal(bp, bins0) and torch.equal(bm, bins0):
+                    worst = max(worst, abs(got[i] - (lp - lm) / (2 * h)) / scale)
+                    break
+            else:
+                skipped += 1
+        print("finite differences curve", b, "max rel err", worst, flush=True)
+        assert worst <= 1e-4, (b, worst)
+    assert skipped <= 2, skipped
+
+
+def _big_curves(kind, rng):
+    """(n_nodes, events, x, density) of curves too large for an LDS-resident VJP."""
+    from quantom_loits_amd.workloads import density_params, eval_density
+
+    if kind == "smooth":
+        nodes, events = [4096, 32767, 9000], [150000, 60000, 100000]
+    elif kind == "spiky":
+        nodes, events = [16384, 6000], [80000, 50000]
+    else:  # negative densities: a CDF that is not monotone
+        nodes, events = [5000, 4500], [40000, 30000]
+    xs, ds = [], []
+    params = density_params(len(nodes), 21)
+    for i, n in enumerate(nodes):
+        xg = np.linspace(0.001, 0.999, n)
+        if kind == "smooth":
+            d = eval_density(xg, params[i])
+        elif kind == "spiky":
+            d = np.full(n, 1e-9)
+            for mu in rng.uniform(0.1, 0.9, 6):
+                d += np.exp(-0.5 * ((xg - mu) / rng.uniform(1e-4, 2e-3)) ** 2)
+        else:
+            d = 1.0 + 0.3 * (4.0 + 2 * i) * np.sin(np.linspace(0, 12, n))
+        xs.append(xg)
+        ds.append(d)
+    return np.array(nodes), np.array(events), np.concatenate(xs), np.concatenate(ds)
+
+
+@pytest.mark.parametrize("kind,dtype", [("smooth", torch.float32), ("smooth", torch.float64),
+                                        ("spiky", torch.float64), ("spiky", torch.float32),
+                                        ("nonmono", torch.float64)])
+def test_batched_vjp_bin_ranges_match_oracle(oracle_port, hip_device, kind, dtype):
+    """Curves whose VJP tables and histograms exceed LDS: the VJP splits the bins
+    into ranges of at most 4096 (one work-group per chunk and range); a curve
+    that is not monotone takes the global-table VJP instead."""
+    from quantom_loits_amd.batched import BatchedLOITS, Tuning
+
+    rng = np.random.default_rng(31)
+    nodes, events, x, d = _big_curves(kind, rng)
+    n = len(nodes)
+    node_off = np.concatenate([[0], np.cumsum(nodes)[:-1]]).astype(np.int64)
+    base = rng.integers(0, 2 ** 40, n).astype(np.uint64)
+    streams = rng.integers(0, 2 ** 33, n).astype(np.uint64)
+    accept = np.ones(n, np.uint8)
+    window = (0.05, 0.95) if kind == "smooth" else None
+    s = BatchedLOITS(nodes, events, base, streams, accept, node_off, dtype=dtype, window=window,
+                     tuning=Tuning(threads=1024, chunk_events=65536,
+                                   scan_mode=1 if dtype == torch.float64 else 0))
+    assert s.layout["range_instances"] == n
+    print("bin ranges", kind, dtype, nodes.tolist(), s.layout["grid_backward"], flush=True)
+    xt = torch.from_numpy(x).to(dtype).cuda()
+    dt = torch.from_numpy(d).to(dtype).cuda()
+    x64, d64 = xt.double().cpu().numpy(), dt.double().cpu().numpy()
+    ref_v, ref_b, ref_k, _, _ = oracle_port.batched_forward(
+        x64, d64, node_off, nodes, events, base, streams, 9, accept=accept, window=window)
+    values, kept, st, ex = s.run_forward(xt, dt, 9, want_bins=True)
+    assert kept.cpu().numpy().tolist() == ref_k.tolist()
+    assert np.array_equal(ex["bins"].cpu().numpy(), ref_b)
+    gv = np.random.default_rng(32).standard_normal(len(ref_v))
+    ref_g = oracle_port.batched_backward(x64, d64, node_off, nodes, events, base, streams, 9, gv,
+                                         accept=accept, window=window)
+    gd = torch.empty_like(dt)
+    s.backward_raw(xt, dt, 9, st, torch.from_numpy(gv).to(dtype).cuda(), gd)
+    # twice in a row on the same buffers: counters and partial sums reset themselves
+    gd2 = torch.empty_like(dt)
+    s.backward_raw(xt, dt, 9, st, torch.from_numpy(gv).to(dtype).cuda(), gd2)
+    for call, g in enumerate((gd, gd2)):
+        got = g.double().cpu().numpy()
+        for i in range(n):
+            sl = slice(int(node_off[i]), int(node_off[i] + nodes[i]))
+            err = np.abs(got[sl] - ref_g[sl]).max() / np.abs(ref_g[sl]).max()
+            print("  call", call, "curve", i, "nodes", nodes[i], "grad rel err", err, flush=True)
+            assert err <= (1e-9 if dtype == torch.float64 else 1e-4), (call, i, err)
