@@ -1022,6 +1022,7 @@ __device__ __forceinline__ void vjp_body(const BatchedParams& P, const Real* gra
   __shared__ double scratch[16];
   __shared__ int sh_last;
   __shared__ uint32_t sh_gmax;
+  __shared__ int sh_outl;  // some cotangent took the fp64 overflow sums
   const ChunkDesc ch = P.chunks[c];
   const InstanceDesc in = P.inst[ch.instance];
   Tables<Real> T = tables_of<Real>(P.tables, in);
@@ -1068,7 +1069,10 @@ __device__ __forceinline__ void vjp_body(const BatchedParams& P, const Real* gra
     h0[i] = 0.0;
     outl[i] = 0.0;
   }
-  if (tid == 0) sh_gmax = 0u;
+  if (tid == 0) {
+    sh_gmax = 0u;
+    sh_outl = 0;
+  }
   __syncthreads();
   const Coef* s_coef = reinterpret_cast<const Coef*>(s_bin);
   const float2* s_vb = reinterpret_cast<const float2*>(s_bin);
@@ -1136,6 +1140,7 @@ __device__ __forceinline__ void vjp_body(const BatchedParams& P, const Real* gra
       atomicAdd(reinterpret_cast<unsigned long long*>(a1 + jj),
                 static_cast<unsigned long long>(static_cast<int64_t>(ib)));
     } else {
+      sh_outl = 1;
       atomicAdd(outl + jj, double(g));
       atomicAdd(outl + nb + jj, double(g) * double(th));
     }
@@ -1257,11 +1262,14 @@ __device__ __forceinline__ void vjp_body(const BatchedParams& P, const Real* gra
 
   if (fixed) {  // integers (and overflow sums) -> h0 = sum g, h1 = sum g t, in place
     const double inv_scale = __builtin_ldexp(1.0, -scale_log2);
+    const bool any_outl = sh_outl != 0;
     for (int64_t i = tid; i < nb; i += nt) {
       double v0 = 0.0, v1 = 0.0;
       if (i < n - 1) {
-        v0 = double(static_cast<int64_t>(a0[i])) * inv_scale + load_l2(outl + i);
-        v1 = (double(static_cast<int64_t>(a1[i])) * inv_scale + load_l2(outl + nb + i)) *
+        v0 = double(static_cast<int64_t>(a0[i])) * inv_scale +
+             (any_outl ? load_l2(outl + i) : 0.0);
+        v1 = (double(static_cast<int64_t>(a1[i])) * inv_scale +
+              (any_outl ? load_l2(outl + nb + i) : 0.0)) *
              vjp_tau_d(T.cdf, i, n);
       }
       h0[i] = v0;
@@ -1342,6 +1350,7 @@ __device__ void vjp_range_body(const BatchedParams& P, const Real* grad, Real* g
   __shared__ int sh_last;
   __shared__ uint32_t sh_gmax;
   __shared__ uint32_t sh_wmax[16];
+  __shared__ int sh_outl;  // some cotangent took the fp64 overflow sums
   constexpr bool fixed = sizeof(Real) == 4;
   const ChunkDesc ch = P.chunks[c];
   const InstanceDesc in = P.inst[ch.instance];
@@ -1394,7 +1403,10 @@ __device__ void vjp_range_body(const BatchedParams& P, const Real* grad, Real* g
       outl[nb + i] = 0.0;
     }
   for (int i = tid; i <= ncell; i += nt) cell_bin[i] = 0u;
-  if (tid == 0) sh_gmax = 0u;
+  if (tid == 0) {
+    sh_gmax = 0u;
+    sh_outl = 0;
+  }
   __syncthreads();
   // Bin t (relative) starts the cells whose first key lies in [T_t, T_next);
   // those cells are disjoint across bins, so each run's first cell gets one
@@ -1504,9 +1516,13 @@ __device__ void vjp_range_body(const BatchedParams& P, const Real* grad, Real* g
     }
   };
 
+  // Per trip, 4 * kTrip keys per lane go through each step together (their
+  // LDS reads are independent): cell bounds, bisection, per-bin data, then the
+  // cotangents and the histogram updates of the kept keys inside the range.
+  constexpr int NK = 4 * kTrip;
   int it = 0;
   for (uint32_t i0 = 0; i0 < nblk; i0 += 64 * kTrip, it += kTrip) {
-    uint32_t k[4 * kTrip];
+    uint32_t k[NK];
 #pragma unroll
     for (int h = 0; h < kTrip; ++h) {
       const Words4 wd =
@@ -1524,73 +1540,129 @@ __device__ void vjp_range_body(const BatchedParams& P, const Real* grad, Real* g
       rw[h] = 0;
       if (h == 0 || i0 + 64 * h < nblk) rows_of(it + h, i0 + 64 * h, kb[h], rw[h]);
     }
+    uint32_t use = 0u;  // kept and inside [K_lo, K_hi)
+    int a[NK], b[NK];
+#pragma unroll
+    for (int e = 0; e < NK; ++e) {
+      const bool in_range = ((kb[e >> 2] >> (e & 3)) & 1) && k[e] >= k_lo && k[e] < k_hi;
+      use |= in_range ? (1u << e) : 0u;
+      const uint32_t cell = in_range ? (k[e] - k_lo) >> cs : 0u;
+      a[e] = int(cell_bin[cell]);
+      b[e] = int(cell_bin[cell + 1]);
+    }
+    uint32_t pending = 0u;
+#pragma unroll
+    for (int e = 0; e < NK; ++e) pending |= (((use >> e) & 1u) && a[e] < b[e]) ? (1u << e) : 0u;
+    while (pending) {  // last t in [a, b] with T_t <= k (T_a <= k holds); usually one step
+#pragma unroll
+      for (int e = 0; e < NK; ++e) {
+        if (!((pending >> e) & 1u)) continue;
+        const int mid = (a[e] + b[e] + 1) >> 1;
+        if (thr[mid] <= k[e]) a[e] = mid; else b[e] = mid - 1;
+        if (a[e] >= b[e]) pending &= ~(1u << e);
+      }
+    }
+    float th[NK];
+    double tc[NK];
+#pragma unroll
+    for (int e = 0; e < NK; ++e) {
+      a[e] = a[e] > jmax ? jmax : a[e];
+      if constexpr (fixed) {
+        const float2 ab = s_vb[a[e]];
+        th[e] = __builtin_fmaf(static_cast<float>(static_cast<int32_t>(k[e] - thr[a[e]])), ab.x,
+                               ab.y);
+      } else {
+        tc[e] = double(k[e]) * 0x1.0p-24 - s_c[a[e]];
+      }
+    }
 #pragma unroll
     for (int h = 0; h < kTrip; ++h) {
-      int64_t r = rw[h];
+      const uint32_t uh = (use >> (4 * h)) & 0xFu;
+      if (uh == 0u) continue;
+      Real g[4];
+      if (R == 1 && kb[h] == 0xF && (rw[h] & 3) == 0) {
+        if constexpr (fixed) {
+          const float4 g4 = *reinterpret_cast<const float4*>(grad + rw[h]);
+          g[0] = g4.x; g[1] = g4.y; g[2] = g4.z; g[3] = g4.w;
+        } else {
+          const double2 ga = reinterpret_cast<const double2*>(grad + rw[h])[0];
+          const double2 gb = reinterpret_cast<const double2*>(grad + rw[h])[1];
+          g[0] = ga.x; g[1] = ga.y; g[2] = gb.x; g[3] = gb.y;
+        }
+      } else {
+        int64_t r = rw[h];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          g[e] = ((uh >> e) & 1u) ? grad[r] : Real(0);
+          r += (kb[h] >> e) & 1;
+        }
+      }
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        if (!((kb[h] >> e) & 1)) continue;
-        const int64_t row = r++;
-        const uint32_t kk = k[4 * h + e];
-        if (kk < k_lo || kk >= k_hi) continue;
-        const uint32_t cell = (kk - k_lo) >> cs;
-        int a = int(cell_bin[cell]), b = int(cell_bin[cell + 1]);
-        while (a < b) {  // last t in [a, b] with T_t <= k (T_a <= k holds)
-          const int mid = (a + b + 1) >> 1;
-          if (thr[mid] <= kk) a = mid; else b = mid - 1;
-        }
-        const int jr = a > jmax ? jmax : a;
-        const Real gr = grad[row];
+        if (!((uh >> e) & 1u)) continue;
+        const int jr = a[4 * h + e];
         if constexpr (fixed) {
-          const float g = float(gr);
-          const float2 ab = s_vb[jr];
-          const float th = __builtin_fmaf(
-              static_cast<float>(static_cast<int32_t>(kk - thr[jr])), ab.x, ab.y);
-          const float gs = g * scale;
+          const float gf = float(g[e]);
+          const float gs = gf * scale;
           if (__builtin_expect(__builtin_fabsf(gs) < 0x1.0p30f, 1)) {
             const int ia = static_cast<int>(__builtin_rintf(gs));
-            const int ib = static_cast<int>(__builtin_rintf(gs * th));
+            const int ib = static_cast<int>(__builtin_rintf(gs * th[4 * h + e]));
             atomicAdd(reinterpret_cast<unsigned long long*>(a0 + jr),
                       static_cast<unsigned long long>(static_cast<int64_t>(ia)));
             atomicAdd(reinterpret_cast<unsigned long long*>(a1 + jr),
                       static_cast<unsigned long long>(static_cast<int64_t>(ib)));
           } else {
-            unsafeAtomicAdd(outl + lo + jr, double(g));
-            unsafeAtomicAdd(outl + nb + lo + jr, double(g) * double(th));
+            sh_outl = 1;
+            unsafeAtomicAdd(outl + lo + jr, double(gf));
+            unsafeAtomicAdd(outl + nb + lo + jr, double(gf) * double(th[4 * h + e]));
           }
         } else {
-          const double ge = double(gr);
-          const double t = double(kk) * 0x1.0p-24 - s_c[jr];
+          const double ge = double(g[e]);
           atomicAdd(d0 + jr, ge);
-          atomicAdd(d1 + jr, ge * t);
+          atomicAdd(d1 + jr, ge * tc[4 * h + e]);
         }
       }
     }
   }
   __syncthreads();
 
-  // histograms -> the chunk's partial sums of bins [lo, hi)
+  // histograms -> sums (h0 = sum g, h1 = sum g t): in LDS when this item is
+  // the instance's only one (its tail reads them there), else into the
+  // chunk's partial sums of bins [lo, hi)
+  const bool any_outl = sh_outl != 0;
+  const uint32_t arrivals = uint32_t(in.n_chunks) * uint32_t(R);
   {
     const double inv_scale = __builtin_ldexp(1.0, -scale_log2);
     for (int i = tid; i < nbins; i += nt) {
       const int j = lo + i;
       double v0, v1;
       if constexpr (fixed) {
-        v0 = double(static_cast<int64_t>(a0[i])) * inv_scale + load_l2(outl + j);
-        v1 = (double(static_cast<int64_t>(a1[i])) * inv_scale + load_l2(outl + nb + j)) *
+        v0 = double(static_cast<int64_t>(a0[i])) * inv_scale +
+             (any_outl ? load_l2(outl + j) : 0.0);
+        v1 = (double(static_cast<int64_t>(a1[i])) * inv_scale +
+              (any_outl ? load_l2(outl + nb + j) : 0.0)) *
              vjp_tau_d(T.cdf, j, n);
       } else {
         v0 = d0[i];
         v1 = d1[i];
       }
-      outl[j] = v0;
-      outl[nb + j] = v1;
+      if (arrivals == 1u) {  // in place (each i read and written by one thread)
+        d0[i] = v0;
+        d1[i] = v1;
+      } else {
+        outl[j] = v0;
+        outl[nb + j] = v1;
+      }
     }
+  }
+  if (arrivals == 1u) {
+    __syncthreads();
+    vjp_tail<Real, MAXPER>(P, in, T, d0, d1, grad_density, scratch);
+    return;
   }
   // one device-scope fence per work-group; the last of the instance's
   // n_chunks * R items sums the chunks' partials in chunk order
   __syncthreads();
-  const uint32_t arrivals = uint32_t(in.n_chunks) * uint32_t(R);
   if (tid == 0) {
     __threadfence();
     const uint32_t old = atomicAdd(&P.done[ch.instance], 1u);

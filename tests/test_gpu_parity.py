@@ -509,3 +509,33 @@ def test_batched_vjp_bin_ranges_match_oracle(oracle_port, hip_device, kind, dtyp
             err = np.abs(got[sl] - ref_g[sl]).max() / np.abs(ref_g[sl]).max()
             print("  call", call, "curve", i, "nodes", nodes[i], "grad rel err", err, flush=True)
             assert err <= (1e-9 if dtype == torch.float64 else 1e-4), (call, i, err)
+
+
+@pytest.mark.parametrize("nodes", [2048, 9000])
+def test_batched_float32_vjp_heavy_tailed_cotangents(oracle_port, hip_device, nodes):
+    """Rare cotangents 1e9 times larger than the rest overflow the VJP's fixed-point
+    scale (set from each segment's first rows) and take its fp64 overflow sums;
+    2048 nodes: LDS-resident VJP, 9000 nodes: bin ranges."""
+    from quantom_loits_amd.batched import BatchedLOITS, Tuning
+    from quantom_loits_amd.workloads import uniform_family
+
+    B, E = 3, 200000
+    x, d = uniform_family(B, nodes, seed=4, dtype=np.float32)
+    s = BatchedLOITS.uniform(B, nodes, E, 0, dtype=torch.float32,
+                             tuning=Tuning(threads=1024, chunk_events=1 << 16))
+    xt = torch.from_numpy(x.reshape(-1)).cuda()
+    dt = torch.from_numpy(d.reshape(-1)).cuda()
+    values, kept, st, _ = s.run_forward(xt, dt, 3)
+    rng = np.random.default_rng(6)
+    gv = rng.standard_normal(values.numel()).astype(np.float32)
+    gv[rng.integers(0, gv.size, gv.size // 1000)] *= 1e9
+    gd = torch.empty_like(dt)
+    s.backward_raw(xt, dt, 3, st, torch.from_numpy(gv).cuda(), gd)
+    desc = uniform_batch(B, nodes, E, 0)
+    ref = oracle_port.batched_backward(x.reshape(-1).astype(np.float64),
+                                       d.reshape(-1).astype(np.float64), *desc, 3,
+                                       gv.astype(np.float64)).reshape(B, nodes)
+    got = gd.double().cpu().numpy().reshape(B, nodes)
+    err = np.abs(got - ref).max(axis=1) / np.abs(ref).max(axis=1)
+    print("heavy-tailed cotangents", nodes, err, flush=True)
+    assert err.max() <= 1e-4
