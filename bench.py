@@ -144,6 +144,45 @@ def extra_workloads(dev, steps):
     out["ragged_4096_window"] = dict(curves=nn, events=ev, kept=kept, fused_graph_ms=fused,
                                      fused_events_per_s=ev / (fused * 1e-3),
                                      per_curve_loop_ms=looped, speedup=looped / fused)
+    del singles, sr
+    out["dropin_two_axis"] = dropin_workloads(dev, steps)
+    return out
+
+
+def dropin_workloads(dev, steps):
+    """The reference-API path (LOITS(backend="hip") bindings, both axes) at the
+    reference's own GAN shapes: grid g x g cells of K = 10 nodes per axis, one
+    theory instance; forward + backward, the forward's two host reads included."""
+    import torch
+    from quantom_loits_amd import load_extension
+
+    ext = load_extension()
+    out = {}
+    rng = np.random.default_rng(3)
+    for grid, n_events in ((10, 10000), (99, 10000), (99, 1000000), (99, 10000000)):
+        k = 10
+        xb = np.sort(rng.uniform(0.001, 0.999, (1, grid, k)), axis=2)
+        qb = np.sort(rng.uniform(1.0, 100.0, (1, grid, k)), axis=2)
+        xs = rng.uniform(0.1, 1.0, (1, grid, grid, k))
+        qs = rng.uniform(0.1, 1.0, (1, grid, grid, k))
+        w = rng.uniform(0.5, 1.5, (1, grid, grid))
+        w /= w.sum()
+        acc = rng.uniform(0, 1, (1, grid, grid)) < 0.95
+        args = [torch.from_numpy(a).to(dev) for a in (xb, xs, qb, qs, w)]
+        acc_t = torch.from_numpy(acc).to(dev)
+
+        def step():
+            st = ext.forward(args[0], args[1], args[2], args[3], args[4], acc_t, n_events, SEED,
+                             SEQUENCE, False)
+            g = torch.ones_like(st[0])
+            ext.backward(g, args[0], args[1], args[2], args[3], acc_t, *st[1:], False)
+            return st[0].shape[0]
+
+        kept = step()
+        ms = time_ms(step, max(3, steps // 2))
+        out[f"grid{grid}_e{n_events}"] = dict(cells=grid * grid, nodes_per_axis=k,
+                                              events=kept, ms=ms,
+                                              events_per_s=kept / (ms * 1e-3))
     return out
 
 
