@@ -102,9 +102,9 @@ typedef struct loits_hip_forward_args {
   double* norm_q;         /* [B, nx, ny] */
   double* cdf_x;          /* like xsec_x, float64 */
   double* cdf_q;          /* like xsec_q, float64 */
-  float* u_x;             /* [B, cells, nmax], every padded slot */
+  float* u_x;             /* [B, cells, nmax], every padded slot (u_x, u_q 16-byte aligned) */
   float* u_q;
-  int16_t* interval_x;    /* [B*cells*nmax]; 0 in inactive slots */
+  int16_t* interval_x;    /* [B*cells*nmax]; 0 in inactive slots (8-byte aligned) */
   int16_t* interval_q;
   void* events;           /* [active, 2] */
   int64_t* packed;        /* [active] */

@@ -187,11 +187,20 @@ __global__ void __launch_bounds__(256) curve_kernel(CurveParams P) {
   P.mono[c] = mono ? 1 : 0;
 }
 
+// Curves of at most kSmallK nodes: the count of cpp/loits_core.cpp:12-18 as
+// it is written, with all thresholds loaded at once (one cache line or two).
+constexpr int kSmallK = 16;
+
 // count(k) = #{t : T_t <= k}; interval = clamp(count - 1, 0, K - 2)
 __device__ __forceinline__ int64_t interval_of(const uint32_t* thr, int64_t K, bool mono,
                                                uint32_t key) {
   int64_t count;
-  if (mono) {
+  if (K <= kSmallK) {
+    int c = 0;
+#pragma unroll
+    for (int t = 0; t < kSmallK; ++t) c += (t < K && thr[t] <= key) ? 1 : 0;
+    count = c;
+  } else if (mono) {
     int64_t lo = 0, hi = K;
     while (lo < hi) {
       const int64_t mid = (lo + hi) >> 1;
@@ -247,37 +256,68 @@ __global__ void __launch_bounds__(kTileThreads) sample_kernel(SampleParams P) {
   double vx[4], vq[4];
   int keepbits = 0;
   const int64_t nxc = s.batch * s.ny * s.nx;
+  const int64_t p0 = 4 * q;
+  // cell g and slot of p0 by one division; the next three slots step from there
+  int64_t g = p0 / P.nmax, slot = p0 - g * P.nmax;
+  int64_t cur_g = -1, count = 0, cxi = 0, cqi = 0, ox = 0, bx = 0, oq = 0, bq = 0;
+  bool mono_x = false, mono_q = false;
+  float ufx[4], ufq[4];
+  int16_t ix4[4] = {0, 0, 0, 0}, iq4[4] = {0, 0, 0, 0};
 #pragma unroll
   for (int e = 0; e < 4; ++e) {
-    const int64_t p = 4 * q + e;
-    if (p >= P.total_slots) continue;
-    const uint32_t kx = word_of(wx, e) >> 8, kq = word_of(wq, e) >> 8;
-    P.u_x[p] = float(kx) * 0x1.0p-24f;
-    P.u_q[p] = float(kq) * 0x1.0p-24f;
-    const int64_t g = p / P.nmax, slot = p - g * P.nmax;
-    if (slot >= P.counts[g]) {
-      P.interval_x[p] = 0;
-      P.interval_q[p] = 0;
-      continue;
+    if (e > 0 && ++slot == P.nmax) {
+      slot = 0;
+      ++g;
     }
-    const int64_t b = g / s.cells, cell = g - b * s.cells;
-    const int64_t ix = cell / s.ny, iy = cell - ix * s.ny;
-    const int64_t cxi = (b * s.ny + iy) * s.nx + ix, cqi = (b * s.nx + ix) * s.ny + iy;
+    const int64_t p = p0 + e;
+    const uint32_t kx = word_of(wx, e) >> 8, kq = word_of(wq, e) >> 8;
+    ufx[e] = float(kx) * 0x1.0p-24f;
+    ufq[e] = float(kq) * 0x1.0p-24f;
+    if (p >= P.total_slots) continue;
+    if (g != cur_g) {
+      cur_g = g;
+      count = P.counts[g];
+      const int64_t b = g / s.cells, cell = g - b * s.cells;
+      const int64_t ix = cell / s.ny, iy = cell - ix * s.ny;
+      cxi = (b * s.ny + iy) * s.nx + ix;
+      cqi = (b * s.nx + ix) * s.ny + iy;
+      ox = cxi * s.kx;
+      bx = (b * s.nx + ix) * s.kx;
+      oq = cqi * s.kq;
+      bq = (b * s.ny + iy) * s.kq;
+      mono_x = P.mono[cxi] != 0;
+      mono_q = P.mono[nxc + cqi] != 0;
+    }
+    if (slot >= count) continue;
     // x: cpp/loits_core.cpp:183-196
-    const int64_t ox = cxi * s.kx, bx = (b * s.nx + ix) * s.kx;
-    const int64_t jx = interval_of(P.thr_x + ox, s.kx, P.mono[cxi] != 0, kx);
+    const int64_t jx = interval_of(P.thr_x + ox, s.kx, mono_x, kx);
     const double ux = double(kx) * 0x1.0p-24;
     vx[e] = __builtin_fma(P.m_x[ox + jx], ux - P.cdf_x[ox + jx], ld<Real>(P.x_bins, bx + jx));
     // q: cpp/loits_core.cpp:219-232
-    const int64_t oq = cqi * s.kq, bq = (b * s.ny + iy) * s.kq;
-    const int64_t jq = interval_of(P.thr_q + oq, s.kq, P.mono[nxc + cqi] != 0, kq);
+    const int64_t jq = interval_of(P.thr_q + oq, s.kq, mono_q, kq);
     const double uq = double(kq) * 0x1.0p-24;
     vq[e] = __builtin_fma(P.m_q[oq + jq], uq - P.cdf_q[oq + jq], ld<Real>(P.q_bins, bq + jq));
-    P.interval_x[p] = int16_t(jx);
-    P.interval_q[p] = int16_t(jq);
+    ix4[e] = int16_t(jx);
+    iq4[e] = int16_t(jq);
     // keep rule: cpp/loits_core.cpp:256
     if (__builtin_isfinite(vx[e]) && __builtin_isfinite(vq[e]) && vx[e] * vq[e] != 0.0)
       keepbits |= 1 << e;
+  }
+  // every padded slot's uniforms and intervals (0 for inactive slots)
+  if (p0 + 3 < P.total_slots) {
+    *reinterpret_cast<float4*>(P.u_x + p0) = make_float4(ufx[0], ufx[1], ufx[2], ufx[3]);
+    *reinterpret_cast<float4*>(P.u_q + p0) = make_float4(ufq[0], ufq[1], ufq[2], ufq[3]);
+    *reinterpret_cast<short4*>(P.interval_x + p0) = make_short4(ix4[0], ix4[1], ix4[2], ix4[3]);
+    *reinterpret_cast<short4*>(P.interval_q + p0) = make_short4(iq4[0], iq4[1], iq4[2], iq4[3]);
+  } else {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      if (p0 + e >= P.total_slots) break;
+      P.u_x[p0 + e] = ufx[e];
+      P.u_q[p0 + e] = ufq[e];
+      P.interval_x[p0 + e] = ix4[e];
+      P.interval_q[p0 + e] = iq4[e];
+    }
   }
   // in-tile ordered offsets
   const int cnt = __popc(keepbits);
@@ -292,39 +332,65 @@ __global__ void __launch_bounds__(kTileThreads) sample_kernel(SampleParams P) {
     agg += wave_tot[w];
     base += w < wave ? wave_tot[w] : 0;
   }
-  // decoupled look-back for the tile's first row
-  if (threadIdx.x == 0) {
+  // decoupled look-back for the tile's first row: wave 0 reads the status
+  // words of 64 predecessors at a time and sums back to the nearest
+  // inclusive prefix (a predecessor without a status yet is waited for).
+  // Each status word carries its own value, so relaxed device-scope atomics
+  // suffice (no fence: nothing else written by a tile is read in this kernel).
+  if (threadIdx.x < kWave) {
+    const int lane = lane_id();
     uint64_t* st = P.tile_status;
     if (tile == 0) {
-      __hip_atomic_store(&st[0], kInclusive | uint64_t(agg), __ATOMIC_RELEASE,
-                         __HIP_MEMORY_SCOPE_AGENT);
-      sh_excl = 0;
+      if (lane == 0) {
+        __hip_atomic_store(&st[0], kInclusive | uint64_t(agg), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+        sh_excl = 0;
+      }
     } else {
-      __hip_atomic_store(&st[tile], kAggregate | uint64_t(agg), __ATOMIC_RELEASE,
-                         __HIP_MEMORY_SCOPE_AGENT);
+      if (lane == 0)
+        __hip_atomic_store(&st[tile], kAggregate | uint64_t(agg), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
       int64_t excl = 0;
-      int64_t prev = tile - 1;
+      int64_t end = tile - 1;
       uint64_t spins = 0;
-      while (prev >= 0) {
-        const uint64_t v = __hip_atomic_load(&st[prev], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+      bool failed = false;
+      while (true) {
+        const int64_t idx = end - lane;
+        const uint64_t v = idx >= 0 ? __hip_atomic_load(&st[idx], __ATOMIC_RELAXED,
+                                                         __HIP_MEMORY_SCOPE_AGENT)
+                                    : kInclusive;
         const uint64_t flag = v & ~kValueMask;
-        if (flag == 0) {
+        const uint64_t inc = ballot(flag == kInclusive);
+        const uint64_t waiting = ballot(flag == 0);
+        const int first = inc ? __builtin_ctzll(inc) : kWave;  // nearest inclusive
+        const uint64_t need = first >= kWave - 1 ? ~0ull : ((2ull << first) - 1ull);
+        if (waiting & need) {
           if (++spins > kSpinLimit) {  // a bounded wait: report instead of hanging
-            atomicOr(reinterpret_cast<unsigned long long*>(&P.valid[1]), 1ull);
+            failed = true;
             break;
           }
           __builtin_amdgcn_s_sleep(1);
           continue;
         }
-        excl += int64_t(v & kValueMask);
-        if (flag == kInclusive) break;
-        --prev;
+        uint64_t part = lane <= first ? (v & kValueMask) : 0ull;
+#pragma unroll
+        for (int d = 1; d < kWave; d <<= 1) {
+          const uint32_t lo = __shfl_xor(uint32_t(part), d, kWave);
+          const uint32_t hi = __shfl_xor(uint32_t(part >> 32), d, kWave);
+          part += uint64_t(lo) | (uint64_t(hi) << 32);
+        }
+        excl += int64_t(part);
+        if (first < kWave) break;
+        end -= kWave;
       }
-      __hip_atomic_store(&st[tile], kInclusive | uint64_t(excl + agg), __ATOMIC_RELEASE,
-                         __HIP_MEMORY_SCOPE_AGENT);
-      sh_excl = excl;
+      if (lane == 0) {
+        if (failed) atomicOr(reinterpret_cast<unsigned long long*>(&P.valid[1]), 1ull);
+        __hip_atomic_store(&st[tile], kInclusive | uint64_t(excl + agg), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+        sh_excl = excl;
+      }
     }
-    if (tile == P.n_tiles - 1) P.valid[0] = sh_excl + agg;
+    if (lane == 0 && tile == P.n_tiles - 1) P.valid[0] = sh_excl + agg;
   }
   __syncthreads();
   int64_t row = sh_excl + base + pre;
@@ -419,6 +485,162 @@ __global__ void __launch_bounds__(256) ivjp_kernel(VjpParams P) {
              ld<Real>(P.q_bins, bq + jq + 1), jq, double(P.u_q[p]),
              ld<Real>(P.grad, 2 * row + 1));
   }
+}
+
+// ivjp_row with the accumulators at stride `st` (LDS, one column per lane).
+__device__ __forceinline__ void ivjp_row_strided(const double* curve, double* gcurve, int st,
+                                                 double bj, double bj1, int64_t j, double uv,
+                                                 double gev) {
+  const double c0 = curve[j];
+  const double d = curve[j + 1] - c0 + kEpsilon;
+  const double inv_d = 1.0 / d;
+  const double t = uv - c0;
+  const double upstream = gev * (bj1 - bj);
+  gcurve[j * st] = __builtin_fma(upstream, __builtin_fma(t * inv_d, inv_d, -inv_d), gcurve[j * st]);
+  gcurve[(j + 1) * st] = __builtin_fma(-(upstream * t * inv_d), inv_d, gcurve[(j + 1) * st]);
+}
+
+// Curves of at most kSmallK nodes per axis: one lane per cell, 64 cells per
+// work-group; the cell's CDF gradients accumulate in LDS (rows in row order,
+// the same operations as ivjp_kernel) and the loads of four rows are issued
+// before their accumulation.
+template <typename Real>
+__global__ void __launch_bounds__(64) ivjp_small_kernel(VjpParams P) {
+  __shared__ double gx[kSmallK * 64], gq[kSmallK * 64];
+  const loits_hip_shape& s = P.s;
+  const int lane = threadIdx.x;
+  const int64_t g = int64_t(blockIdx.x) * 64 + lane;
+  if (g >= s.batch * s.cells) return;  // (no barrier below)
+  const int64_t b = g / s.cells, cell = g - b * s.cells;
+  const int64_t ix = cell / s.ny, iy = cell - ix * s.ny;
+  const int64_t ox = ((b * s.ny + iy) * s.nx + ix) * s.kx, bx = (b * s.nx + ix) * s.kx;
+  const int64_t oq = ((b * s.nx + ix) * s.ny + iy) * s.kq, bq = (b * s.ny + iy) * s.kq;
+  double* ax = gx + lane;
+  double* aq = gq + lane;
+  for (int64_t t = 0; t < s.kx; ++t) ax[t * 64] = 0.0;
+  for (int64_t t = 0; t < s.kq; ++t) aq[t * 64] = 0.0;
+  // rows of this cell: packed slots in [g nmax, (g + 1) nmax), two searches together
+  const int64_t key0 = g * P.nmax, key1 = key0 + P.nmax;
+  int64_t lo0 = 0, hi0 = P.nvalid, lo1 = 0, hi1 = P.nvalid;
+  while (lo0 < hi0 || lo1 < hi1) {
+    if (lo0 < hi0) {
+      const int64_t mid = (lo0 + hi0) >> 1;
+      if (P.packed[mid] < key0) lo0 = mid + 1; else hi0 = mid;
+    }
+    if (lo1 < hi1) {
+      const int64_t mid = (lo1 + hi1) >> 1;
+      if (P.packed[mid] < key1) lo1 = mid + 1; else hi1 = mid;
+    }
+  }
+  const double* cx = P.cdf_x + ox;
+  const double* cq = P.cdf_q + oq;
+  for (int64_t row = lo0; row < lo1; row += 4) {
+    const int nr = lo1 - row < 4 ? int(lo1 - row) : 4;
+    int64_t jx[4], jq[4];
+    double ux[4], uq[4], gex[4], geq[4], x0[4], x1[4], q0[4], q1[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int64_t rr = r < nr ? row + r : row;
+      const int64_t p = P.packed[rr];
+      jx[r] = P.interval_x[p];
+      jq[r] = P.interval_q[p];
+      ux[r] = double(P.u_x[p]);
+      uq[r] = double(P.u_q[p]);
+      gex[r] = ld<Real>(P.grad, 2 * rr);
+      geq[r] = ld<Real>(P.grad, 2 * rr + 1);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      x0[r] = ld<Real>(P.x_bins, bx + jx[r]);
+      x1[r] = ld<Real>(P.x_bins, bx + jx[r] + 1);
+      q0[r] = ld<Real>(P.q_bins, bq + jq[r]);
+      q1[r] = ld<Real>(P.q_bins, bq + jq[r] + 1);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      if (r >= nr) break;
+      ivjp_row_strided(cx, ax, 64, x0[r], x1[r], jx[r], ux[r], gex[r]);
+      ivjp_row_strided(cq, aq, 64, q0[r], q1[r], jq[r], uq[r], geq[r]);
+    }
+  }
+  for (int64_t t = 0; t < s.kx; ++t) P.gcdf_x[ox + t] = ax[t * 64];
+  for (int64_t t = 0; t < s.kq; ++t) P.gcdf_q[oq + t] = aq[t * 64];
+}
+
+struct RowF {
+  double upx, ax, bx, idx, upq, aq, bq, idq;
+  int32_t jx, jq;
+};
+
+template <typename Real>
+__global__ void __launch_bounds__(256) ivjp_wave_kernel(VjpParams P) {
+  __shared__ RowF rows[4 * kWave];
+  const loits_hip_shape& s = P.s;
+  const int lane = lane_id();
+  const int64_t g = int64_t(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (g >= s.batch * s.cells) return;  // whole waves (no barrier below)
+  const int64_t b = g / s.cells, cell = g - b * s.cells;
+  const int64_t ix = cell / s.ny, iy = cell - ix * s.ny;
+  const int64_t ox = ((b * s.ny + iy) * s.nx + ix) * s.kx, bx = (b * s.nx + ix) * s.kx;
+  const int64_t oq = ((b * s.nx + ix) * s.ny + iy) * s.kq, bq = (b * s.ny + iy) * s.kq;
+  const int64_t key0 = g * P.nmax, key1 = key0 + P.nmax;
+  int64_t lo0 = 0, hi0 = P.nvalid, lo1 = 0, hi1 = P.nvalid;
+  while (lo0 < hi0 || lo1 < hi1) {  // rows of this cell: [lo0, lo1)
+    if (lo0 < hi0) {
+      const int64_t mid = (lo0 + hi0) >> 1;
+      if (P.packed[mid] < key0) lo0 = mid + 1; else hi0 = mid;
+    }
+    if (lo1 < hi1) {
+      const int64_t mid = (lo1 + hi1) >> 1;
+      if (P.packed[mid] < key1) lo1 = mid + 1; else hi1 = mid;
+    }
+  }
+  const double* cx = P.cdf_x + ox;
+  const double* cq = P.cdf_q + oq;
+  double accx = 0.0, accq = 0.0;
+  for (int64_t base = lo0; base < lo1; base += kWave) {
+    const int nr = lo1 - base < kWave ? int(lo1 - base) : kWave;
+    const int64_t row = base + (lane < nr ? lane : 0);
+    const int64_t p = P.packed[row];
+    const int jx = P.interval_x[p], jq = P.interval_q[p];
+    double upx, ax, bxm, idx, upq, aq, bqm, idq;
+    {  // x: the factors of ivjp_row
+      const double c0 = cx[jx];
+      const double d = cx[jx + 1] - c0 + kEpsilon;
+      const double inv_d = 1.0 / d;
+      const double t = double(P.u_x[p]) - c0;
+      upx = ld<Real>(P.grad, 2 * row) * (ld<Real>(P.x_bins, bx + jx + 1) - ld<Real>(P.x_bins, bx + jx));
+      ax = __builtin_fma(t * inv_d, inv_d, -inv_d);
+      bxm = -(upx * t * inv_d);
+      idx = inv_d;
+    }
+    {  // q
+      const double c0 = cq[jq];
+      const double d = cq[jq + 1] - c0 + kEpsilon;
+      const double inv_d = 1.0 / d;
+      const double t = double(P.u_q[p]) - c0;
+      upq = ld<Real>(P.grad, 2 * row + 1) *
+            (ld<Real>(P.q_bins, bq + jq + 1) - ld<Real>(P.q_bins, bq + jq));
+      aq = __builtin_fma(t * inv_d, inv_d, -inv_d);
+      bqm = -(upq * t * inv_d);
+      idq = inv_d;
+    }
+    // this wave's rows in LDS; every lane then walks them in row order
+    RowF* rf = rows + (threadIdx.x >> 6) * kWave;
+    rf[lane] = RowF{upx, ax, bxm, idx, upq, aq, bqm, idq, jx, jq};
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll 4
+    for (int r = 0; r < nr; ++r) {
+      const RowF f = rf[r];
+      if (lane == f.jx) accx = __builtin_fma(f.upx, f.ax, accx);
+      else if (lane == f.jx + 1) accx = __builtin_fma(f.bx, f.idx, accx);
+      if (lane == f.jq) accq = __builtin_fma(f.upq, f.aq, accq);
+      else if (lane == f.jq + 1) accq = __builtin_fma(f.bq, f.idq, accq);
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+  if (lane < s.kx) P.gcdf_x[ox + lane] = accx;
+  if (lane < s.kq) P.gcdf_q[oq + lane] = accq;
 }
 
 template <typename Real>
@@ -591,6 +813,11 @@ int loits_hip_forward(const loits_hip_forward_args* a, void* stream) {
   const int64_t slots = s.batch * s.cells * a->nmax;
   if (slots > 0 && (!a->u_x || !a->u_q || !a->interval_x || !a->interval_q))
     return fail(LOITS_HIP_EINVAL, "forward: u/interval outputs are NULL");
+  auto misaligned = [](const void* p, uintptr_t a) { return (reinterpret_cast<uintptr_t>(p) & (a - 1)) != 0; };
+  if (slots > 0 && (misaligned(a->u_x, 16) || misaligned(a->u_q, 16) ||
+                    misaligned(a->interval_x, 8) || misaligned(a->interval_q, 8)))
+    return fail(LOITS_HIP_EINVAL,
+                "forward: u_x/u_q must be 16-byte and interval_x/interval_q 8-byte aligned");
   if (a->active > 0 && (!a->events || !a->packed))
     return fail(LOITS_HIP_EINVAL, "forward: events/packed outputs are NULL");
   hipStream_t st = static_cast<hipStream_t>(stream);
@@ -676,10 +903,18 @@ int loits_hip_backward(const loits_hip_backward_args* a, void* stream) {
               a->grad_xsec_q};
   const bool f32 = a->dtype == LOITS_HIP_F32;
   const int64_t gcells = s.batch * s.cells;
-  if (f32)
+  if (s.kx <= kWave && s.kq <= kWave) {
+    if (f32)
+      hipLaunchKernelGGL(ivjp_wave_kernel<float>, dim3(blocks_for(gcells, 4)), dim3(256), 0, st,
+                         V);
+    else
+      hipLaunchKernelGGL(ivjp_wave_kernel<double>, dim3(blocks_for(gcells, 4)), dim3(256), 0, st,
+                         V);
+  } else if (f32) {
     hipLaunchKernelGGL(ivjp_kernel<float>, dim3(blocks_for(gcells, 256)), dim3(256), 0, st, V);
-  else
+  } else {
     hipLaunchKernelGGL(ivjp_kernel<double>, dim3(blocks_for(gcells, 256)), dim3(256), 0, st, V);
+  }
   rc = check_hip(hipGetLastError(), "backward interpolation launch");
   if (rc != LOITS_HIP_OK) return rc;
   const int64_t curves = 2 * gcells;
