@@ -256,6 +256,24 @@ int loits_hip_batched_pack(const loits_hip_batched_args* args, void* packed, int
 int loits_hip_batched_backward(const loits_hip_batched_args* args, const void* grad_events,
                                void* grad_density, int32_t recompute_tables, void* stream);
 
+/* ---------------------------------------------------------------------- */
+/* Soft-binning histograms of packed events, the discriminator input of the
+ * GAN-style training harness.  No reference counterpart (the reference's
+ * discriminator scores events directly, pytorch/gan.py:11-23).  For instance i
+ * with rows [row_start[i], row_start[i+1]) of `values` (float32):
+ *   hist[i][c] = sum_r exp(-(t_r - c - 1/2)^2 / (2 width^2)),  t = (v - lo) bins / (hi - lo),
+ * over the seven bins nearest t_r; `hist` [n_instances, bins] is overwritten.
+ * The backward writes d(sum_ic ghist[i][c] hist[i][c]) / d v_r to grad_values.
+ * row_start is a device array of n_instances + 1 ascending offsets; bins <= 128.
+ * Float atomics: sums may differ from run to run at float rounding. */
+int loits_hip_soft_histogram(const float* values, const int64_t* row_start, int64_t n_instances,
+                             int64_t n_rows, float lo, float hi, int bins, float width,
+                             float* hist, void* stream);
+int loits_hip_soft_histogram_backward(const float* values, const int64_t* row_start,
+                                      int64_t n_instances, int64_t n_rows, float lo, float hi,
+                                      int bins, float width, const float* grad_hist,
+                                      float* grad_values, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -250,6 +250,42 @@ at::Tensor philox_words(int64_t first_block, int64_t blocks, uint64_t stream_id,
   return out;
 }
 
+// Soft-binning histograms (training harness).
+at::Tensor soft_histogram(at::Tensor values, at::Tensor row_start, double lo, double hi,
+                          int64_t bins, double width, py::object stream_obj) {
+  require_device(values, "values");
+  require_device(row_start, "row_start");
+  if (values.scalar_type() != at::kFloat || row_start.scalar_type() != at::kLong)
+    throw std::invalid_argument("values must be float32 and row_start int64");
+  const int64_t n = row_start.numel() - 1;
+  at::Tensor hist = at::empty({n > 0 ? n : 0, bins}, values.options());
+  check(loits_hip_soft_histogram(values.data_ptr<float>(), row_start.data_ptr<int64_t>(), n,
+                                 values.numel(), float(lo), float(hi), int(bins), float(width),
+                                 hist.data_ptr<float>(), stream_of(stream_obj, values)),
+        "loits_hip_soft_histogram");
+  return hist;
+}
+
+at::Tensor soft_histogram_backward(at::Tensor values, at::Tensor row_start, double lo, double hi,
+                                   int64_t bins, double width, at::Tensor grad_hist,
+                                   py::object stream_obj) {
+  require_device(values, "values");
+  require_device(row_start, "row_start");
+  require_device(grad_hist, "grad_hist");
+  if (values.scalar_type() != at::kFloat || row_start.scalar_type() != at::kLong ||
+      grad_hist.scalar_type() != at::kFloat)
+    throw std::invalid_argument("values and grad_hist must be float32, row_start int64");
+  const int64_t n = row_start.numel() - 1;
+  if (grad_hist.numel() != n * bins) throw std::invalid_argument("grad_hist must be [n, bins]");
+  at::Tensor grad = at::empty_like(values);
+  check(loits_hip_soft_histogram_backward(values.data_ptr<float>(), row_start.data_ptr<int64_t>(),
+                                          n, values.numel(), float(lo), float(hi), int(bins),
+                                          float(width), grad_hist.data_ptr<float>(),
+                                          grad.data_ptr<float>(), stream_of(stream_obj, values)),
+        "loits_hip_soft_histogram_backward");
+  return grad;
+}
+
 // ---------------------------------------------------------------------------
 // Batched 1-D path.
 class BatchedPlan {
@@ -430,6 +466,12 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
         py::arg("stream") = py::none());
   m.def("device_info", &device_info, py::arg("device") = 0);
   m.def("version", []() { return std::string(loits_hip_version()); });
+  m.def("soft_histogram", &soft_histogram, py::arg("values"), py::arg("row_start"),
+        py::arg("lo"), py::arg("hi"), py::arg("bins"), py::arg("width"),
+        py::arg("stream") = py::none());
+  m.def("soft_histogram_backward", &soft_histogram_backward, py::arg("values"),
+        py::arg("row_start"), py::arg("lo"), py::arg("hi"), py::arg("bins"), py::arg("width"),
+        py::arg("grad_hist"), py::arg("stream") = py::none());
   py::class_<BatchedPlan>(m, "BatchedPlan")
       .def(py::init<at::Tensor, int64_t, int64_t, int64_t, int64_t, int64_t, int64_t>())
       .def("layout", &BatchedPlan::layout)

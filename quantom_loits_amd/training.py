@@ -84,6 +84,40 @@ def soft_histogram(values, kept, lo=WINDOW[0], hi=WINDOW[1], bins=BINS, width=0.
     return hist.view(B, bins) / kept.clamp(min=1).unsqueeze(1).float()
 
 
+class _SoftHistogramHip(torch.autograd.Function):
+    """soft_histogram on the GPU: the HIP kernels of loits_hip_soft_histogram."""
+
+    @staticmethod
+    def forward(ctx, values, kept, lo, hi, bins, width):
+        from ._native import load_extension
+
+        ext = load_extension()
+        v = values.detach().float().contiguous()
+        row_start = torch.zeros(kept.numel() + 1, dtype=torch.int64, device=values.device)
+        torch.cumsum(kept, 0, out=row_start[1:])
+        hist = ext.soft_histogram(v, row_start, lo, hi, bins, width)
+        ctx.save_for_backward(v, row_start)
+        ctx.params = (lo, hi, bins, width)
+        ctx.dtype = values.dtype
+        return hist
+
+    @staticmethod
+    def backward(ctx, grad_hist):
+        from ._native import load_extension
+
+        v, row_start = ctx.saved_tensors
+        lo, hi, bins, width = ctx.params
+        g = load_extension().soft_histogram_backward(v, row_start, lo, hi, bins, width,
+                                                     grad_hist.float().contiguous())
+        return g.to(ctx.dtype), None, None, None, None, None
+
+
+def soft_histogram_hip(values, kept, lo=WINDOW[0], hi=WINDOW[1], bins=BINS, width=0.5):
+    """soft_histogram with the forward and backward in HIP kernels (GPU tensors)."""
+    hist = _SoftHistogramHip.apply(values, kept, lo, hi, bins, width)
+    return hist / kept.clamp(min=1).unsqueeze(1).float()
+
+
 class TorchSampler:
     """Torch-ops comparator for the 1-D sampler (comparison leg only).
 
@@ -194,9 +228,10 @@ class BatchedGanTrainer:
             self.sampler = TorchSampler(self.x.view(batch, nodes), events)
         _, target = uniform_family(batch, nodes, seed=self.seed, dtype=np.float32)
         target = torch.from_numpy(target.reshape(-1)).to(self.device)
+        self.histogram = soft_histogram_hip if self.device.type == "cuda" else soft_histogram
         with torch.no_grad():
             values, kept = self._sample(target)
-            self.real_hist = soft_histogram(values, kept)
+            self.real_hist = self.histogram(values, kept)
 
     # -- the sampler call, one new sequence per call ---------------------------
     def _sample(self, density):
@@ -226,7 +261,7 @@ class BatchedGanTrainer:
             self.d_optimizer.zero_grad(set_to_none=True)
             with torch.no_grad():
                 values, kept = self._sample(self.generator(self.z))
-                fake_hist = soft_histogram(values, kept)
+                fake_hist = self.histogram(values, kept)
             real = self.discriminator(self.real_hist)
             fake = self.discriminator(fake_hist)
             d_loss = (self.loss(real, torch.ones_like(real))
@@ -236,7 +271,7 @@ class BatchedGanTrainer:
         with self._region("gan::generator_step"):
             self.g_optimizer.zero_grad(set_to_none=True)
             values, kept = self._sample(self.generator(self.z))
-            logits = self.discriminator(soft_histogram(values, kept))
+            logits = self.discriminator(self.histogram(values, kept))
             g_loss = self.loss(logits, torch.ones_like(logits))
             g_loss.backward()
             self.g_optimizer.step()

@@ -78,3 +78,24 @@ def test_torch_comparator_step_runs(hip_device):
     t = BatchedGanTrainer("torch", "cuda", batch=4, nodes=128, events=2048)
     step_ms, loits_ms = loits_share(t, warmup=1, iterations=2)
     assert 0 < loits_ms < step_ms
+
+
+def test_hip_soft_histogram_matches_torch_layer(hip_device):
+    """The HIP soft-binning kernels against the torch formulation: values,
+    gradients, an empty instance, instances spanning work-groups, and values
+    outside the window."""
+    from quantom_loits_amd.training import soft_histogram, soft_histogram_hip
+
+    print("soft histogram", flush=True)
+    g = torch.Generator(device="cuda").manual_seed(5)
+    kept = torch.tensor([30000, 0, 5, 20000, 1, 17000], device="cuda")
+    v = torch.rand(int(kept.sum()), device="cuda", generator=g) * 1.1 - 0.05
+    v1 = v.clone().requires_grad_(True)
+    v2 = v.clone().requires_grad_(True)
+    h1 = soft_histogram_hip(v1, kept)
+    h2 = soft_histogram(v2, kept)
+    assert torch.allclose(h1, h2, rtol=1e-4, atol=1e-6)
+    w = torch.randn(h1.shape, device="cuda", generator=g)
+    (h1 * w).sum().backward()
+    (h2 * w).sum().backward()
+    assert torch.allclose(v1.grad, v2.grad, rtol=1e-3, atol=1e-3 * float(v2.grad.abs().max()))
