@@ -194,7 +194,7 @@ typedef struct loits_hip_layout {
   int64_t lds_forward;      /* dynamic LDS of the sampling kernel */
   int64_t lds_backward;     /* dynamic LDS of the VJP kernel */
   int64_t lds_tables;       /* dynamic LDS of the standalone table build (VJP recompute) */
-  int64_t n_segments;       /* chunks * waves per work-group */
+  int64_t n_segments;       /* chunks * segments per chunk (4 per wave of a work-group) */
   int64_t mask_words;       /* 64-bit keep-mask words in scratch */
   int64_t partial_doubles;  /* partial VJP sums in scratch (multi-chunk instances) */
   int32_t dtype, threads, scan_mode, persistent;

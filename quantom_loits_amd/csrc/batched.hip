@@ -695,8 +695,15 @@ __device__ __forceinline__ int64_t chunk_blocks(const InstanceDesc& in, const Ch
                  ((in.counter_base + uint64_t(ch.slot_lo)) >> 2));
 }
 
+// Next segment for this wave from the work-group's LDS counter (uniform).
+__device__ __forceinline__ int next_segment(int* counter) {
+  int s = 0;
+  if (lane_id() == 0) s = atomicAdd(counter, 1);
+  return __builtin_amdgcn_readfirstlane(s);
+}
+
 // ---------------------------------------------------------------------------
-// Sampling: one work-group per chunk, one segment per wave.  While every slot
+// Sampling: one work-group per chunk; waves take segments from a counter.  While every slot
 // of a segment so far was kept, rows are positional and no keep mask is
 // written; the first iteration with a dropped slot is recorded, and masks are
 // written from that iteration on.
@@ -704,17 +711,17 @@ __device__ __forceinline__ int64_t chunk_blocks(const InstanceDesc& in, const Ch
 // a guide in LDS and the per-bin arrays in global memory (forceinline keeps
 // each call's address spaces known to the compiler).
 template <typename Real, bool WINDOW>
-__device__ __forceinline__ void sample_segments(const BatchedParams& P, int c, const ChunkDesc& ch,
-                                                const InstanceDesc& in, const uint32_t* s_guide,
-                                                const uint32_t* s_srt,
-                                                const typename CoefOf<Real>::type* s_coef,
-                                                const uint32_t* s_thr, const double* cdf,
-                                                bool safe_curve) {
+__device__ __forceinline__ void sample_segment(const BatchedParams& P, int c, const ChunkDesc& ch,
+                                               const InstanceDesc& in, int w,
+                                               const uint32_t* s_guide, const uint32_t* s_srt,
+                                               const typename CoefOf<Real>::type* s_coef,
+                                               const uint32_t* s_thr, const double* cdf,
+                                               bool safe_curve) {
   const int n = static_cast<int>(in.n_nodes);
   const int shift = 24 - in.guide_log2;
   const Real* x_g = static_cast<const Real*>(P.x) + in.node_off;
 
-  const int nw = blockDim.x >> 6, w = threadIdx.x >> 6, lane = lane_id();
+  const int nw = P.segs, lane = lane_id();  // segment w of nw in this chunk
   const Segment sg = segment_of(in.counter_base, ch.slot_lo, ch.slot_hi, w, nw);
   const int64_t seg = int64_t(c) * nw + w;
   const int64_t base_row = in.out_off + int64_t(sg.p_lo - in.counter_base);
@@ -817,10 +824,12 @@ template <typename Real, bool WINDOW, bool SEQUENTIAL>
 __global__ void __launch_bounds__(1024) sample_kernel(BatchedParams P) {
   extern __shared__ __align__(16) uint8_t lds[];
   __shared__ BuildShared sh;
+  __shared__ int sh_next;  // next unclaimed segment (read after the barriers below)
   const int c = blockIdx.x;
   const ChunkDesc ch = P.chunks[c];
   const InstanceDesc in = P.inst[ch.instance];
   const int64_t nb = pad4(in.n_nodes);
+  if (threadIdx.x == 0) sh_next = 0;
   if (in.global_tables) {  // built beforehand by tables_kernel; stage the guide
     const Tables<Real> T = tables_of<Real>(P.tables, in);
     lds_copy(lds, T.guide, (int64_t(2) << in.guide_log2) * 4);
@@ -829,16 +838,18 @@ __global__ void __launch_bounds__(1024) sample_kernel(BatchedParams P) {
     const uint32_t* thr = (sh.flags & kMonotone) != 0 || sizeof(Real) != 4
                               ? T.srt
                               : coef_f32_view(T.coef, nb).thr;
-    sample_segments<Real, WINDOW>(P, c, ch, in, reinterpret_cast<const uint32_t*>(lds), T.srt,
-                                  T.coef, thr, T.cdf, (sh.flags & kSafe) != 0);
+    for (int sgi = next_segment(&sh_next); sgi < P.segs; sgi = next_segment(&sh_next))
+      sample_segment<Real, WINDOW>(P, c, ch, in, sgi, reinterpret_cast<const uint32_t*>(lds),
+                                   T.srt, T.coef, thr, T.cdf, (sh.flags & kSafe) != 0);
   } else {
     build_tables<Real, SEQUENTIAL>(P, ch.instance, in, lds, ch.index == 0, sh);
     const LdsTables<Real> L = lds_tables_of<Real>(lds, in);
     const uint32_t* thr = (sh.flags & kMonotone) != 0 || sizeof(Real) != 4
                               ? L.srt
                               : coef_f32_view(L.coef, nb).thr;
-    sample_segments<Real, WINDOW>(P, c, ch, in, L.guide, L.srt, L.coef, thr, L.cdf,
-                                  (sh.flags & kSafe) != 0);
+    for (int sgi = next_segment(&sh_next); sgi < P.segs; sgi = next_segment(&sh_next))
+      sample_segment<Real, WINDOW>(P, c, ch, in, sgi, L.guide, L.srt, L.coef, thr, L.cdf,
+                                   (sh.flags & kSafe) != 0);
   }
 }
 
@@ -877,8 +888,8 @@ __global__ void __launch_bounds__(1024) finalize_kernel(BatchedParams P) {
   __syncthreads();  // seg_row written above is visible to the whole work-group
   for (int64_t i = tid; i < P.n_instances; i += nt) {
     const InstanceDesc in = P.inst[i];
-    const int64_t s0 = int64_t(in.first_chunk) * P.waves;
-    const int64_t s1 = s0 + int64_t(in.n_chunks) * P.waves;
+    const int64_t s0 = int64_t(in.first_chunk) * P.segs;
+    const int64_t s1 = s0 + int64_t(in.n_chunks) * P.segs;
     P.kept[i] = (s1 < nseg ? P.seg_row[s1] : carry) - P.seg_row[s0];
   }
   if (tid == 0) {
@@ -887,22 +898,24 @@ __global__ void __launch_bounds__(1024) finalize_kernel(BatchedParams P) {
   }
 }
 
-// Pack: one work-group per chunk, one segment per wave; a no-op when dense.
+// Pack: one work-group per chunk, segments round-robin over its waves; a no-op when dense.
 template <typename Real>
 __global__ void __launch_bounds__(1024) pack_kernel(BatchedParams P, Real* packed,
                                                     int32_t* packed_bins) {
   if (P.flags[0]) return;
   const ChunkDesc ch = P.chunks[blockIdx.x];
   const InstanceDesc in = P.inst[ch.instance];
-  const int nw = blockDim.x >> 6, w = threadIdx.x >> 6;
-  const Segment sg = segment_of(in.counter_base, ch.slot_lo, ch.slot_hi, w, nw);
-  const int64_t seg = int64_t(blockIdx.x) * nw + w;
-  const int64_t cnt = P.seg_kept[seg];
-  const int64_t src = in.out_off + int64_t(sg.p_lo - in.counter_base), dst = P.seg_row[seg];
-  const Real* ev = static_cast<const Real*>(P.events);
-  for (int64_t r = lane_id(); r < cnt; r += 64) {
-    packed[dst + r] = ev[src + r];
-    if (packed_bins && P.bins) packed_bins[dst + r] = P.bins[src + r];
+  const int nw = blockDim.x >> 6;
+  for (int w = threadIdx.x >> 6; w < P.segs; w += nw) {
+    const Segment sg = segment_of(in.counter_base, ch.slot_lo, ch.slot_hi, w, P.segs);
+    const int64_t seg = int64_t(blockIdx.x) * P.segs + w;
+    const int64_t cnt = P.seg_kept[seg];
+    const int64_t src = in.out_off + int64_t(sg.p_lo - in.counter_base), dst = P.seg_row[seg];
+    const Real* ev = static_cast<const Real*>(P.events);
+    for (int64_t r = lane_id(); r < cnt; r += 64) {
+      packed[dst + r] = ev[src + r];
+      if (packed_bins && P.bins) packed_bins[dst + r] = P.bins[src + r];
+    }
   }
 }
 
@@ -1023,6 +1036,7 @@ __device__ __forceinline__ void vjp_body(const BatchedParams& P, const Real* gra
   __shared__ int sh_last;
   __shared__ uint32_t sh_gmax;
   __shared__ int sh_outl;  // some cotangent took the fp64 overflow sums
+  __shared__ int sh_next;  // next unclaimed segment
   const ChunkDesc ch = P.chunks[c];
   const InstanceDesc in = P.inst[ch.instance];
   Tables<Real> T = tables_of<Real>(P.tables, in);
@@ -1072,42 +1086,46 @@ __device__ __forceinline__ void vjp_body(const BatchedParams& P, const Real* gra
   if (tid == 0) {
     sh_gmax = 0u;
     sh_outl = 0;
+    sh_next = 0;
   }
   __syncthreads();
   const Coef* s_coef = reinterpret_cast<const Coef*>(s_bin);
   const float2* s_vb = reinterpret_cast<const float2*>(s_bin);
 
-  const int nw = nt >> 6, w = tid >> 6, lane = lane_id();
-  const Segment sg = segment_of(in.counter_base, ch.slot_lo, ch.slot_hi, w, nw);
-  const int64_t seg = int64_t(c) * nw + w;
-  const int first_drop = P.seg_drop[seg];
-  const int64_t row0 = P.seg_row[seg];
+  const int nw = P.segs, lane = lane_id();
 
   // Fixed-point scale: the largest |g| over the first rows of each segment
   // holding a non-zero cotangent (bit patterns order |g|; NaN above inf).
   float scale = 0.0f;
   int scale_log2 = 0;
   if (fixed) {
-    const int64_t kept = P.seg_kept[seg];
-    uint32_t m = 0u;
-    for (int64_t r0 = 0; r0 < kept && m == 0u; r0 += 256) {
-      uint32_t lm = 0u;
+    uint32_t m_all = 0u;
+    for (int sw = tid >> 6; sw < nw; sw += nt >> 6) {
+      const int64_t sseg = int64_t(c) * nw + sw;
+      const int64_t kept = P.seg_kept[sseg];
+      const int64_t srow0 = P.seg_row[sseg];
+      uint32_t m = 0u;
+      for (int64_t r0 = 0; r0 < kept && m == 0u; r0 += 256) {
+        uint32_t lm = 0u;
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int64_t r = r0 + 4 * lane + q;
-        if (r < kept) {
-          const uint32_t bits = __builtin_bit_cast(uint32_t, float(grad[row0 + r])) & 0x7FFFFFFFu;
-          lm = bits > lm ? bits : lm;
+        for (int q = 0; q < 4; ++q) {
+          const int64_t r = r0 + 4 * lane + q;
+          if (r < kept) {
+            const uint32_t bits =
+                __builtin_bit_cast(uint32_t, float(grad[srow0 + r])) & 0x7FFFFFFFu;
+            lm = bits > lm ? bits : lm;
+          }
         }
-      }
 #pragma unroll
-      for (int d = 32; d >= 1; d >>= 1) {
-        const uint32_t o = uint32_t(__shfl_xor(int(lm), d, kWave));
-        lm = o > lm ? o : lm;
+        for (int d = 32; d >= 1; d >>= 1) {
+          const uint32_t o = uint32_t(__shfl_xor(int(lm), d, kWave));
+          lm = o > lm ? o : lm;
+        }
+        m = lm;
       }
-      m = lm;
+      m_all = m > m_all ? m : m_all;
     }
-    if (lane == 0 && m != 0u) atomicMax(&sh_gmax, m);
+    if (lane == 0 && m_all != 0u) atomicMax(&sh_gmax, m_all);
     __syncthreads();
     const uint32_t gm = sh_gmax;
     const int ex = int(gm >> 23);  // |g| < 2^(ex - 126)
@@ -1120,136 +1138,143 @@ __device__ __forceinline__ void vjp_body(const BatchedParams& P, const Real* gra
     }
   }
 
-  const int64_t per_wave = int64_t(blocks_per_wave(uint64_t(chunk_blocks(in, ch)), nw) / 64);
-  const uint64_t* mask = P.keepmask + ch.mask_off + int64_t(w) * per_wave * 4;
-  const uint32_t nblk = uint32_t(sg.qb - sg.qa);
-  const uint32_t off0 = uint32_t(sg.p_lo - 4 * sg.qa);
-  const uint32_t span = uint32_t(sg.p_hi - 4 * sg.qa);
-  // kept slots before the first masked iteration: every live slot before it
-  int64_t running = int64_t(4 * 64 * int64_t(first_drop)) - int64_t(off0);
-  running = running < 0 ? 0 : running;
+  // segments of the chunk, taken from the work-group's counter
+  for (int w = next_segment(&sh_next); w < nw; w = next_segment(&sh_next)) {
+    const Segment sg = segment_of(in.counter_base, ch.slot_lo, ch.slot_hi, w, nw);
+    const int64_t seg = int64_t(c) * nw + w;
+    const int first_drop = P.seg_drop[seg];
+    const int64_t row0 = P.seg_row[seg];
+    const int64_t per_wave = int64_t(blocks_per_wave(uint64_t(chunk_blocks(in, ch)), nw) / 64);
+    const uint64_t* mask = P.keepmask + ch.mask_off + int64_t(w) * per_wave * 4;
+    const uint32_t nblk = uint32_t(sg.qb - sg.qa);
+    const uint32_t off0 = uint32_t(sg.p_lo - 4 * sg.qa);
+    const uint32_t span = uint32_t(sg.p_hi - 4 * sg.qa);
+    // kept slots before the first masked iteration: every live slot before it
+    int64_t running = int64_t(4 * 64 * int64_t(first_drop)) - int64_t(off0);
+    running = running < 0 ? 0 : running;
 
-  // one kept slot: bin j, cotangent g, key k
-  auto add_fixed = [&](int jj, float g, float th) {
-    const float gs = g * scale;
-    if (__builtin_expect(__builtin_fabsf(gs) < 0x1.0p30f, 1)) {
-      const int ia = static_cast<int>(__builtin_rintf(gs));
-      const int ib = static_cast<int>(__builtin_rintf(gs * th));
-      atomicAdd(reinterpret_cast<unsigned long long*>(a0 + jj),
-                static_cast<unsigned long long>(static_cast<int64_t>(ia)));
-      atomicAdd(reinterpret_cast<unsigned long long*>(a1 + jj),
-                static_cast<unsigned long long>(static_cast<int64_t>(ib)));
-    } else {
-      sh_outl = 1;
-      atomicAdd(outl + jj, double(g));
-      atomicAdd(outl + nb + jj, double(g) * double(th));
-    }
-  };
+    // one kept slot: bin j, cotangent g, key k
+    auto add_fixed = [&](int jj, float g, float th) {
+      const float gs = g * scale;
+      if (__builtin_expect(__builtin_fabsf(gs) < 0x1.0p30f, 1)) {
+        const int ia = static_cast<int>(__builtin_rintf(gs));
+        const int ib = static_cast<int>(__builtin_rintf(gs * th));
+        atomicAdd(reinterpret_cast<unsigned long long*>(a0 + jj),
+                  static_cast<unsigned long long>(static_cast<int64_t>(ia)));
+        atomicAdd(reinterpret_cast<unsigned long long*>(a1 + jj),
+                  static_cast<unsigned long long>(static_cast<int64_t>(ib)));
+      } else {
+        sh_outl = 1;
+        atomicAdd(outl + jj, double(g));
+        atomicAdd(outl + nb + jj, double(g) * double(th));
+      }
+    };
 
-  // keep bits and first packed row of a lane's four slots in iteration `it`
-  auto rows_of = [&](int it, uint32_t i0, int& keepbits, int64_t& row) {
-    const uint32_t i = i0 + uint32_t(lane);
-    const bool full_iter = (i0 > 0 || off0 == 0) && 4 * (i0 + 64) <= span;
-    keepbits = 0xF;
-    if (it < first_drop) {  // every live slot of these iterations was kept
-      if (!full_iter) {
-        keepbits = 0;
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          const uint32_t r = 4 * i + e;
-          keepbits |= (i < nblk && r >= off0 && r < span) ? (1 << e) : 0;
-        }
-      }
-      const int32_t r0 = int32_t(4 * i) - int32_t(off0);
-      row = row0 + (r0 < 0 ? 0 : r0);
-    } else {
-      const uint64_t* mw = mask + int64_t(it) * 4;
-      keepbits = 0;
-#pragma unroll
-      for (int e = 0; e < 4; ++e) keepbits |= int((mw[e] >> lane) & 1ull) << e;
-      int total;
-      row = row0 + running + wave_scan_counts(__popc(keepbits), &total);
-      running += total;
-    }
-  };
-
-  // kTrip iterations per trip (independent Philox / search chains).
-  int it = 0;
-  for (uint32_t i0 = 0; i0 < nblk; i0 += 64 * kTrip, it += kTrip) {
-    uint32_t k[4 * kTrip];
-#pragma unroll
-    for (int h = 0; h < kTrip; ++h) {
-      const Words4 wd =
-          philox4x32_10_uniform(sg.qa + i0 + 64 * h + uint32_t(lane), in.stream, P.seed);
-      k[4 * h] = wd.w0 >> 8;
-      k[4 * h + 1] = wd.w1 >> 8;
-      k[4 * h + 2] = wd.w2 >> 8;
-      k[4 * h + 3] = wd.w3 >> 8;
-    }
-    int kb[kTrip];
-    int64_t rw[kTrip];
-#pragma unroll
-    for (int h = 0; h < kTrip; ++h) {
-      kb[h] = 0;
-      rw[h] = 0;
-      if (h == 0 || i0 + 64 * h < nblk) rows_of(it + h, i0 + 64 * h, kb[h], rw[h]);
-    }
-    int j[4 * kTrip];
-    find_bins<4 * kTrip>(k, s_guide, s_srt, shift, n, j);
-    if (fixed) {
-      float th[4 * kTrip];
-#pragma unroll
-      for (int e = 0; e < 4 * kTrip; ++e) {
-        const float2 ab = s_vb[j[e]];
-        th[e] = __builtin_fmaf(static_cast<float>(static_cast<int32_t>(k[e] - s_srt[j[e]])), ab.x,
-                               ab.y);
-      }
-      float g[4 * kTrip];
-#pragma unroll
-      for (int h = 0; h < kTrip; ++h) {
-        if (kb[h] == 0xF && (rw[h] & 3) == 0) {
-          const float4 g4 = *reinterpret_cast<const float4*>(grad + rw[h]);
-          g[4 * h] = g4.x; g[4 * h + 1] = g4.y; g[4 * h + 2] = g4.z; g[4 * h + 3] = g4.w;
-        } else {
-          int64_t r = rw[h];
-#pragma unroll
-          for (int e = 0; e < 4; ++e) g[4 * h + e] = ((kb[h] >> e) & 1) ? float(grad[r++]) : 0.0f;
-        }
-      }
-#pragma unroll
-      for (int h = 0; h < kTrip; ++h) {
-        if (kb[h] == 0xF) {
-#pragma unroll
-          for (int e = 0; e < 4; ++e) add_fixed(j[4 * h + e], g[4 * h + e], th[4 * h + e]);
-        } else {
-#pragma unroll
-          for (int e = 0; e < 4; ++e)
-            if ((kb[h] >> e) & 1) add_fixed(j[4 * h + e], g[4 * h + e], th[4 * h + e]);
-        }
-      }
-    } else {
-#pragma unroll
-      for (int h = 0; h < kTrip; ++h) {
-        double t[4];
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          const int je = j[4 * h + e];
-          if constexpr (sizeof(Real) == 4) {
-            const CoefF32View cv = coef_f32_view(const_cast<uint8_t*>(s_bin), nb);
-            t[e] = double(static_cast<float>(static_cast<int32_t>(k[4 * h + e] - cv.thr[je])) *
-                              0x1.0p-24f +
-                          cv.r[je]);
-          } else {
-            t[e] = double(k[4 * h + e]) * 0x1.0p-24 - s_coef[je].c;
+    // keep bits and first packed row of a lane's four slots in iteration `it`
+    auto rows_of = [&](int it, uint32_t i0, int& keepbits, int64_t& row) {
+      const uint32_t i = i0 + uint32_t(lane);
+      const bool full_iter = (i0 > 0 || off0 == 0) && 4 * (i0 + 64) <= span;
+      keepbits = 0xF;
+      if (it < first_drop) {  // every live slot of these iterations was kept
+        if (!full_iter) {
+          keepbits = 0;
+  #pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const uint32_t r = 4 * i + e;
+            keepbits |= (i < nblk && r >= off0 && r < span) ? (1 << e) : 0;
           }
         }
-        int64_t r = rw[h];
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          if (!((kb[h] >> e) & 1)) continue;
-          const double ge = double(grad[r++]);
-          add_f64<GLOBAL>(&h0[j[4 * h + e]], ge);
-          add_f64<GLOBAL>(&h1[j[4 * h + e]], ge * t[e]);
+        const int32_t r0 = int32_t(4 * i) - int32_t(off0);
+        row = row0 + (r0 < 0 ? 0 : r0);
+      } else {
+        const uint64_t* mw = mask + int64_t(it) * 4;
+        keepbits = 0;
+  #pragma unroll
+        for (int e = 0; e < 4; ++e) keepbits |= int((mw[e] >> lane) & 1ull) << e;
+        int total;
+        row = row0 + running + wave_scan_counts(__popc(keepbits), &total);
+        running += total;
+      }
+    };
+
+    // kTrip iterations per trip (independent Philox / search chains).
+    int it = 0;
+    for (uint32_t i0 = 0; i0 < nblk; i0 += 64 * kTrip, it += kTrip) {
+      uint32_t k[4 * kTrip];
+  #pragma unroll
+      for (int h = 0; h < kTrip; ++h) {
+        const Words4 wd =
+            philox4x32_10_uniform(sg.qa + i0 + 64 * h + uint32_t(lane), in.stream, P.seed);
+        k[4 * h] = wd.w0 >> 8;
+        k[4 * h + 1] = wd.w1 >> 8;
+        k[4 * h + 2] = wd.w2 >> 8;
+        k[4 * h + 3] = wd.w3 >> 8;
+      }
+      int kb[kTrip];
+      int64_t rw[kTrip];
+  #pragma unroll
+      for (int h = 0; h < kTrip; ++h) {
+        kb[h] = 0;
+        rw[h] = 0;
+        if (h == 0 || i0 + 64 * h < nblk) rows_of(it + h, i0 + 64 * h, kb[h], rw[h]);
+      }
+      int j[4 * kTrip];
+      find_bins<4 * kTrip>(k, s_guide, s_srt, shift, n, j);
+      if (fixed) {
+        float th[4 * kTrip];
+  #pragma unroll
+        for (int e = 0; e < 4 * kTrip; ++e) {
+          const float2 ab = s_vb[j[e]];
+          th[e] = __builtin_fmaf(static_cast<float>(static_cast<int32_t>(k[e] - s_srt[j[e]])), ab.x,
+                                 ab.y);
+        }
+        float g[4 * kTrip];
+  #pragma unroll
+        for (int h = 0; h < kTrip; ++h) {
+          if (kb[h] == 0xF && (rw[h] & 3) == 0) {
+            const float4 g4 = *reinterpret_cast<const float4*>(grad + rw[h]);
+            g[4 * h] = g4.x; g[4 * h + 1] = g4.y; g[4 * h + 2] = g4.z; g[4 * h + 3] = g4.w;
+          } else {
+            int64_t r = rw[h];
+  #pragma unroll
+            for (int e = 0; e < 4; ++e) g[4 * h + e] = ((kb[h] >> e) & 1) ? float(grad[r++]) : 0.0f;
+          }
+        }
+  #pragma unroll
+        for (int h = 0; h < kTrip; ++h) {
+          if (kb[h] == 0xF) {
+  #pragma unroll
+            for (int e = 0; e < 4; ++e) add_fixed(j[4 * h + e], g[4 * h + e], th[4 * h + e]);
+          } else {
+  #pragma unroll
+            for (int e = 0; e < 4; ++e)
+              if ((kb[h] >> e) & 1) add_fixed(j[4 * h + e], g[4 * h + e], th[4 * h + e]);
+          }
+        }
+      } else {
+  #pragma unroll
+        for (int h = 0; h < kTrip; ++h) {
+          double t[4];
+  #pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const int je = j[4 * h + e];
+            if constexpr (sizeof(Real) == 4) {
+              const CoefF32View cv = coef_f32_view(const_cast<uint8_t*>(s_bin), nb);
+              t[e] = double(static_cast<float>(static_cast<int32_t>(k[4 * h + e] - cv.thr[je])) *
+                                0x1.0p-24f +
+                            cv.r[je]);
+            } else {
+              t[e] = double(k[4 * h + e]) * 0x1.0p-24 - s_coef[je].c;
+            }
+          }
+          int64_t r = rw[h];
+  #pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            if (!((kb[h] >> e) & 1)) continue;
+            const double ge = double(grad[r++]);
+            add_f64<GLOBAL>(&h0[j[4 * h + e]], ge);
+            add_f64<GLOBAL>(&h1[j[4 * h + e]], ge * t[e]);
+          }
         }
       }
     }
@@ -1351,6 +1376,7 @@ __device__ void vjp_range_body(const BatchedParams& P, const Real* grad, Real* g
   __shared__ uint32_t sh_gmax;
   __shared__ uint32_t sh_wmax[16];
   __shared__ int sh_outl;  // some cotangent took the fp64 overflow sums
+  __shared__ int sh_next;  // next unclaimed segment
   constexpr bool fixed = sizeof(Real) == 4;
   const ChunkDesc ch = P.chunks[c];
   const InstanceDesc in = P.inst[ch.instance];
@@ -1406,6 +1432,7 @@ __device__ void vjp_range_body(const BatchedParams& P, const Real* grad, Real* g
   if (tid == 0) {
     sh_gmax = 0u;
     sh_outl = 0;
+    sh_next = 0;
   }
   __syncthreads();
   // Bin t (relative) starts the cells whose first key lies in [T_t, T_next);
@@ -1434,37 +1461,40 @@ __device__ void vjp_range_body(const BatchedParams& P, const Real* grad, Real* g
     if (tid == 0) cell_bin[ncell] = uint32_t(nbins);
   }
 
-  const int nw = nt >> 6, w = tid >> 6, lane = lane_id();
-  const Segment sg = segment_of(in.counter_base, ch.slot_lo, ch.slot_hi, w, nw);
-  const int64_t seg = int64_t(c) * nw + w;
-  const int first_drop = P.seg_drop[seg];
-  const int64_t row0 = P.seg_row[seg];
+  const int nw = P.segs, lane = lane_id();
 
   // Fixed-point scale from the first rows of each segment holding a non-zero
   // cotangent (as in the LDS-resident VJP).
   float scale = 0.0f;
   int scale_log2 = 0;
   if constexpr (fixed) {
-    const int64_t kept = P.seg_kept[seg];
-    uint32_t m = 0u;
-    for (int64_t r0 = 0; r0 < kept && m == 0u; r0 += 256) {
-      uint32_t lm = 0u;
+    uint32_t m_all = 0u;
+    for (int sw = tid >> 6; sw < nw; sw += nt >> 6) {
+      const int64_t sseg = int64_t(c) * nw + sw;
+      const int64_t kept = P.seg_kept[sseg];
+      const int64_t srow0 = P.seg_row[sseg];
+      uint32_t m = 0u;
+      for (int64_t r0 = 0; r0 < kept && m == 0u; r0 += 256) {
+        uint32_t lm = 0u;
 #pragma unroll
-      for (int qq = 0; qq < 4; ++qq) {
-        const int64_t r = r0 + 4 * lane + qq;
-        if (r < kept) {
-          const uint32_t bits = __builtin_bit_cast(uint32_t, float(grad[row0 + r])) & 0x7FFFFFFFu;
-          lm = bits > lm ? bits : lm;
+        for (int qq = 0; qq < 4; ++qq) {
+          const int64_t r = r0 + 4 * lane + qq;
+          if (r < kept) {
+            const uint32_t bits =
+                __builtin_bit_cast(uint32_t, float(grad[srow0 + r])) & 0x7FFFFFFFu;
+            lm = bits > lm ? bits : lm;
+          }
         }
-      }
 #pragma unroll
-      for (int d = 32; d >= 1; d >>= 1) {
-        const uint32_t o = uint32_t(__shfl_xor(int(lm), d, kWave));
-        lm = o > lm ? o : lm;
+        for (int d = 32; d >= 1; d >>= 1) {
+          const uint32_t o = uint32_t(__shfl_xor(int(lm), d, kWave));
+          lm = o > lm ? o : lm;
+        }
+        m = lm;
       }
-      m = lm;
+      m_all = m > m_all ? m : m_all;
     }
-    if (lane == 0 && m != 0u) atomicMax(&sh_gmax, m);
+    if (lane == 0 && m_all != 0u) atomicMax(&sh_gmax, m_all);
   }
   __syncthreads();
   if constexpr (fixed) {
@@ -1478,148 +1508,155 @@ __device__ void vjp_range_body(const BatchedParams& P, const Real* grad, Real* g
       scale = __builtin_ldexpf(1.0f, scale_log2);
     }
   }
-
-  const int64_t per_wave = int64_t(blocks_per_wave(uint64_t(chunk_blocks(in, ch)), nw) / 64);
-  const uint64_t* mask = P.keepmask + ch.mask_off + int64_t(w) * per_wave * 4;
-  const uint32_t nblk = uint32_t(sg.qb - sg.qa);
-  const uint32_t off0 = uint32_t(sg.p_lo - 4 * sg.qa);
-  const uint32_t span_s = uint32_t(sg.p_hi - 4 * sg.qa);
-  int64_t running = int64_t(4 * 64 * int64_t(first_drop)) - int64_t(off0);
-  running = running < 0 ? 0 : running;
   const int jmax = n - 2 - lo;  // relative clamp (only the last range reaches it)
   const float2* s_vb = reinterpret_cast<const float2*>(bd);
   const double* s_c = reinterpret_cast<const double*>(bd);
 
-  auto rows_of = [&](int it, uint32_t i0, int& keepbits, int64_t& row) {
-    const uint32_t i = i0 + uint32_t(lane);
-    const bool full_iter = (i0 > 0 || off0 == 0) && 4 * (i0 + 64) <= span_s;
-    keepbits = 0xF;
-    if (it < first_drop) {
-      if (!full_iter) {
-        keepbits = 0;
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          const uint32_t r = 4 * i + e;
-          keepbits |= (i < nblk && r >= off0 && r < span_s) ? (1 << e) : 0;
-        }
-      }
-      const int32_t r0 = int32_t(4 * i) - int32_t(off0);
-      row = row0 + (r0 < 0 ? 0 : r0);
-    } else {
-      const uint64_t* mw = mask + int64_t(it) * 4;
-      keepbits = 0;
-#pragma unroll
-      for (int e = 0; e < 4; ++e) keepbits |= int((mw[e] >> lane) & 1ull) << e;
-      int total;
-      row = row0 + running + wave_scan_counts(__popc(keepbits), &total);
-      running += total;
-    }
-  };
+  // segments of the chunk, taken from the work-group's counter
+  for (int w = next_segment(&sh_next); w < nw; w = next_segment(&sh_next)) {
+    const Segment sg = segment_of(in.counter_base, ch.slot_lo, ch.slot_hi, w, nw);
+    const int64_t seg = int64_t(c) * nw + w;
+    const int first_drop = P.seg_drop[seg];
+    const int64_t row0 = P.seg_row[seg];
+    const int64_t per_wave = int64_t(blocks_per_wave(uint64_t(chunk_blocks(in, ch)), nw) / 64);
+    const uint64_t* mask = P.keepmask + ch.mask_off + int64_t(w) * per_wave * 4;
+    const uint32_t nblk = uint32_t(sg.qb - sg.qa);
+    const uint32_t off0 = uint32_t(sg.p_lo - 4 * sg.qa);
+    const uint32_t span_s = uint32_t(sg.p_hi - 4 * sg.qa);
+    int64_t running = int64_t(4 * 64 * int64_t(first_drop)) - int64_t(off0);
+    running = running < 0 ? 0 : running;
 
-  // Per trip, 4 * kTrip keys per lane go through each step together (their
-  // LDS reads are independent): cell bounds, bisection, per-bin data, then the
-  // cotangents and the histogram updates of the kept keys inside the range.
-  constexpr int NK = 4 * kTrip;
-  int it = 0;
-  for (uint32_t i0 = 0; i0 < nblk; i0 += 64 * kTrip, it += kTrip) {
-    uint32_t k[NK];
-#pragma unroll
-    for (int h = 0; h < kTrip; ++h) {
-      const Words4 wd =
-          philox4x32_10_uniform(sg.qa + i0 + 64 * h + uint32_t(lane), in.stream, P.seed);
-      k[4 * h] = wd.w0 >> 8;
-      k[4 * h + 1] = wd.w1 >> 8;
-      k[4 * h + 2] = wd.w2 >> 8;
-      k[4 * h + 3] = wd.w3 >> 8;
-    }
-    int kb[kTrip];
-    int64_t rw[kTrip];
-#pragma unroll
-    for (int h = 0; h < kTrip; ++h) {
-      kb[h] = 0;
-      rw[h] = 0;
-      if (h == 0 || i0 + 64 * h < nblk) rows_of(it + h, i0 + 64 * h, kb[h], rw[h]);
-    }
-    uint32_t use = 0u;  // kept and inside [K_lo, K_hi)
-    int a[NK], b[NK];
-#pragma unroll
-    for (int e = 0; e < NK; ++e) {
-      const bool in_range = ((kb[e >> 2] >> (e & 3)) & 1) && k[e] >= k_lo && k[e] < k_hi;
-      use |= in_range ? (1u << e) : 0u;
-      const uint32_t cell = in_range ? (k[e] - k_lo) >> cs : 0u;
-      a[e] = int(cell_bin[cell]);
-      b[e] = int(cell_bin[cell + 1]);
-    }
-    uint32_t pending = 0u;
-#pragma unroll
-    for (int e = 0; e < NK; ++e) pending |= (((use >> e) & 1u) && a[e] < b[e]) ? (1u << e) : 0u;
-    while (pending) {  // last t in [a, b] with T_t <= k (T_a <= k holds); usually one step
-#pragma unroll
+    auto rows_of = [&](int it, uint32_t i0, int& keepbits, int64_t& row) {
+      const uint32_t i = i0 + uint32_t(lane);
+      const bool full_iter = (i0 > 0 || off0 == 0) && 4 * (i0 + 64) <= span_s;
+      keepbits = 0xF;
+      if (it < first_drop) {
+        if (!full_iter) {
+          keepbits = 0;
+  #pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const uint32_t r = 4 * i + e;
+            keepbits |= (i < nblk && r >= off0 && r < span_s) ? (1 << e) : 0;
+          }
+        }
+        const int32_t r0 = int32_t(4 * i) - int32_t(off0);
+        row = row0 + (r0 < 0 ? 0 : r0);
+      } else {
+        const uint64_t* mw = mask + int64_t(it) * 4;
+        keepbits = 0;
+  #pragma unroll
+        for (int e = 0; e < 4; ++e) keepbits |= int((mw[e] >> lane) & 1ull) << e;
+        int total;
+        row = row0 + running + wave_scan_counts(__popc(keepbits), &total);
+        running += total;
+      }
+    };
+
+    // Per trip, 4 * kTrip keys per lane go through each step together (their
+    // LDS reads are independent): cell bounds, bisection, per-bin data, then the
+    // cotangents and the histogram updates of the kept keys inside the range.
+    constexpr int NK = 4 * kTrip;
+    int it = 0;
+    for (uint32_t i0 = 0; i0 < nblk; i0 += 64 * kTrip, it += kTrip) {
+      uint32_t k[NK];
+  #pragma unroll
+      for (int h = 0; h < kTrip; ++h) {
+        const Words4 wd =
+            philox4x32_10_uniform(sg.qa + i0 + 64 * h + uint32_t(lane), in.stream, P.seed);
+        k[4 * h] = wd.w0 >> 8;
+        k[4 * h + 1] = wd.w1 >> 8;
+        k[4 * h + 2] = wd.w2 >> 8;
+        k[4 * h + 3] = wd.w3 >> 8;
+      }
+      int kb[kTrip];
+      int64_t rw[kTrip];
+  #pragma unroll
+      for (int h = 0; h < kTrip; ++h) {
+        kb[h] = 0;
+        rw[h] = 0;
+        if (h == 0 || i0 + 64 * h < nblk) rows_of(it + h, i0 + 64 * h, kb[h], rw[h]);
+      }
+      uint32_t use = 0u;  // kept and inside [K_lo, K_hi)
+      int a[NK], b[NK];
+  #pragma unroll
       for (int e = 0; e < NK; ++e) {
-        if (!((pending >> e) & 1u)) continue;
-        const int mid = (a[e] + b[e] + 1) >> 1;
-        if (thr[mid] <= k[e]) a[e] = mid; else b[e] = mid - 1;
-        if (a[e] >= b[e]) pending &= ~(1u << e);
+        const bool in_range = ((kb[e >> 2] >> (e & 3)) & 1) && k[e] >= k_lo && k[e] < k_hi;
+        use |= in_range ? (1u << e) : 0u;
+        const uint32_t cell = in_range ? (k[e] - k_lo) >> cs : 0u;
+        a[e] = int(cell_bin[cell]);
+        b[e] = int(cell_bin[cell + 1]);
       }
-    }
-    float th[NK];
-    double tc[NK];
-#pragma unroll
-    for (int e = 0; e < NK; ++e) {
-      a[e] = a[e] > jmax ? jmax : a[e];
-      if constexpr (fixed) {
-        const float2 ab = s_vb[a[e]];
-        th[e] = __builtin_fmaf(static_cast<float>(static_cast<int32_t>(k[e] - thr[a[e]])), ab.x,
-                               ab.y);
-      } else {
-        tc[e] = double(k[e]) * 0x1.0p-24 - s_c[a[e]];
+      uint32_t pending = 0u;
+  #pragma unroll
+      for (int e = 0; e < NK; ++e) pending |= (((use >> e) & 1u) && a[e] < b[e]) ? (1u << e) : 0u;
+      while (pending) {  // last t in [a, b] with T_t <= k (T_a <= k holds); usually one step
+  #pragma unroll
+        for (int e = 0; e < NK; ++e) {
+          if (!((pending >> e) & 1u)) continue;
+          const int mid = (a[e] + b[e] + 1) >> 1;
+          if (thr[mid] <= k[e]) a[e] = mid; else b[e] = mid - 1;
+          if (a[e] >= b[e]) pending &= ~(1u << e);
+        }
       }
-    }
-#pragma unroll
-    for (int h = 0; h < kTrip; ++h) {
-      const uint32_t uh = (use >> (4 * h)) & 0xFu;
-      if (uh == 0u) continue;
-      Real g[4];
-      if (R == 1 && kb[h] == 0xF && (rw[h] & 3) == 0) {
+      float th[NK];
+      double tc[NK];
+  #pragma unroll
+      for (int e = 0; e < NK; ++e) {
+        a[e] = a[e] > jmax ? jmax : a[e];
         if constexpr (fixed) {
-          const float4 g4 = *reinterpret_cast<const float4*>(grad + rw[h]);
-          g[0] = g4.x; g[1] = g4.y; g[2] = g4.z; g[3] = g4.w;
+          const float2 ab = s_vb[a[e]];
+          th[e] = __builtin_fmaf(static_cast<float>(static_cast<int32_t>(k[e] - thr[a[e]])), ab.x,
+                                 ab.y);
         } else {
-          const double2 ga = reinterpret_cast<const double2*>(grad + rw[h])[0];
-          const double2 gb = reinterpret_cast<const double2*>(grad + rw[h])[1];
-          g[0] = ga.x; g[1] = ga.y; g[2] = gb.x; g[3] = gb.y;
-        }
-      } else {
-        int64_t r = rw[h];
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          g[e] = ((uh >> e) & 1u) ? grad[r] : Real(0);
-          r += (kb[h] >> e) & 1;
+          tc[e] = double(k[e]) * 0x1.0p-24 - s_c[a[e]];
         }
       }
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        if (!((uh >> e) & 1u)) continue;
-        const int jr = a[4 * h + e];
-        if constexpr (fixed) {
-          const float gf = float(g[e]);
-          const float gs = gf * scale;
-          if (__builtin_expect(__builtin_fabsf(gs) < 0x1.0p30f, 1)) {
-            const int ia = static_cast<int>(__builtin_rintf(gs));
-            const int ib = static_cast<int>(__builtin_rintf(gs * th[4 * h + e]));
-            atomicAdd(reinterpret_cast<unsigned long long*>(a0 + jr),
-                      static_cast<unsigned long long>(static_cast<int64_t>(ia)));
-            atomicAdd(reinterpret_cast<unsigned long long*>(a1 + jr),
-                      static_cast<unsigned long long>(static_cast<int64_t>(ib)));
+  #pragma unroll
+      for (int h = 0; h < kTrip; ++h) {
+        const uint32_t uh = (use >> (4 * h)) & 0xFu;
+        if (uh == 0u) continue;
+        Real g[4];
+        if (R == 1 && kb[h] == 0xF && (rw[h] & 3) == 0) {
+          if constexpr (fixed) {
+            const float4 g4 = *reinterpret_cast<const float4*>(grad + rw[h]);
+            g[0] = g4.x; g[1] = g4.y; g[2] = g4.z; g[3] = g4.w;
           } else {
-            sh_outl = 1;
-            unsafeAtomicAdd(outl + lo + jr, double(gf));
-            unsafeAtomicAdd(outl + nb + lo + jr, double(gf) * double(th[4 * h + e]));
+            const double2 ga = reinterpret_cast<const double2*>(grad + rw[h])[0];
+            const double2 gb = reinterpret_cast<const double2*>(grad + rw[h])[1];
+            g[0] = ga.x; g[1] = ga.y; g[2] = gb.x; g[3] = gb.y;
           }
         } else {
-          const double ge = double(g[e]);
-          atomicAdd(d0 + jr, ge);
-          atomicAdd(d1 + jr, ge * tc[4 * h + e]);
+          int64_t r = rw[h];
+  #pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            g[e] = ((uh >> e) & 1u) ? grad[r] : Real(0);
+            r += (kb[h] >> e) & 1;
+          }
+        }
+  #pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          if (!((uh >> e) & 1u)) continue;
+          const int jr = a[4 * h + e];
+          if constexpr (fixed) {
+            const float gf = float(g[e]);
+            const float gs = gf * scale;
+            if (__builtin_expect(__builtin_fabsf(gs) < 0x1.0p30f, 1)) {
+              const int ia = static_cast<int>(__builtin_rintf(gs));
+              const int ib = static_cast<int>(__builtin_rintf(gs * th[4 * h + e]));
+              atomicAdd(reinterpret_cast<unsigned long long*>(a0 + jr),
+                        static_cast<unsigned long long>(static_cast<int64_t>(ia)));
+              atomicAdd(reinterpret_cast<unsigned long long*>(a1 + jr),
+                        static_cast<unsigned long long>(static_cast<int64_t>(ib)));
+            } else {
+              sh_outl = 1;
+              unsafeAtomicAdd(outl + lo + jr, double(gf));
+              unsafeAtomicAdd(outl + nb + lo + jr, double(gf) * double(th[4 * h + e]));
+            }
+          } else {
+            const double ge = double(g[e]);
+            atomicAdd(d0 + jr, ge);
+            atomicAdd(d1 + jr, ge * tc[4 * h + e]);
+          }
         }
       }
     }

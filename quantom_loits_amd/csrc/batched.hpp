@@ -27,8 +27,8 @@ struct BatchedParams {
   int64_t* total;        // [1]
   double* norm_out;      // optional [n_instances]
   double* cdf_out;       // optional [total_nodes]
-  int64_t n_segments;    // n_chunks * waves
-  int32_t waves;         // waves per work-group
+  int64_t n_segments;    // n_chunks * segs
+  int32_t segs;          // segments per chunk (waves per work-group * kSegsPerWave)
   int32_t pad2;
   // scratch
   int32_t* flags;        // [0] = dense

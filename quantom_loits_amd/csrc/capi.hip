@@ -101,6 +101,7 @@ int make_plan(const loits_hip_instance* instances, int64_t n, int dtype,
   chunk_events = (chunk_events + per_iter - 1) / per_iter * per_iter;
   const int64_t coef_size = dtype == LOITS_HIP_F32 ? 16 : 32;
   const int nwaves = threads / 64;
+  const int nsegs = nwaves * kSegsPerWave;  // segments per chunk
 
   loits_hip_layout& L = plan->layout;
   std::memset(&L, 0, sizeof(L));
@@ -195,9 +196,9 @@ int make_plan(const loits_hip_instance* instances, int64_t n, int dtype,
       const uint64_t p_lo = s.counter_base + uint64_t(ch.slot_lo);
       const uint64_t p_hi = s.counter_base + uint64_t(ch.slot_hi);
       const uint64_t blocks = ch.slot_hi > ch.slot_lo ? ((p_hi + 3) >> 2) - (p_lo >> 2) : 0;
-      const int64_t per_wave = int64_t(blocks_per_wave(blocks, nwaves) / 64);
+      const int64_t per_seg = int64_t(blocks_per_wave(blocks, nsegs) / 64);
       ch.mask_off = mask_words;
-      mask_words += per_wave * nwaves * 4;
+      mask_words += per_seg * nsegs * 4;
       ch.partial_off = partial_doubles;  // VJP overflow sums, then partials
       partial_doubles += 2 * pad4(s.n_nodes);
       if (d.vjp_ranges == 0)
@@ -222,7 +223,7 @@ int make_plan(const loits_hip_instance* instances, int64_t n, int dtype,
   L.lds_forward = lds_f;
   L.lds_backward = lds_b;
   L.lds_tables = lds_t;
-  L.n_segments = L.n_chunks * nwaves;
+  L.n_segments = L.n_chunks * nsegs;
   if (lds_f > kMaxLds || lds_t > kMaxLds)
     return fail(LOITS_HIP_EUNSUPPORTED,
                 "batched: curve tables of " + std::to_string(L.max_nodes) +
@@ -271,7 +272,7 @@ BatchedParams params_of(const loits_hip_batched_args* a) {
   P.cdf_out = a->cdf;
   uint8_t* s = static_cast<uint8_t*>(a->scratch);
   P.n_segments = L.n_segments;
-  P.waves = L.threads / 64;
+  P.segs = (L.threads / 64) * kSegsPerWave;
   P.flags = reinterpret_cast<int32_t*>(s);
   s += 64;
   P.seg_kept = reinterpret_cast<int64_t*>(s);

@@ -218,7 +218,15 @@ struct ChunkDesc {
   int32_t index;        // position among the instance's chunks
 };
 
-// Blocks of wave w of a chunk: [qa, qb), and the slot counters it covers.
+// A chunk's Philox blocks are split into kSegsPerWave segments per wave of
+// its work-group; waves take segments from an LDS counter until none is left,
+// so a wave that runs fast takes more of them.
+#ifndef QLOITS_SEGS_PER_WAVE
+#define QLOITS_SEGS_PER_WAVE 4
+#endif
+constexpr int kSegsPerWave = QLOITS_SEGS_PER_WAVE;
+
+// Blocks of segment w of nwaves in a chunk: [qa, qb), and the slot counters it covers.
 struct Segment {
   uint64_t qa, qb;      // Philox blocks
   uint64_t p_lo, p_hi;  // counters (slots + counter_base), p_lo <= p_hi

@@ -76,7 +76,7 @@ def test_batched_layout_sizes_and_tiles(lib):
     assert lib.loits_hip_batched_layout(inst, 3, 1, C.byref(tun), C.byref(lay)) == 0
     assert lay.capacity == 14096 and lay.total_nodes == 3 * 2048 and lay.max_nodes == 2048
     assert lay.n_chunks == 3 + 1 + 1  # ceil(10000/4096), empty instance keeps one tile, 1
-    assert lay.n_segments == lay.n_chunks * 4  # 256 threads = 4 waves
+    assert lay.n_segments == lay.n_chunks * 4 * 4  # 256 threads = 4 waves, 4 segments each
     assert lay.lds_forward > 0 and lay.lds_backward > lay.lds_forward - 32 * 2048
     assert lay.partial_doubles == lay.n_chunks * 2 * 2048  # 2 pad4(n) per chunk
     blob = (C.c_uint8 * lay.desc_bytes)()
