@@ -124,6 +124,21 @@ int64_t loits_hip_forward_workspace_size(const loits_hip_shape* shape, int64_t n
  * reference's.  No host synchronisation: valid[0] holds the kept count. */
 int loits_hip_forward(const loits_hip_forward_args* args, void* stream);
 
+/* Sync-free form of loits_hip_forward, for callers that cannot read nmax and
+ * active on the host between allocation and sampling (HIP graph capture, a
+ * loop that keeps buffers across calls).  args->nmax and args->active are
+ * CAPACITIES: they size u_x/u_q/interval_x/interval_q (B*cells*nmax_cap),
+ * events/packed (active_cap) and the workspace; the true nmax and active are
+ * read on the device from nmax_active (written by loits_hip_allocate earlier
+ * on the stream).  The reference's arrays are then prefixes of the capacity
+ * buffers (u_x: the first B*cells*nmax floats, [B, cells, nmax]; events: the
+ * first valid[0] rows).  If nmax or active exceeds its capacity nothing is
+ * sampled and valid[1] gets bit 1 (value 2).  No reference counterpart: the
+ * reference reads the sizes on the host (cpp/bindings.cpp:80-92); this entry
+ * point is for the callers that cannot. */
+int loits_hip_forward_bounded(const loits_hip_forward_args* args, const int64_t* nmax_active,
+                              void* stream);
+
 typedef struct loits_hip_backward_args {
   loits_hip_shape shape;
   int dtype;
@@ -154,6 +169,13 @@ int64_t loits_hip_backward_workspace_size(const loits_hip_shape* shape, int64_t 
  * (cpp/loits_core.cpp:268-338, rows of one cell in row order as
  * openmp/loits_core.cpp:285-352), CDF VJP (:340-398), density VJP (:400-472). */
 int loits_hip_backward(const loits_hip_backward_args* args, void* stream);
+
+/* Sync-free form of loits_hip_backward after loits_hip_forward_bounded:
+ * nmax is read from nmax_active[0] and the row count from valid[0] on the
+ * device (args->nmax, args->nvalid are the capacities).  No reference
+ * counterpart (see loits_hip_forward_bounded). */
+int loits_hip_backward_bounded(const loits_hip_backward_args* args, const int64_t* nmax_active,
+                               const int64_t* valid, void* stream);
 
 /* ---------------------------------------------------------------------- */
 /* Batched 1-D path                                                         */

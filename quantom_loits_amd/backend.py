@@ -104,3 +104,87 @@ class HipLOITS(nn.Module):
         with _region(self.profile_regions, "loits::autograd::forward"):
             return _HipLOITSFunction.apply(x_bins, xsec_x, q2_bins, xsec_q2, weights, acceptance,
                                            n_events, self.seed, sequence, self.profile_regions)
+
+
+class BoundedLOITS:
+    """The reference forward/backward with the sizes kept on the device.
+
+    The reference reads nmax and active on the host between allocation and
+    sampling (cpp/bindings.cpp:80-92).  Here every buffer is allocated once for
+    capacities (``nmax_cap`` slots per cell, ``active_cap`` events) and
+    :meth:`forward` / :meth:`backward` only enqueue kernels, so a caller can
+    capture them in a HIP graph.  :meth:`state` reads the sizes once and returns
+    the reference's 11-tensor state as views of the buffers (each array of the
+    reference is a prefix of its capacity buffer).  If a call needs more than
+    the capacities it samples nothing and :meth:`sizes` reports error bit 2.
+    Cotangents go into ``grad_events`` ([active_cap, 2]) before :meth:`backward`.
+    """
+
+    def __init__(self, x_bins, q_bins, dtype, nmax_cap, active_cap):
+        ext = prepare_extension(x_bins.device)
+        self.ext = ext
+        B, nx, kx = x_bins.shape
+        ny, kq = q_bins.shape[1], q_bins.shape[2]
+        self.shape = (B, nx, ny, kx, kq)
+        g = B * nx * ny
+        dev = x_bins.device
+        i64 = dict(dtype=torch.int64, device=dev)
+        f64 = dict(dtype=torch.float64, device=dev)
+        self.nmax_cap, self.active_cap = int(nmax_cap), int(active_cap)
+        self.counts = torch.zeros(g, **i64)
+        self.na = torch.zeros(2, **i64)
+        self.valid = torch.zeros(2, **i64)
+        self.norm_x = torch.empty(B, ny, nx, **f64)
+        self.norm_q = torch.empty(B, nx, ny, **f64)
+        self.cdf_x = torch.empty(B, ny, nx, kx, **f64)
+        self.cdf_q = torch.empty(B, nx, ny, kq, **f64)
+        cap = max(g * self.nmax_cap, 1)
+        self.u_x = torch.empty(cap, dtype=torch.float32, device=dev)
+        self.u_q = torch.empty(cap, dtype=torch.float32, device=dev)
+        self.interval_x = torch.empty(cap, dtype=torch.int16, device=dev)
+        self.interval_q = torch.empty(cap, dtype=torch.int16, device=dev)
+        acap = max(self.active_cap, 1)
+        self.events = torch.empty(acap, 2, dtype=dtype, device=dev)
+        self.packed = torch.empty(acap, **i64)
+        self.row_offsets = torch.empty(g + 1, **i64)
+        self.fwd_ws = torch.empty(max(int(ext.forward_workspace_size(x_bins, q_bins,
+                                                                     self.nmax_cap)), 1),
+                                  dtype=torch.uint8, device=dev)
+        self.bwd_ws = torch.empty(max(int(ext.backward_workspace_size(x_bins, q_bins)), 1),
+                                  dtype=torch.uint8, device=dev)
+        self.grad_events = torch.zeros(acap, 2, dtype=dtype, device=dev)
+        self.grad_x = torch.empty(B, ny, nx, kx, dtype=dtype, device=dev)
+        self.grad_q = torch.empty(B, nx, ny, kq, dtype=dtype, device=dev)
+
+    def forward(self, x_bins, xsec_x, q_bins, xsec_q, weights, acceptance, n_events, seed,
+                sequence):
+        self.ext.forward_bounded(x_bins, xsec_x, q_bins, xsec_q, weights, acceptance,
+                                 int(n_events), int(seed), int(sequence), self.counts, self.na,
+                                 self.norm_x, self.norm_q, self.cdf_x, self.cdf_q, self.u_x,
+                                 self.u_q, self.interval_x, self.interval_q, self.events,
+                                 self.packed, self.row_offsets, self.fwd_ws, self.valid, None)
+
+    def backward(self, x_bins, xsec_x, q_bins, xsec_q, acceptance):
+        self.ext.backward_bounded(self.grad_events, x_bins, xsec_x, q_bins, xsec_q, acceptance,
+                                  self.norm_x, self.norm_q, self.cdf_x, self.cdf_q, self.u_x,
+                                  self.u_q, self.interval_x, self.interval_q, self.packed,
+                                  self.na, self.valid, self.grad_x, self.grad_q, self.bwd_ws,
+                                  None)
+
+    def sizes(self):
+        """(nmax, active, kept, error bits) with one host read."""
+        nmax, active = (int(v) for v in self.na.cpu())
+        kept, err = (int(v) for v in self.valid.cpu())
+        return nmax, active, kept, err
+
+    def state(self):
+        nmax, _, kept, err = self.sizes()
+        if err:
+            raise RuntimeError(f"bounded forward failed (error bits {err}); raise the capacities")
+        B, nx, ny, _, _ = self.shape
+        g = B * nx * ny
+        return [self.events[:kept], self.norm_x, self.norm_q, self.cdf_x, self.cdf_q,
+                self.u_x[:g * nmax].view(B, nx * ny, nmax),
+                self.u_q[:g * nmax].view(B, nx * ny, nmax),
+                self.interval_x[:g * nmax], self.interval_q[:g * nmax], self.packed[:kept],
+                self.row_offsets]

@@ -232,6 +232,120 @@ std::vector<at::Tensor> backward(at::Tensor grad_events, at::Tensor x_bins, at::
   return {grad_x, grad_q};
 }
 
+// Sync-free reference forward/backward into caller-owned capacity buffers
+// (loits_hip_forward_bounded / loits_hip_backward_bounded): nothing is read on
+// the host, so a caller can capture the calls in a HIP graph.  `na` ([2],
+// int64) receives nmax / active, `valid` ([2]) the kept count and error bits.
+void forward_bounded(at::Tensor x_bins, at::Tensor xsec_x, at::Tensor q_bins, at::Tensor xsec_q,
+                     at::Tensor weights, at::Tensor acceptance, int64_t n_events, uint64_t seed,
+                     uint64_t sequence, at::Tensor counts, at::Tensor na, at::Tensor norm_x,
+                     at::Tensor norm_q, at::Tensor cdf_x, at::Tensor cdf_q, at::Tensor u_x,
+                     at::Tensor u_q, at::Tensor interval_x, at::Tensor interval_q,
+                     at::Tensor events, at::Tensor packed, at::Tensor row_offsets,
+                     at::Tensor workspace, at::Tensor valid, py::object stream_obj) {
+  for (auto* p : {&x_bins, &xsec_x, &q_bins, &xsec_q, &weights, &acceptance, &counts, &na,
+                  &norm_x, &norm_q, &cdf_x, &cdf_q, &u_x, &u_q, &interval_x, &interval_q,
+                  &events, &packed, &row_offsets, &workspace, &valid})
+    require_device(*p, "bounded forward tensor");
+  const int dt = dtype_code(xsec_x, "xsec_x");
+  const loits_hip_shape s = shape_of(x_bins, q_bins);
+  const int64_t gcells = s.batch * s.cells;
+  const int64_t nmax_cap = gcells > 0 ? u_x.numel() / gcells : 0;
+  const int64_t active_cap = packed.numel();
+  if (u_q.numel() < gcells * nmax_cap || interval_x.numel() < gcells * nmax_cap ||
+      interval_q.numel() < gcells * nmax_cap || events.numel() < 2 * active_cap ||
+      row_offsets.numel() < gcells + 1 || na.numel() < 2 || valid.numel() < 2 ||
+      counts.numel() < gcells)
+    throw std::invalid_argument("bounded forward: a buffer is smaller than its capacity");
+  const int64_t ws = loits_hip_forward_workspace_size(&s, nmax_cap);
+  if (ws < 0 || workspace.numel() < ws)
+    throw std::invalid_argument("bounded forward: workspace too small");
+  void* stream = stream_of(stream_obj, xsec_x);
+  check(loits_hip_allocate(weights.data_ptr(), dt, gcells, n_events, counts.data_ptr<int64_t>(),
+                           na.data_ptr<int64_t>(), stream),
+        "loits_hip_allocate");
+  at::Tensor acc_u8 = acceptance.view(at::kByte);
+  loits_hip_forward_args a{};
+  a.shape = s;
+  a.dtype = dt;
+  a.nmax = nmax_cap;
+  a.active = active_cap;
+  a.seed = seed;
+  a.sequence = sequence;
+  a.x_bins = x_bins.data_ptr();
+  a.xsec_x = xsec_x.data_ptr();
+  a.q_bins = q_bins.data_ptr();
+  a.xsec_q = xsec_q.data_ptr();
+  a.acceptance = acc_u8.data_ptr<uint8_t>();
+  a.counts = counts.data_ptr<int64_t>();
+  a.norm_x = norm_x.data_ptr<double>();
+  a.norm_q = norm_q.data_ptr<double>();
+  a.cdf_x = cdf_x.data_ptr<double>();
+  a.cdf_q = cdf_q.data_ptr<double>();
+  a.u_x = u_x.data_ptr<float>();
+  a.u_q = u_q.data_ptr<float>();
+  a.interval_x = interval_x.data_ptr<int16_t>();
+  a.interval_q = interval_q.data_ptr<int16_t>();
+  a.events = events.data_ptr();
+  a.packed = packed.data_ptr<int64_t>();
+  a.row_offsets = row_offsets.data_ptr<int64_t>();
+  a.workspace = workspace.data_ptr();
+  a.valid = valid.data_ptr<int64_t>();
+  check(loits_hip_forward_bounded(&a, na.data_ptr<int64_t>(), stream), "loits_hip_forward_bounded");
+}
+
+void backward_bounded(at::Tensor grad_events, at::Tensor x_bins, at::Tensor xsec_x,
+                      at::Tensor q_bins, at::Tensor xsec_q, at::Tensor acceptance,
+                      at::Tensor norm_x, at::Tensor norm_q, at::Tensor cdf_x, at::Tensor cdf_q,
+                      at::Tensor u_x, at::Tensor u_q, at::Tensor interval_x,
+                      at::Tensor interval_q, at::Tensor packed, at::Tensor na, at::Tensor valid,
+                      at::Tensor grad_x, at::Tensor grad_q, at::Tensor workspace,
+                      py::object stream_obj) {
+  for (auto* p : {&grad_events, &x_bins, &xsec_x, &q_bins, &xsec_q, &acceptance, &norm_x,
+                  &norm_q, &cdf_x, &cdf_q, &u_x, &u_q, &interval_x, &interval_q, &packed, &na,
+                  &valid, &grad_x, &grad_q, &workspace})
+    require_device(*p, "bounded backward tensor");
+  const int dt = dtype_code(xsec_x, "xsec_x");
+  if (grad_events.scalar_type() != xsec_x.scalar_type() ||
+      grad_x.scalar_type() != xsec_x.scalar_type() || grad_q.scalar_type() != xsec_x.scalar_type())
+    throw std::invalid_argument("bounded backward: gradients must have xsec_x's dtype");
+  const loits_hip_shape s = shape_of(x_bins, q_bins);
+  const int64_t gcells = s.batch * s.cells;
+  if (grad_events.numel() < 2 * packed.numel() || grad_x.numel() != xsec_x.numel() ||
+      grad_q.numel() != xsec_q.numel())
+    throw std::invalid_argument("bounded backward: gradient buffers have the wrong size");
+  const int64_t ws = loits_hip_backward_workspace_size(&s, 0);
+  if (ws < 0 || workspace.numel() < ws)
+    throw std::invalid_argument("bounded backward: workspace too small");
+  at::Tensor acc_u8 = acceptance.view(at::kByte);
+  loits_hip_backward_args a{};
+  a.shape = s;
+  a.dtype = dt;
+  a.nmax = gcells > 0 ? u_x.numel() / gcells : 0;
+  a.nvalid = packed.numel();
+  a.grad_events = grad_events.data_ptr();
+  a.x_bins = x_bins.data_ptr();
+  a.xsec_x = xsec_x.data_ptr();
+  a.q_bins = q_bins.data_ptr();
+  a.xsec_q = xsec_q.data_ptr();
+  a.acceptance = acc_u8.data_ptr<uint8_t>();
+  a.norm_x = norm_x.data_ptr<double>();
+  a.norm_q = norm_q.data_ptr<double>();
+  a.cdf_x = cdf_x.data_ptr<double>();
+  a.cdf_q = cdf_q.data_ptr<double>();
+  a.u_x = u_x.data_ptr<float>();
+  a.u_q = u_q.data_ptr<float>();
+  a.interval_x = interval_x.data_ptr<int16_t>();
+  a.interval_q = interval_q.data_ptr<int16_t>();
+  a.packed = packed.data_ptr<int64_t>();
+  a.grad_xsec_x = grad_x.data_ptr();
+  a.grad_xsec_q = grad_q.data_ptr();
+  a.workspace = workspace.data_ptr();
+  check(loits_hip_backward_bounded(&a, na.data_ptr<int64_t>(), valid.data_ptr<int64_t>(),
+                                   stream_of(stream_obj, xsec_x)),
+        "loits_hip_backward_bounded");
+}
+
 at::Tensor philox_fill(int64_t count, uint64_t seed, uint64_t stream_id, int64_t device,
                        py::object stream_obj) {
   at::Tensor out = at::empty({count}, at::TensorOptions().dtype(at::kFloat).device(at::kCUDA, device));
@@ -466,6 +580,16 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
         py::arg("stream") = py::none());
   m.def("device_info", &device_info, py::arg("device") = 0);
   m.def("version", []() { return std::string(loits_hip_version()); });
+  m.def("forward_bounded", &forward_bounded);
+  m.def("backward_bounded", &backward_bounded);
+  m.def("forward_workspace_size", [](at::Tensor x_bins, at::Tensor q_bins, int64_t nmax) {
+    const loits_hip_shape s = shape_of(x_bins, q_bins);
+    return loits_hip_forward_workspace_size(&s, nmax);
+  });
+  m.def("backward_workspace_size", [](at::Tensor x_bins, at::Tensor q_bins) {
+    const loits_hip_shape s = shape_of(x_bins, q_bins);
+    return loits_hip_backward_workspace_size(&s, 0);
+  });
   m.def("soft_histogram", &soft_histogram, py::arg("values"), py::arg("row_start"),
         py::arg("lo"), py::arg("hi"), py::arg("bins"), py::arg("width"),
         py::arg("stream") = py::none());

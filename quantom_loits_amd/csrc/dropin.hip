@@ -240,16 +240,32 @@ struct SampleParams {
   uint64_t* tile_status;
   uint32_t* tile_counter;
   int64_t* valid;  // [0] kept, [1] error flag
+  // sync-free form: nmax and active read here; the values above are capacities
+  const int64_t* dev_na;
+  int64_t active_cap;
 };
 
 template <typename Real>
-__global__ void __launch_bounds__(kTileThreads) sample_kernel(SampleParams P) {
+__global__ void __launch_bounds__(kTileThreads) sample_kernel(SampleParams P0) {
   __shared__ int32_t wave_tot[16];
   __shared__ int64_t sh_tile, sh_excl;
+  SampleParams P = P0;
   const loits_hip_shape& s = P.s;
+  if (P0.dev_na) {  // sizes from the device; the launch covers the capacities
+    const int64_t nm = P0.dev_na[0], act = P0.dev_na[1];
+    if (nm > P0.nmax || act > P0.active_cap) {
+      if (blockIdx.x == 0 && threadIdx.x == 0)
+        atomicOr(reinterpret_cast<unsigned long long*>(&P0.valid[1]), 2ull);
+      return;
+    }
+    P.nmax = nm;
+    P.total_slots = s.batch * s.cells * nm;
+    P.n_tiles = (P.total_slots + kTileSlots - 1) / kTileSlots;
+  }
   if (threadIdx.x == 0) sh_tile = int64_t(atomicAdd(P.tile_counter, 1u));
   __syncthreads();
   const int64_t tile = sh_tile;
+  if (tile >= P.n_tiles) return;  // (capacity tiles past the live slots)
   const int64_t q = tile * kTileThreads + threadIdx.x;  // Philox block of this thread
   const Words4 wx = philox4x32_10(uint64_t(q), P.sid_x, P.seed);
   const Words4 wq = philox4x32_10(uint64_t(q), P.sid_q, P.seed);
@@ -407,9 +423,10 @@ __global__ void __launch_bounds__(kTileThreads) sample_kernel(SampleParams P) {
 }
 
 __global__ void row_offsets_kernel(const int64_t* packed, const int64_t* valid, int64_t nmax,
-                                   int64_t gcells, int64_t* row_offsets) {
+                                   int64_t gcells, int64_t* row_offsets, const int64_t* dev_na) {
   const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (g > gcells) return;
+  if (dev_na) nmax = dev_na[0];
   const int64_t key = g * nmax;
   int64_t lo = 0, hi = valid[0];
   while (lo < hi) {
@@ -444,7 +461,19 @@ struct VjpParams {
   double* grho;  // scratch per curve, same layout as gcdf (x then q)
   void* grad_x;
   void* grad_q;
+  // sync-free form: nmax and nvalid read here
+  const int64_t* dev_nmax;
+  const int64_t* dev_nvalid;
 };
+
+__device__ __forceinline__ VjpParams with_device_sizes(const VjpParams& P0) {
+  VjpParams P = P0;
+  if (P0.dev_nmax) {
+    P.nmax = *P0.dev_nmax;
+    P.nvalid = *P0.dev_nvalid;
+  }
+  return P;
+}
 
 // cpp/loits_core.cpp:294-301, one row
 __device__ __forceinline__ void ivjp_row(const double* curve, double* gcurve, double bj,
@@ -459,7 +488,8 @@ __device__ __forceinline__ void ivjp_row(const double* curve, double* gcurve, do
 }
 
 template <typename Real>
-__global__ void __launch_bounds__(256) ivjp_kernel(VjpParams P) {
+__global__ void __launch_bounds__(256) ivjp_kernel(VjpParams P0) {
+  const VjpParams P = with_device_sizes(P0);
   const loits_hip_shape& s = P.s;
   const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (g >= s.batch * s.cells) return;
@@ -505,7 +535,8 @@ __device__ __forceinline__ void ivjp_row_strided(const double* curve, double* gc
 // the same operations as ivjp_kernel) and the loads of four rows are issued
 // before their accumulation.
 template <typename Real>
-__global__ void __launch_bounds__(64) ivjp_small_kernel(VjpParams P) {
+__global__ void __launch_bounds__(64) ivjp_small_kernel(VjpParams P0) {
+  const VjpParams P = with_device_sizes(P0);
   __shared__ double gx[kSmallK * 64], gq[kSmallK * 64];
   const loits_hip_shape& s = P.s;
   const int lane = threadIdx.x;
@@ -573,7 +604,8 @@ struct RowF {
 };
 
 template <typename Real>
-__global__ void __launch_bounds__(256) ivjp_wave_kernel(VjpParams P) {
+__global__ void __launch_bounds__(256) ivjp_wave_kernel(VjpParams P0) {
+  const VjpParams P = with_device_sizes(P0);
   __shared__ RowF rows[4 * kWave];
   const loits_hip_shape& s = P.s;
   const int lane = lane_id();
@@ -799,7 +831,13 @@ int64_t loits_hip_forward_workspace_size(const loits_hip_shape* shape, int64_t n
   return fwd_workspace(*shape, nmax, nullptr).bytes;
 }
 
-int loits_hip_forward(const loits_hip_forward_args* a, void* stream) {
+}  // extern "C"
+
+namespace {
+
+// dev_na: NULL (a->nmax, a->active are the sizes) or the device array the
+// allocation wrote (a->nmax, a->active are then capacities).
+int forward_impl(const loits_hip_forward_args* a, const int64_t* dev_na, void* stream) {
   if (!a) return fail(LOITS_HIP_EINVAL, "forward: args is NULL");
   const loits_hip_shape& s = a->shape;
   int rc = check_shape(s);
@@ -851,7 +889,7 @@ int loits_hip_forward(const loits_hip_forward_args* a, void* stream) {
                    w.thr_q,    w.m_x,      w.m_q,      w.mono,      a->u_x,
                    a->u_q,     a->interval_x,          a->interval_q,
                    a->events,  a->packed,  w.tile_status,           w.tile_counter,
-                   a->valid};
+                   a->valid,   dev_na,     a->active};
     if (f32)
       hipLaunchKernelGGL(sample_kernel<float>, dim3(unsigned(tiles)), dim3(kTileThreads), 0, st, S);
     else
@@ -862,10 +900,24 @@ int loits_hip_forward(const loits_hip_forward_args* a, void* stream) {
   if (a->row_offsets) {
     const int64_t gcells = s.batch * s.cells;
     hipLaunchKernelGGL(row_offsets_kernel, dim3(blocks_for(gcells + 1, 256)), dim3(256), 0, st,
-                       a->packed, a->valid, a->nmax, gcells, a->row_offsets);
+                       a->packed, a->valid, a->nmax, gcells, a->row_offsets, dev_na);
     rc = check_hip(hipGetLastError(), "forward row_offsets launch");
   }
   return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int loits_hip_forward(const loits_hip_forward_args* a, void* stream) {
+  return forward_impl(a, nullptr, stream);
+}
+
+int loits_hip_forward_bounded(const loits_hip_forward_args* a, const int64_t* nmax_active,
+                              void* stream) {
+  if (!nmax_active) return fail(LOITS_HIP_EINVAL, "forward_bounded: nmax_active is NULL");
+  return forward_impl(a, nmax_active, stream);
 }
 
 int64_t loits_hip_backward_workspace_size(const loits_hip_shape* shape, int64_t nmax) {
@@ -876,7 +928,12 @@ int64_t loits_hip_backward_workspace_size(const loits_hip_shape* shape, int64_t 
   return 2 * (align16(8 * nxe) + align16(8 * nqe));
 }
 
-int loits_hip_backward(const loits_hip_backward_args* a, void* stream) {
+}  // extern "C"
+
+namespace {
+
+int backward_impl(const loits_hip_backward_args* a, const int64_t* dev_nmax,
+                  const int64_t* dev_nvalid, void* stream) {
   if (!a) return fail(LOITS_HIP_EINVAL, "backward: args is NULL");
   const loits_hip_shape& s = a->shape;
   int rc = check_shape(s);
@@ -900,7 +957,7 @@ int loits_hip_backward(const loits_hip_backward_args* a, void* stream) {
               a->q_bins, a->xsec_q,    a->acceptance, a->norm_x,   a->norm_q, a->cdf_x,
               a->cdf_q,  a->u_x,       a->u_q,     a->interval_x,  a->interval_q,
               a->packed, gcdf_x,       gcdf_q,     grho,           a->grad_xsec_x,
-              a->grad_xsec_q};
+              a->grad_xsec_q, dev_nmax, dev_nvalid};
   const bool f32 = a->dtype == LOITS_HIP_F32;
   const int64_t gcells = s.batch * s.cells;
   if (s.kx <= kWave && s.kq <= kWave) {
@@ -925,6 +982,21 @@ int loits_hip_backward(const loits_hip_backward_args* a, void* stream) {
     hipLaunchKernelGGL(curve_vjp_kernel<double>, dim3(blocks_for(curves, 256)), dim3(256), 0, st,
                        V);
   return check_hip(hipGetLastError(), "backward curve launch");
+}
+
+}  // namespace
+
+extern "C" {
+
+int loits_hip_backward(const loits_hip_backward_args* a, void* stream) {
+  return backward_impl(a, nullptr, nullptr, stream);
+}
+
+int loits_hip_backward_bounded(const loits_hip_backward_args* a, const int64_t* nmax_active,
+                               const int64_t* valid, void* stream) {
+  if (!nmax_active || !valid)
+    return fail(LOITS_HIP_EINVAL, "backward_bounded: nmax_active or valid is NULL");
+  return backward_impl(a, nmax_active, valid, stream);
 }
 
 }  // extern "C"

@@ -539,3 +539,79 @@ def test_batched_float32_vjp_heavy_tailed_cotangents(oracle_port, hip_device, no
     err = np.abs(got - ref).max(axis=1) / np.abs(ref).max(axis=1)
     print("heavy-tailed cotangents", nodes, err, flush=True)
     assert err.max() <= 1e-4
+
+
+def _dropin_inputs(rng, B, nx, ny, k, dtype):
+    xb = np.sort(rng.uniform(0.001, 0.999, (B, nx, k)), axis=2)
+    qb = np.sort(rng.uniform(1.0, 100.0, (B, ny, k)), axis=2)
+    xs = rng.uniform(0.1, 1.0, (B, ny, nx, k))
+    qs = rng.uniform(0.1, 1.0, (B, nx, ny, k))
+    w = rng.uniform(0.5, 1.5, (B, nx, ny))
+    w /= w.sum()
+    acc = rng.uniform(0, 1, (B, nx, ny)) < 0.8
+    return [torch.from_numpy(a).to(dtype).cuda() for a in (xb, xs, qb, qs, w)] + \
+        [torch.from_numpy(acc).cuda()]
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_bounded_dropin_matches_reference_path_and_replays_in_a_graph(hip_device, dtype):
+    """The sync-free reference forward/backward (device-side nmax / active,
+    capacity buffers) against the host-sized binding, eagerly and as a HIP graph
+    replayed on new weights with eager work between replays; capacities that
+    are too small report an error and sample nothing."""
+    from quantom_loits_amd import load_extension
+    from quantom_loits_amd.backend import BoundedLOITS
+
+    ext = load_extension()
+    rng = np.random.default_rng(40)
+    xb, xs, qb, qs, w, acc = _dropin_inputs(rng, 2, 7, 5, 10, dtype)
+    E, seed, seq = 20000, 99, 3
+    b = BoundedLOITS(xb, qb, dtype, nmax_cap=2000, active_cap=2 * E)
+
+    def reference(weights, grad_rows):
+        st = ext.forward(xb, xs, qb, qs, weights, acc, E, seed, seq, False)
+        g = grad_rows[: st[0].shape[0]].contiguous()
+        gx, gq = ext.backward(g, xb, xs, qb, qs, acc, *st[1:], False)
+        return st, gx, gq
+
+    def check(weights):
+        st, gx, gq = reference(weights, b.grad_events)
+        got = b.state()
+        for a, r in zip(got, st):
+            assert a.shape == r.shape and torch.equal(a, r)
+        assert torch.equal(b.grad_x, gx) and torch.equal(b.grad_q, gq)
+
+    print("bounded eager", dtype, flush=True)
+    b.grad_events.copy_(torch.randn(b.grad_events.shape, dtype=dtype, device="cuda"))
+    b.forward(xb, xs, qb, qs, w, acc, E, seed, seq)
+    b.backward(xb, xs, qb, qs, acc)
+    check(w)
+
+    print("bounded graph", dtype, flush=True)
+    w_in = w.clone()
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        b.forward(xb, xs, qb, qs, w_in, acc, E, seed, seq)
+        b.backward(xb, xs, qb, qs, acc)
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        b.forward(xb, xs, qb, qs, w_in, acc, E, seed, seq)
+        b.backward(xb, xs, qb, qs, acc)
+    for r in range(3):
+        nw = torch.from_numpy(rng.uniform(0.2, 2.0, tuple(w.shape))).to(dtype).cuda()
+        w_in.copy_(nw / nw.sum())
+        graph.replay()
+        ext.forward(xb, xs, qb, qs, w, acc, 1000, 5, 1, False)  # unrelated eager work
+        torch.cuda.synchronize()
+        check(w_in)
+
+    print("bounded overflow", dtype, flush=True)
+    small = BoundedLOITS(xb, qb, dtype, nmax_cap=10, active_cap=2 * E)
+    small.forward(xb, xs, qb, qs, w, acc, E, seed, seq)
+    nmax, active, kept, err = small.sizes()
+    assert nmax > 10 and err & 2 and kept == 0
+    with pytest.raises(RuntimeError, match="capacities"):
+        small.state()
