@@ -587,12 +587,6 @@ __device__ __forceinline__ void find_bins(const uint32_t (&k)[NK], const uint32_
   }
 }
 
-__device__ __forceinline__ void find_bins4(const uint32_t (&k)[4], const uint32_t* guide,
-                                           const uint32_t* srt, int shift, int n,
-                                           int (&j)[4]) {
-  find_bins<4>(k, guide, srt, shift, n, j);
-}
-
 template <bool WINDOW>
 __device__ __forceinline__ bool keep_value(double v, const BatchedParams& P) {
   if (!(__builtin_isfinite(v) && v != 0.0)) return false;

@@ -13,9 +13,13 @@
 //                      (:239-266) in a single pass, tiles ordered by a
 //                      decoupled look-back
 //   row_offsets_kernel openmp/loits_core.cpp:244-262 from the packed slots
-//   ivjp_kernel        one thread per cell walks its rows in order (the
-//                      accumulation order of openmp/loits_core.cpp:285-352)
+//   ivjp_wave_kernel   curves of at most 64 nodes: one wave per cell, rows in
+//                      order (the accumulation order of
+//                      openmp/loits_core.cpp:285-352); ivjp_kernel, one
+//                      thread per cell, for larger curves
 //   curve_vjp_kernel   one thread per curve: CDF and density adjoints (:340-472)
+// The *_bounded entry points read nmax, active and the kept count on the
+// device instead of taking them from the host.
 // Built with -ffp-contract=off; every fused multiply-add of the reference
 // build is an explicit fma, so fp64 inputs give the reference's bits.
 #include <hip/hip_runtime.h>
@@ -515,87 +519,6 @@ __global__ void __launch_bounds__(256) ivjp_kernel(VjpParams P0) {
              ld<Real>(P.q_bins, bq + jq + 1), jq, double(P.u_q[p]),
              ld<Real>(P.grad, 2 * row + 1));
   }
-}
-
-// ivjp_row with the accumulators at stride `st` (LDS, one column per lane).
-__device__ __forceinline__ void ivjp_row_strided(const double* curve, double* gcurve, int st,
-                                                 double bj, double bj1, int64_t j, double uv,
-                                                 double gev) {
-  const double c0 = curve[j];
-  const double d = curve[j + 1] - c0 + kEpsilon;
-  const double inv_d = 1.0 / d;
-  const double t = uv - c0;
-  const double upstream = gev * (bj1 - bj);
-  gcurve[j * st] = __builtin_fma(upstream, __builtin_fma(t * inv_d, inv_d, -inv_d), gcurve[j * st]);
-  gcurve[(j + 1) * st] = __builtin_fma(-(upstream * t * inv_d), inv_d, gcurve[(j + 1) * st]);
-}
-
-// Curves of at most kSmallK nodes per axis: one lane per cell, 64 cells per
-// work-group; the cell's CDF gradients accumulate in LDS (rows in row order,
-// the same operations as ivjp_kernel) and the loads of four rows are issued
-// before their accumulation.
-template <typename Real>
-__global__ void __launch_bounds__(64) ivjp_small_kernel(VjpParams P0) {
-  const VjpParams P = with_device_sizes(P0);
-  __shared__ double gx[kSmallK * 64], gq[kSmallK * 64];
-  const loits_hip_shape& s = P.s;
-  const int lane = threadIdx.x;
-  const int64_t g = int64_t(blockIdx.x) * 64 + lane;
-  if (g >= s.batch * s.cells) return;  // (no barrier below)
-  const int64_t b = g / s.cells, cell = g - b * s.cells;
-  const int64_t ix = cell / s.ny, iy = cell - ix * s.ny;
-  const int64_t ox = ((b * s.ny + iy) * s.nx + ix) * s.kx, bx = (b * s.nx + ix) * s.kx;
-  const int64_t oq = ((b * s.nx + ix) * s.ny + iy) * s.kq, bq = (b * s.ny + iy) * s.kq;
-  double* ax = gx + lane;
-  double* aq = gq + lane;
-  for (int64_t t = 0; t < s.kx; ++t) ax[t * 64] = 0.0;
-  for (int64_t t = 0; t < s.kq; ++t) aq[t * 64] = 0.0;
-  // rows of this cell: packed slots in [g nmax, (g + 1) nmax), two searches together
-  const int64_t key0 = g * P.nmax, key1 = key0 + P.nmax;
-  int64_t lo0 = 0, hi0 = P.nvalid, lo1 = 0, hi1 = P.nvalid;
-  while (lo0 < hi0 || lo1 < hi1) {
-    if (lo0 < hi0) {
-      const int64_t mid = (lo0 + hi0) >> 1;
-      if (P.packed[mid] < key0) lo0 = mid + 1; else hi0 = mid;
-    }
-    if (lo1 < hi1) {
-      const int64_t mid = (lo1 + hi1) >> 1;
-      if (P.packed[mid] < key1) lo1 = mid + 1; else hi1 = mid;
-    }
-  }
-  const double* cx = P.cdf_x + ox;
-  const double* cq = P.cdf_q + oq;
-  for (int64_t row = lo0; row < lo1; row += 4) {
-    const int nr = lo1 - row < 4 ? int(lo1 - row) : 4;
-    int64_t jx[4], jq[4];
-    double ux[4], uq[4], gex[4], geq[4], x0[4], x1[4], q0[4], q1[4];
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int64_t rr = r < nr ? row + r : row;
-      const int64_t p = P.packed[rr];
-      jx[r] = P.interval_x[p];
-      jq[r] = P.interval_q[p];
-      ux[r] = double(P.u_x[p]);
-      uq[r] = double(P.u_q[p]);
-      gex[r] = ld<Real>(P.grad, 2 * rr);
-      geq[r] = ld<Real>(P.grad, 2 * rr + 1);
-    }
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      x0[r] = ld<Real>(P.x_bins, bx + jx[r]);
-      x1[r] = ld<Real>(P.x_bins, bx + jx[r] + 1);
-      q0[r] = ld<Real>(P.q_bins, bq + jq[r]);
-      q1[r] = ld<Real>(P.q_bins, bq + jq[r] + 1);
-    }
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      if (r >= nr) break;
-      ivjp_row_strided(cx, ax, 64, x0[r], x1[r], jx[r], ux[r], gex[r]);
-      ivjp_row_strided(cq, aq, 64, q0[r], q1[r], jq[r], uq[r], geq[r]);
-    }
-  }
-  for (int64_t t = 0; t < s.kx; ++t) P.gcdf_x[ox + t] = ax[t * 64];
-  for (int64_t t = 0; t < s.kq; ++t) P.gcdf_q[oq + t] = aq[t * 64];
 }
 
 struct RowF {
