@@ -546,14 +546,18 @@ __device__ __forceinline__ void find_bins(const uint32_t (&k)[NK], const uint32_
     j[e] = guide_direct_bin(w[e], k[e] & cmask);
     any |= w[e];
   }
-  if (__builtin_expect((any & kSpecial) != 0, 0)) {
+  if (ballot((any & kSpecial) != 0) != 0ull) {
+    // Level 2 for every key of the wave at once (a key outside crowded cells
+    // reads word 0 and keeps its bin): one uniform step instead of a branch
+    // per key, since most trips have some lane in a crowded cell.
 #pragma unroll
     for (int e = 0; e < NK; ++e) {
-      if ((w[e] & kSearch) == kSpecial) {  // indirect
-        const int ss = shift - int((w[e] >> 24) & 0xFu);
-        w[e] = guide[(w[e] & 0xFFFFFFu) + ((k[e] & cmask) >> ss)];
-        j[e] = guide_direct_bin(w[e], k[e] & ((1u << ss) - 1u));
-      }
+      const bool ind = (w[e] & kSearch) == kSpecial;
+      const int ss = shift - (ind ? int((w[e] >> 24) & 0xFu) : 0);
+      const uint32_t idx = ind ? (w[e] & 0xFFFFFFu) + ((k[e] & cmask) >> ss) : 0u;
+      const uint32_t w2 = guide[idx];
+      j[e] = ind ? guide_direct_bin(w2, k[e] & ((1u << ss) - 1u)) : j[e];
+      w[e] = ind ? w2 : w[e];
     }
     int lo[NK], len[NK], pending = 0;
 #pragma unroll
