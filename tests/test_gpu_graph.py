@@ -5,6 +5,8 @@ the captured buffers, with unrelated eager launches and host synchronisations
 between replays, and is checked against an eager run of the same step (and, on
 the first and last replay, against the oracle).
 """
+import json
+
 import numpy as np
 import pytest
 import torch
@@ -98,3 +100,32 @@ def test_graph_replays_match_eager_and_oracle(oracle_port, hip_device, dtype, wi
                                                  accept=accept, window=window)
             err = _grad_err(gd.double().cpu().numpy(), ref_g, w["node_off"], w["n_nodes"])
             assert err <= (1e-9 if dtype == torch.float64 else 1e-4), (r, err)
+
+
+def test_autotuner_times_candidates_on_the_gpu(oracle_port, hip_device, tmp_path, monkeypatch):
+    """The GPU timer runs each candidate shape; the chosen shape is cached and a
+    sampler built with tuning="auto" matches the oracle."""
+    from quantom_loits_amd import autotune
+    from quantom_loits_amd.batched import BatchedLOITS, Tuning
+    from quantom_loits_amd.workloads import uniform_family
+
+    print("autotune", flush=True)
+    monkeypatch.setenv("QUANTOM_LOITS_TUNE_CACHE", str(tmp_path / "tune.json"))
+    two = lambda scan_mode=0: [  # noqa: E731
+        Tuning(threads=256, chunk_events=1 << 16, guide_shift=1, scan_mode=scan_mode),
+        Tuning(threads=1024, chunk_events=1 << 16, guide_shift=2, scan_mode=scan_mode)]
+    monkeypatch.setattr(autotune, "_DEFAULT", autotune.Autotuner(candidates_fn=two))
+    B, N, E = 4, 300, 5000
+    s = BatchedLOITS.uniform(B, N, E, 1, dtype=torch.float32, tuning="auto")
+    assert s.tuning.threads in (256, 1024)
+    entry = json.loads((tmp_path / "tune.json").read_text())
+    assert len(entry) == 1 and next(iter(entry.values()))["candidates"] == 2
+    x, d = uniform_family(B, N, seed=2, dtype=np.float32)
+    vals = s(torch.from_numpy(x.reshape(-1)).cuda(), torch.from_numpy(d.reshape(-1)).cuda(), 9)
+    from oracle.oracle import uniform_batch
+
+    ref_v, _, _, _, _ = oracle_port.batched_forward(x.reshape(-1).astype(np.float64),
+                                                    d.reshape(-1).astype(np.float64),
+                                                    *uniform_batch(B, N, E, 1), 9)
+    v = vals.cpu().numpy().astype(np.float64)
+    assert v.shape == ref_v.shape and np.max(np.abs(v - ref_v) / np.abs(ref_v)) <= 1e-5
