@@ -714,7 +714,7 @@ __device__ __forceinline__ void sample_segment(const BatchedParams& P, int c, co
                                                const uint32_t* s_guide, const uint32_t* s_srt,
                                                const typename CoefOf<Real>::type* s_coef,
                                                const uint32_t* s_thr, const double* cdf,
-                                               bool safe_curve) {
+                                               bool safe_curve, int64_t* sh_kept) {
   const int n = static_cast<int>(in.n_nodes);
   const int shift = 24 - in.guide_log2;
   const Real* x_g = static_cast<const Real*>(P.x) + in.node_off;
@@ -813,7 +813,7 @@ __device__ __forceinline__ void sample_segment(const BatchedParams& P, int c, co
   }
   if (lane == 0) {
     P.seg_kept[seg] = running;
-    P.seg_cap[seg] = int64_t(sg.p_hi - sg.p_lo);
+    sh_kept[w] = running;
     P.seg_drop[seg] = first_drop;
   }
 }
@@ -823,6 +823,7 @@ __global__ void __launch_bounds__(1024) sample_kernel(BatchedParams P) {
   extern __shared__ __align__(16) uint8_t lds[];
   __shared__ BuildShared sh;
   __shared__ int sh_next;  // next unclaimed segment (read after the barriers below)
+  __shared__ int64_t sh_kept[kWave];  // kept events per segment (at most 64 per chunk)
   const int c = blockIdx.x;
   const ChunkDesc ch = P.chunks[c];
   const InstanceDesc in = P.inst[ch.instance];
@@ -838,7 +839,7 @@ __global__ void __launch_bounds__(1024) sample_kernel(BatchedParams P) {
                               : coef_f32_view(T.coef, nb).thr;
     for (int sgi = next_segment(&sh_next); sgi < P.segs; sgi = next_segment(&sh_next))
       sample_segment<Real, WINDOW>(P, c, ch, in, sgi, reinterpret_cast<const uint32_t*>(lds),
-                                   T.srt, T.coef, thr, T.cdf, (sh.flags & kSafe) != 0);
+                                   T.srt, T.coef, thr, T.cdf, (sh.flags & kSafe) != 0, sh_kept);
   } else {
     build_tables<Real, SEQUENTIAL>(P, ch.instance, in, lds, ch.index == 0, sh);
     const LdsTables<Real> L = lds_tables_of<Real>(lds, in);
@@ -847,48 +848,65 @@ __global__ void __launch_bounds__(1024) sample_kernel(BatchedParams P) {
                               : coef_f32_view(L.coef, nb).thr;
     for (int sgi = next_segment(&sh_next); sgi < P.segs; sgi = next_segment(&sh_next))
       sample_segment<Real, WINDOW>(P, c, ch, in, sgi, L.guide, L.srt, L.coef, thr, L.cdf,
-                                   (sh.flags & kSafe) != 0);
+                                   (sh.flags & kSafe) != 0, sh_kept);
+  }
+  // rows of the chunk's segments relative to the chunk, and the chunk's total
+  __syncthreads();
+  if (threadIdx.x < kWave) {
+    const int lane = lane_id();
+    const int64_t v = lane < P.segs ? sh_kept[lane] : 0;
+    int64_t inc = v;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+      const uint32_t lo = __shfl_up(uint32_t(inc), d, kWave);
+      const uint32_t hi = __shfl_up(uint32_t(uint64_t(inc) >> 32), d, kWave);
+      if (lane >= d) inc += int64_t(uint64_t(lo) | (uint64_t(hi) << 32));
+    }
+    if (lane < P.segs) P.seg_row[int64_t(c) * P.segs + lane] = inc - v;
+    if (lane == kWave - 1) P.chunk_kept[c] = inc;
   }
 }
 
 // ---------------------------------------------------------------------------
-// Finalize: one work-group.  seg_row = exclusive scan of seg_kept (four
-// segments per thread per tile); kept per instance; total; dense flag (every
-// segment kept all of its slots).
+// Finalize: one work-group.  chunk_row = exclusive scan of chunk_kept (four
+// chunks per thread per tile); kept per instance; total; dense flag (every
+// chunk kept all of its slots).
 __global__ void __launch_bounds__(1024) finalize_kernel(BatchedParams P) {
   __shared__ int64_t scratch[16];
   __shared__ int sh_sparse;
   const int tid = threadIdx.x, nt = blockDim.x;
   if (tid == 0) sh_sparse = 0;
-  const int64_t nseg = P.n_segments;
+  const int64_t nch = P.n_chunks;
   int64_t carry = 0;
   bool sparse = false;
-  for (int64_t base = 0; base < nseg; base += 4 * int64_t(nt)) {
-    const int64_t s0 = base + 4 * int64_t(tid);
+  for (int64_t base = 0; base < nch; base += 4 * int64_t(nt)) {
+    const int64_t c0 = base + 4 * int64_t(tid);
     int64_t v[4], local = 0;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      const int64_t sgi = s0 + q;
-      v[q] = sgi < nseg ? P.seg_kept[sgi] : 0;
-      sparse = sparse || (sgi < nseg && v[q] != P.seg_cap[sgi]);
+      const int64_t ci = c0 + q;
+      v[q] = ci < nch ? P.chunk_kept[ci] : 0;
+      if (ci < nch) {
+        const ChunkDesc ch = P.chunks[ci];
+        sparse = sparse || v[q] != ch.slot_hi - ch.slot_lo;
+      }
       local += v[q];
     }
     int64_t tot;
     int64_t run = carry + block_exclusive_scan(local, scratch, &tot);
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      if (s0 + q < nseg) P.seg_row[s0 + q] = run;
+      if (c0 + q < nch) P.chunk_row[c0 + q] = run;
       run += v[q];
     }
     carry += tot;
   }
   if (sparse) atomicOr(&sh_sparse, 1);
-  __syncthreads();  // seg_row written above is visible to the whole work-group
+  __syncthreads();  // chunk_row written above is visible to the whole work-group
   for (int64_t i = tid; i < P.n_instances; i += nt) {
     const InstanceDesc in = P.inst[i];
-    const int64_t s0 = int64_t(in.first_chunk) * P.segs;
-    const int64_t s1 = s0 + int64_t(in.n_chunks) * P.segs;
-    P.kept[i] = (s1 < nseg ? P.seg_row[s1] : carry) - P.seg_row[s0];
+    const int64_t c1 = int64_t(in.first_chunk) + in.n_chunks;
+    P.kept[i] = (c1 < nch ? P.chunk_row[c1] : carry) - P.chunk_row[in.first_chunk];
   }
   if (tid == 0) {
     P.total[0] = carry;
@@ -908,7 +926,8 @@ __global__ void __launch_bounds__(1024) pack_kernel(BatchedParams P, Real* packe
     const Segment sg = segment_of(in.counter_base, ch.slot_lo, ch.slot_hi, w, P.segs);
     const int64_t seg = int64_t(blockIdx.x) * P.segs + w;
     const int64_t cnt = P.seg_kept[seg];
-    const int64_t src = in.out_off + int64_t(sg.p_lo - in.counter_base), dst = P.seg_row[seg];
+    const int64_t src = in.out_off + int64_t(sg.p_lo - in.counter_base);
+    const int64_t dst = P.chunk_row[blockIdx.x] + P.seg_row[seg];
     const Real* ev = static_cast<const Real*>(P.events);
     for (int64_t r = lane_id(); r < cnt; r += 64) {
       packed[dst + r] = ev[src + r];
@@ -1101,7 +1120,7 @@ __device__ __forceinline__ void vjp_body(const BatchedParams& P, const Real* gra
     for (int sw = tid >> 6; sw < nw; sw += nt >> 6) {
       const int64_t sseg = int64_t(c) * nw + sw;
       const int64_t kept = P.seg_kept[sseg];
-      const int64_t srow0 = P.seg_row[sseg];
+      const int64_t srow0 = P.chunk_row[c] + P.seg_row[sseg];
       uint32_t m = 0u;
       for (int64_t r0 = 0; r0 < kept && m == 0u; r0 += 256) {
         uint32_t lm = 0u;
@@ -1141,7 +1160,7 @@ __device__ __forceinline__ void vjp_body(const BatchedParams& P, const Real* gra
     const Segment sg = segment_of(in.counter_base, ch.slot_lo, ch.slot_hi, w, nw);
     const int64_t seg = int64_t(c) * nw + w;
     const int first_drop = P.seg_drop[seg];
-    const int64_t row0 = P.seg_row[seg];
+    const int64_t row0 = P.chunk_row[c] + P.seg_row[seg];
     const int64_t per_wave = int64_t(blocks_per_wave(uint64_t(chunk_blocks(in, ch)), nw) / 64);
     const uint64_t* mask = P.keepmask + ch.mask_off + int64_t(w) * per_wave * 4;
     const uint32_t nblk = uint32_t(sg.qb - sg.qa);
@@ -1470,7 +1489,7 @@ __device__ void vjp_range_body(const BatchedParams& P, const Real* grad, Real* g
     for (int sw = tid >> 6; sw < nw; sw += nt >> 6) {
       const int64_t sseg = int64_t(c) * nw + sw;
       const int64_t kept = P.seg_kept[sseg];
-      const int64_t srow0 = P.seg_row[sseg];
+      const int64_t srow0 = P.chunk_row[c] + P.seg_row[sseg];
       uint32_t m = 0u;
       for (int64_t r0 = 0; r0 < kept && m == 0u; r0 += 256) {
         uint32_t lm = 0u;
@@ -1515,7 +1534,7 @@ __device__ void vjp_range_body(const BatchedParams& P, const Real* grad, Real* g
     const Segment sg = segment_of(in.counter_base, ch.slot_lo, ch.slot_hi, w, nw);
     const int64_t seg = int64_t(c) * nw + w;
     const int first_drop = P.seg_drop[seg];
-    const int64_t row0 = P.seg_row[seg];
+    const int64_t row0 = P.chunk_row[c] + P.seg_row[seg];
     const int64_t per_wave = int64_t(blocks_per_wave(uint64_t(chunk_blocks(in, ch)), nw) / 64);
     const uint64_t* mask = P.keepmask + ch.mask_off + int64_t(w) * per_wave * 4;
     const uint32_t nblk = uint32_t(sg.qb - sg.qa);

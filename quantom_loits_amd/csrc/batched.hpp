@@ -33,8 +33,9 @@ struct BatchedParams {
   // scratch
   int32_t* flags;        // [0] = dense
   int64_t* seg_kept;     // [n_segments]
-  int64_t* seg_cap;      // [n_segments]
-  int64_t* seg_row;      // [n_segments] first packed row
+  int64_t* chunk_kept;   // [n_chunks] kept events of each chunk
+  int64_t* chunk_row;    // [n_chunks] first packed row of each chunk
+  int64_t* seg_row;      // [n_segments] first packed row within its chunk
   int32_t* seg_drop;     // [n_segments] first iteration with a dropped slot
   uint32_t* done;        // [n_instances]
   uint64_t* keepmask;    // per-segment keep bits

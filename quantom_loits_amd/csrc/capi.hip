@@ -233,7 +233,8 @@ int make_plan(const loits_hip_instance* instances, int64_t n, int dtype,
     return fail(LOITS_HIP_EUNSUPPORTED, "batched: n_nodes / threads must be <= 32");
   L.mask_words = mask_words;
   L.partial_doubles = partial_doubles;
-  // scratch: flags | seg_kept | seg_cap | seg_row | seg_drop | done | keepmask | partials
+  // scratch: flags | seg_kept | chunk_kept, chunk_row (in n_segments words) | seg_row |
+  // seg_drop | done | keepmask | partials
   L.scratch_bytes = 64 + align16(8 * L.n_segments) * 3 + align16(4 * L.n_segments) + align16(4 * n) +
                     align16(8 * mask_words) + align16(8 * partial_doubles);
   L.grid_forward = int32_t(L.n_chunks);
@@ -277,7 +278,8 @@ BatchedParams params_of(const loits_hip_batched_args* a) {
   s += 64;
   P.seg_kept = reinterpret_cast<int64_t*>(s);
   s += align16(8 * L.n_segments);
-  P.seg_cap = reinterpret_cast<int64_t*>(s);
+  P.chunk_kept = reinterpret_cast<int64_t*>(s);  // n_segments >= 2 n_chunks (>= 16 per chunk)
+  P.chunk_row = P.chunk_kept + L.n_chunks;
   s += align16(8 * L.n_segments);
   P.seg_row = reinterpret_cast<int64_t*>(s);
   s += align16(8 * L.n_segments);
