@@ -1122,17 +1122,10 @@ __device__ __forceinline__ void vjp_body(const BatchedParams& P, const Real* gra
       const int64_t kept = P.seg_kept[sseg];
       const int64_t srow0 = P.chunk_row[c] + P.seg_row[sseg];
       uint32_t m = 0u;
-      for (int64_t r0 = 0; r0 < kept && m == 0u; r0 += 256) {
+      for (int64_t r0 = 0; r0 < kept && m == 0u; r0 += kWave) {  // one row per lane
         uint32_t lm = 0u;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const int64_t r = r0 + 4 * lane + q;
-          if (r < kept) {
-            const uint32_t bits =
-                __builtin_bit_cast(uint32_t, float(grad[srow0 + r])) & 0x7FFFFFFFu;
-            lm = bits > lm ? bits : lm;
-          }
-        }
+        const int64_t r = r0 + lane;
+        if (r < kept) lm = __builtin_bit_cast(uint32_t, float(grad[srow0 + r])) & 0x7FFFFFFFu;
 #pragma unroll
         for (int d = 32; d >= 1; d >>= 1) {
           const uint32_t o = uint32_t(__shfl_xor(int(lm), d, kWave));
@@ -1491,17 +1484,10 @@ __device__ void vjp_range_body(const BatchedParams& P, const Real* grad, Real* g
       const int64_t kept = P.seg_kept[sseg];
       const int64_t srow0 = P.chunk_row[c] + P.seg_row[sseg];
       uint32_t m = 0u;
-      for (int64_t r0 = 0; r0 < kept && m == 0u; r0 += 256) {
+      for (int64_t r0 = 0; r0 < kept && m == 0u; r0 += kWave) {  // one row per lane
         uint32_t lm = 0u;
-#pragma unroll
-        for (int qq = 0; qq < 4; ++qq) {
-          const int64_t r = r0 + 4 * lane + qq;
-          if (r < kept) {
-            const uint32_t bits =
-                __builtin_bit_cast(uint32_t, float(grad[srow0 + r])) & 0x7FFFFFFFu;
-            lm = bits > lm ? bits : lm;
-          }
-        }
+        const int64_t r = r0 + lane;
+        if (r < kept) lm = __builtin_bit_cast(uint32_t, float(grad[srow0 + r])) & 0x7FFFFFFFu;
 #pragma unroll
         for (int d = 32; d >= 1; d >>= 1) {
           const uint32_t o = uint32_t(__shfl_xor(int(lm), d, kWave));
