@@ -115,11 +115,30 @@ __device__ __forceinline__ Tables<Real> tables_of(uint8_t* base, const InstanceD
   return t;
 }
 
-// 16-byte copies by the whole work-group (LDS <-> global).
+// 16-byte copies by the whole work-group (LDS <-> global), four loads per
+// thread in flight before their stores (an index past the end reloads the
+// last unit and stores nothing).
 __device__ __forceinline__ void lds_copy(void* dst, const void* src, int64_t bytes) {
   const uint4* s = static_cast<const uint4*>(src);
   uint4* d = static_cast<uint4*>(dst);
-  for (int64_t i = threadIdx.x; i < bytes / 16; i += blockDim.x) d[i] = s[i];
+  const int64_t units = bytes / 16, nt = blockDim.x;
+  for (int64_t i = threadIdx.x; i < units; i += 4 * nt) {
+    uint4 v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int64_t u = i + k * nt;
+      v[k] = s[u < units ? u : units - 1];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (i + k * nt < units) d[i + k * nt] = v[k];
+  }
+}
+
+// Bytes of a built guide in use: the G cell words and the level-2 words.
+template <typename Real>
+__device__ __forceinline__ int64_t guide_bytes(const Tables<Real>& T, const InstanceDesc& in) {
+  return pad4((int64_t(1) << in.guide_log2) + int64_t(T.hdr->level2_words)) * 4;
 }
 
 // ---------------------------------------------------------------------------
@@ -492,13 +511,15 @@ __device__ void build_tables(const BatchedParams& P, int inst, const InstanceDes
   }
   __syncthreads();
 
-  // 7. publish
+  // 7. publish: the guide words in use; the per-bin coefficients only where
+  //    the VJP reads them (the fixed-point VJP of a monotone float32 curve
+  //    derives its per-bin data from srt and cdf)
   if (writer) {
     Tables<Real> T = tables_of<Real>(P.tables, in);
     if constexpr (!GLOBAL) {
       lds_copy(T.srt, L.srt, pad4(n) * 4);
-      lds_copy(T.guide, L.guide, 2 * G * 4);
-      lds_copy(T.coef, L.coef, pad4(n) * int64_t(sizeof(Coef)));
+      lds_copy(T.guide, L.guide, pad4(G + used) * 4);
+      if (!(sizeof(Real) == 4 && mono)) lds_copy(T.coef, L.coef, pad4(n) * int64_t(sizeof(Coef)));
       lds_copy(T.cdf, L.cdf, pad4(n) * 8);
     }
     if (P.cdf_out)
@@ -509,6 +530,7 @@ __device__ void build_tables(const BatchedParams& P, int inst, const InstanceDes
       T.hdr->flags = sh.flags;
       T.hdr->n = static_cast<uint32_t>(n);
       T.hdr->guide_log2 = static_cast<uint32_t>(in.guide_log2);
+      T.hdr->level2_words = static_cast<uint32_t>(used);
       if (P.norm_out) P.norm_out[inst] = sh.norm;
       P.done[inst] = 0u;
     }
@@ -831,7 +853,7 @@ __global__ void __launch_bounds__(1024) sample_kernel(BatchedParams P) {
   if (threadIdx.x == 0) sh_next = 0;
   if (in.global_tables) {  // built beforehand by tables_kernel; stage the guide
     const Tables<Real> T = tables_of<Real>(P.tables, in);
-    lds_copy(lds, T.guide, (int64_t(2) << in.guide_log2) * 4);
+    lds_copy(lds, T.guide, guide_bytes(T, in));
     if (threadIdx.x == 0) sh.flags = T.hdr->flags;
     __syncthreads();
     const uint32_t* thr = (sh.flags & kMonotone) != 0 || sizeof(Real) != 4
@@ -1014,11 +1036,78 @@ __device__ void vjp_tail(const BatchedParams& P, const InstanceDesc& in, const T
 // monotone curve (tau_j = 1 except for the last bin, which also takes the keys
 // above T_{n-1}); T_j is the sorted threshold array itself.
 
-__device__ __forceinline__ double vjp_tau_d(const double* cdf, int64_t j, int64_t n) {
-  const double d = cdf[j + 1] - cdf[j] + kEpsilon;
-  if (j != n - 2) return d;
-  const double tail = (1.0 - cdf[j]) / d;
+// |g| bits of cotangent row r of a segment (0 past its kept rows): bit
+// patterns order |g|, NaN above inf.
+template <typename Real>
+__device__ __forceinline__ uint32_t grad_bits(const Real* grad, int64_t kept, int64_t srow0,
+                                              int64_t r) {
+  return r < kept ? __builtin_bit_cast(uint32_t, float(grad[srow0 + r])) & 0x7FFFFFFFu : 0u;
+}
+
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) {
+    const uint32_t o = uint32_t(__shfl_xor(int(v), d, kWave));
+    v = o > v ? o : v;
+  }
+  return v;
+}
+
+// The fixed-point scale's sample of four of a wave's segments (s0, s0 + nwv,
+// s0 + 2 nwv, s0 + 3 nwv): loaded before the work-group's set-up so that its
+// descriptor and first-row loads overlap the set-up's copies.
+struct ScaleProbe {
+  int64_t kept[4], srow0[4];
+  uint32_t bits[4];
+
+  template <typename Real>
+  __device__ __forceinline__ void load(const BatchedParams& P, const Real* grad, int c, int s0,
+                                       int nwv, int lane) {
+    const int nw = P.segs;
+    const int64_t crow = P.chunk_row[c];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int sw = s0 + q * nwv;
+      const int64_t sseg = int64_t(c) * nw + sw;
+      kept[q] = sw < nw ? P.seg_kept[sseg] : 0;
+      srow0[q] = sw < nw ? crow + P.seg_row[sseg] : 0;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) bits[q] = grad_bits(grad, kept[q], srow0[q], lane);
+  }
+
+  // Largest |g| bits over the four segments, each from its first 64-row block
+  // holding a non-zero cotangent.
+  template <typename Real>
+  __device__ __forceinline__ uint32_t max(const Real* grad, int lane) const {
+    uint32_t m_all = 0u;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      uint32_t m = wave_max_u32(bits[q]);
+      for (int64_t r0 = kWave; r0 < kept[q] && m == 0u; r0 += kWave)
+        m = wave_max_u32(grad_bits(grad, kept[q], srow0[q], r0 + lane));
+      m_all = m > m_all ? m : m_all;
+    }
+    return m_all;
+  }
+};
+
+__device__ __forceinline__ double vjp_tau_d_of(double cj, double cj1, bool last) {
+  const double d = cj1 - cj + kEpsilon;
+  if (!last) return d;
+  const double tail = (1.0 - cj) / d;
   return (tail > 1.0 ? tail : 1.0) * d;
+}
+
+__device__ __forceinline__ double vjp_tau_d(const double* cdf, int64_t j, int64_t n) {
+  return vjp_tau_d_of(cdf[j], cdf[j + 1], j == n - 2);
+}
+
+// {alpha_j, beta_j} from C_j, C_{j+1} and T_j.
+__device__ __forceinline__ float2 vjp_alpha_beta(double cj, double cj1, uint32_t tj, bool last) {
+  const double tau_d = vjp_tau_d_of(cj, cj1, last);
+  const double r = double(tj) * 0x1.0p-24 - cj;
+  return make_float2(static_cast<float>(0x1.0p-24 / tau_d), static_cast<float>(r / tau_d));
 }
 
 // VJP: one work-group per chunk, one segment per wave; the chunk that
@@ -1061,6 +1150,9 @@ __device__ __forceinline__ void vjp_body(const BatchedParams& P, const Real* gra
   const int shift = 24 - in.guide_log2;
   const int tid = threadIdx.x, nt = blockDim.x;
   const bool fixed = !GLOBAL && sizeof(Real) == 4 && (T.hdr->flags & kMonotone) != 0;
+  const int probe_wave = __builtin_amdgcn_readfirstlane(tid >> 6), probe_nwv = nt >> 6;
+  ScaleProbe probe;
+  if (fixed) probe.load(P, grad, c, probe_wave, probe_nwv, lane_id());
 
   // LDS: srt | guide | per-bin data (coef, or {alpha, beta} when fixed) | h0 | h1
   // (GLOBAL: guide only)
@@ -1071,13 +1163,13 @@ __device__ __forceinline__ void vjp_body(const BatchedParams& P, const Real* gra
   const uint32_t* s_srt;
   const uint32_t* s_guide;
   if constexpr (GLOBAL) {
-    lds_copy(lds, T.guide, (int64_t(2) << in.guide_log2) * 4);
+    lds_copy(lds, T.guide, guide_bytes(T, in));
     s_bin = reinterpret_cast<uint8_t*>(T.coef);
     h0 = outl;
     s_srt = T.srt;
     s_guide = reinterpret_cast<const uint32_t*>(lds);
   } else {
-    lds_copy(lds, T.srt, T.L.coef - T.L.thr);
+    lds_copy(lds, T.srt, (T.L.guide - T.L.thr) + guide_bytes(T, in));  // srt | guide in use
     s_bin = lds + (T.L.coef - T.L.thr);
     h0 = reinterpret_cast<double*>(s_bin + nb * int64_t(sizeof(Coef)));
     s_srt = reinterpret_cast<const uint32_t*>(lds);
@@ -1086,12 +1178,24 @@ __device__ __forceinline__ void vjp_body(const BatchedParams& P, const Real* gra
   double* h1 = h0 + nb;
   uint64_t* a0 = reinterpret_cast<uint64_t*>(h0);
   uint64_t* a1 = reinterpret_cast<uint64_t*>(h1);
+  constexpr int kDepth = MAXPER < 4 ? MAXPER : 4;  // bins per thread with loads in flight
   if (fixed) {
     float2* vb = reinterpret_cast<float2*>(s_bin);
-    for (int64_t j = tid; j < n - 1; j += nt) {
-      const double tau_d = vjp_tau_d(T.cdf, j, n);
-      const double r = double(T.srt[j]) * 0x1.0p-24 - T.cdf[j];
-      vb[j] = make_float2(static_cast<float>(0x1.0p-24 / tau_d), static_cast<float>(r / tau_d));
+    for (int64_t j0 = tid; j0 < n - 1; j0 += kDepth * nt) {
+      double cj[kDepth], cj1[kDepth];
+      uint32_t tj[kDepth];
+#pragma unroll
+      for (int k = 0; k < kDepth; ++k) {
+        const int64_t j = j0 + k * nt, jj = j < n - 1 ? j : 0;
+        cj[k] = T.cdf[jj];
+        cj1[k] = T.cdf[jj + 1];
+        tj[k] = T.srt[jj];
+      }
+#pragma unroll
+      for (int k = 0; k < kDepth; ++k) {
+        const int64_t j = j0 + k * nt;
+        if (j < n - 1) vb[j] = vjp_alpha_beta(cj[k], cj1[k], tj[k], j == n - 2);
+      }
     }
   } else if (!GLOBAL) {
     lds_copy(s_bin, T.coef, nb * int64_t(sizeof(Coef)));
@@ -1116,23 +1220,10 @@ __device__ __forceinline__ void vjp_body(const BatchedParams& P, const Real* gra
   float scale = 0.0f;
   int scale_log2 = 0;
   if (fixed) {
-    uint32_t m_all = 0u;
-    for (int sw = tid >> 6; sw < nw; sw += nt >> 6) {
-      const int64_t sseg = int64_t(c) * nw + sw;
-      const int64_t kept = P.seg_kept[sseg];
-      const int64_t srow0 = P.chunk_row[c] + P.seg_row[sseg];
-      uint32_t m = 0u;
-      for (int64_t r0 = 0; r0 < kept && m == 0u; r0 += kWave) {  // one row per lane
-        uint32_t lm = 0u;
-        const int64_t r = r0 + lane;
-        if (r < kept) lm = __builtin_bit_cast(uint32_t, float(grad[srow0 + r])) & 0x7FFFFFFFu;
-#pragma unroll
-        for (int d = 32; d >= 1; d >>= 1) {
-          const uint32_t o = uint32_t(__shfl_xor(int(lm), d, kWave));
-          lm = o > lm ? o : lm;
-        }
-        m = lm;
-      }
+    uint32_t m_all = probe.max(grad, lane);
+    for (int s0 = probe_wave + 4 * probe_nwv; s0 < nw; s0 += 4 * probe_nwv) {
+      probe.load(P, grad, c, s0, probe_nwv, lane);  // more than four segments per wave
+      const uint32_t m = probe.max(grad, lane);
       m_all = m > m_all ? m : m_all;
     }
     if (lane == 0 && m_all != 0u) atomicMax(&sh_gmax, m_all);
@@ -1298,17 +1389,28 @@ __device__ __forceinline__ void vjp_body(const BatchedParams& P, const Real* gra
   if (fixed) {  // integers (and overflow sums) -> h0 = sum g, h1 = sum g t, in place
     const double inv_scale = __builtin_ldexp(1.0, -scale_log2);
     const bool any_outl = sh_outl != 0;
-    for (int64_t i = tid; i < nb; i += nt) {
-      double v0 = 0.0, v1 = 0.0;
-      if (i < n - 1) {
-        v0 = double(static_cast<int64_t>(a0[i])) * inv_scale +
-             (any_outl ? load_l2(outl + i) : 0.0);
-        v1 = (double(static_cast<int64_t>(a1[i])) * inv_scale +
-              (any_outl ? load_l2(outl + nb + i) : 0.0)) *
-             vjp_tau_d(T.cdf, i, n);
+    for (int64_t i0 = tid; i0 < nb; i0 += kDepth * nt) {
+      double td[kDepth];
+#pragma unroll
+      for (int k = 0; k < kDepth; ++k) {
+        const int64_t i = i0 + k * nt, ii = i < n - 1 ? i : 0;
+        td[k] = vjp_tau_d_of(T.cdf[ii], T.cdf[ii + 1], i == n - 2);
       }
-      h0[i] = v0;
-      h1[i] = v1;
+#pragma unroll
+      for (int k = 0; k < kDepth; ++k) {
+        const int64_t i = i0 + k * nt;
+        if (i >= nb) continue;
+        double v0 = 0.0, v1 = 0.0;
+        if (i < n - 1) {
+          v0 = double(static_cast<int64_t>(a0[i])) * inv_scale +
+               (any_outl ? load_l2(outl + i) : 0.0);
+          v1 = (double(static_cast<int64_t>(a1[i])) * inv_scale +
+                (any_outl ? load_l2(outl + nb + i) : 0.0)) *
+               td[k];
+        }
+        h0[i] = v0;
+        h1[i] = v1;
+      }
     }
     __syncthreads();
   }
@@ -1400,6 +1502,9 @@ __device__ void vjp_range_body(const BatchedParams& P, const Real* grad, Real* g
   const int tid = threadIdx.x, nt = blockDim.x;
   const int64_t nb = pad4(n);
   double* outl = P.partials + ch.partial_off;  // [2 nb]: overflow sums, then partial sums
+  const int probe_wave = __builtin_amdgcn_readfirstlane(tid >> 6), probe_nwv = nt >> 6;
+  ScaleProbe probe;
+  if constexpr (fixed) probe.load(P, grad, c, probe_wave, probe_nwv, lane_id());
 
   const uint32_t k_lo = lo == 0 ? 0u : T.srt[lo];
   const uint32_t k_hi = hi == n - 1 ? (1u << 24) : T.srt[hi];
@@ -1478,23 +1583,10 @@ __device__ void vjp_range_body(const BatchedParams& P, const Real* grad, Real* g
   float scale = 0.0f;
   int scale_log2 = 0;
   if constexpr (fixed) {
-    uint32_t m_all = 0u;
-    for (int sw = tid >> 6; sw < nw; sw += nt >> 6) {
-      const int64_t sseg = int64_t(c) * nw + sw;
-      const int64_t kept = P.seg_kept[sseg];
-      const int64_t srow0 = P.chunk_row[c] + P.seg_row[sseg];
-      uint32_t m = 0u;
-      for (int64_t r0 = 0; r0 < kept && m == 0u; r0 += kWave) {  // one row per lane
-        uint32_t lm = 0u;
-        const int64_t r = r0 + lane;
-        if (r < kept) lm = __builtin_bit_cast(uint32_t, float(grad[srow0 + r])) & 0x7FFFFFFFu;
-#pragma unroll
-        for (int d = 32; d >= 1; d >>= 1) {
-          const uint32_t o = uint32_t(__shfl_xor(int(lm), d, kWave));
-          lm = o > lm ? o : lm;
-        }
-        m = lm;
-      }
+    uint32_t m_all = probe.max(grad, lane);
+    for (int s0 = probe_wave + 4 * probe_nwv; s0 < nw; s0 += 4 * probe_nwv) {
+      probe.load(P, grad, c, s0, probe_nwv, lane);  // more than four segments per wave
+      const uint32_t m = probe.max(grad, lane);
       m_all = m > m_all ? m : m_all;
     }
     if (lane == 0 && m_all != 0u) atomicMax(&sh_gmax, m_all);

@@ -275,7 +275,7 @@ struct TableHeader {
   uint32_t flags;   // kMonotone | kSafe
   uint32_t n;
   uint32_t guide_log2;
-  uint32_t pad;
+  uint32_t level2_words;  // guide words [G, G + level2_words) are in use
   double reserved[4];
 };
 static_assert(sizeof(TableHeader) == 64, "header size");

@@ -231,6 +231,64 @@ def test_batched_ragged_window_matches_oracle(oracle_port, hip_device):
             assert err <= (1e-10 if dtype == torch.float64 else 1e-4), (i, err)
 
 
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_batched_tiny_curves_and_event_counts_match_oracle(oracle_port, hip_device, dtype):
+    """Two- and three-node curves (one and two bins) and event counts at the
+    edges of the Philox blocks, segments and chunks: none, one, 63/64/65 slots
+    (a partial 4-word block, one wave of blocks, one over), several chunks."""
+    from quantom_loits_amd.batched import BatchedLOITS, Tuning
+
+    n_nodes = np.array([2, 3, 2, 5, 3, 64, 2, 17], np.int64)
+    events = np.array([0, 1, 3, 63, 64, 65, 4097, 2500], np.int64)
+    rng = np.random.default_rng(21)
+    xs, ds = [], []
+    for n in n_nodes:
+        xs.append(np.sort(rng.uniform(0.1, 3.0, n)))  # x > 0: relative value errors
+        ds.append(rng.uniform(0.1, 2.0, n))
+    x64, d64 = np.concatenate(xs), np.concatenate(ds)
+    if dtype == torch.float32:
+        x64, d64 = x64.astype(np.float32).astype(np.float64), d64.astype(np.float32).astype(np.float64)
+    node_off = np.concatenate([[0], np.cumsum(n_nodes)[:-1]]).astype(np.int64)
+    base = rng.integers(0, 2 ** 40, len(n_nodes)).astype(np.uint64)
+    streams = rng.integers(0, 2 ** 33, len(n_nodes)).astype(np.uint64)
+    accept = np.ones(len(n_nodes), np.uint8)
+    ref_v, ref_b, ref_k, _, _ = oracle_port.batched_forward(x64, d64, node_off, n_nodes, events,
+                                                            base, streams, 7, accept=accept)
+    assert ref_k.tolist() == events.tolist()
+    print("tiny curves", dtype, flush=True)
+    s = BatchedLOITS(n_nodes, events, base, streams, accept, node_off, dtype=dtype,
+                     tuning=Tuning(threads=256, chunk_events=1024, scan_mode=1))
+    x = torch.from_numpy(x64).to(dtype).cuda()
+    d = torch.from_numpy(d64).to(dtype).cuda()
+    values, kept, st, ex = s.run_forward(x, d, 7, want_bins=True)
+    assert kept.cpu().numpy().tolist() == ref_k.tolist()
+    assert np.array_equal(ex["bins"].cpu().numpy(), ref_b)
+    v = values.cpu().numpy().astype(np.float64)
+    if dtype == torch.float64:
+        assert np.array_equal(v, ref_v)
+    else:
+        assert np.max(np.abs(v - ref_v) / np.abs(ref_v)) <= 1e-5
+    gv = np.random.default_rng(22).standard_normal(len(ref_v))
+    ref_g = oracle_port.batched_backward(x64, d64, node_off, n_nodes, events, base, streams, 7,
+                                         gv, accept=accept)
+    gd = torch.empty_like(d)
+    s.backward_raw(x, d, 7, st, torch.from_numpy(gv).to(dtype).cuda(), gd)
+    got = gd.double().cpu().numpy()
+    rows = np.concatenate([[0], np.cumsum(ref_k)])
+    for i in range(len(n_nodes)):
+        sl = slice(int(node_off[i]), int(node_off[i] + n_nodes[i]))
+        if events[i] == 0:  # no events: no gradient
+            assert np.abs(got[sl]).max() == 0, i
+            continue
+        # rounding floor: eps-sized terms |g| |x| summed over the events
+        floor = 1e-12 * np.abs(gv[rows[i]:rows[i + 1]]).sum() * np.abs(x64[sl]).max()
+        if n_nodes[i] == 2:  # one bin spanning C = 0..1: events do not depend on the density
+            assert np.abs(got[sl]).max() <= floor and np.abs(ref_g[sl]).max() <= floor, i
+            continue
+        err = np.abs(got[sl] - ref_g[sl]).max() / np.abs(ref_g[sl]).max()
+        assert err <= (1e-10 if dtype == torch.float64 else 1e-4), (i, err)
+
+
 def test_batched_large_curves_global_tables_match_oracle(oracle_port, hip_device):
     """Curves too large for LDS (global tables) mixed with LDS-resident ones,
     several chunks per curve, a window cut; both precisions."""
