@@ -289,6 +289,53 @@ def test_batched_tiny_curves_and_event_counts_match_oracle(oracle_port, hip_devi
         assert err <= (1e-10 if dtype == torch.float64 else 1e-4), (i, err)
 
 
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_batched_nonfinite_densities_match_oracle(oracle_port, hip_device, dtype):
+    """All-zero, infinite and NaN densities (norm 0, inf, NaN): no finite events
+    survive compaction and the density adjoint is NaN over the whole curve, as
+    in the reference's cores; a neighbouring curve with a negative node is
+    unaffected."""
+    from quantom_loits_amd.batched import BatchedLOITS, Tuning
+
+    n_nodes = np.array([5, 5, 5, 4], np.int64)
+    events = np.array([100, 100, 100, 50], np.int64)
+    x64 = np.concatenate([np.linspace(0.2, 1.0, 5)] * 3 + [np.linspace(0.2, 1.0, 4)])
+    d64 = np.concatenate([np.zeros(5), [1, 2, np.inf, 1, 1], [1, np.nan, 1, 1, 1], [1, -1, 1, 1]])
+    if dtype == torch.float32:
+        x64 = x64.astype(np.float32).astype(np.float64)
+    node_off = np.array([0, 5, 10, 15], np.int64)
+    base = np.zeros(4, np.uint64)
+    streams = np.arange(4, dtype=np.uint64)
+    accept = np.ones(4, np.uint8)
+    ref_v, ref_b, ref_k, _, _ = oracle_port.batched_forward(x64, d64, node_off, n_nodes, events,
+                                                            base, streams, 3, accept=accept)
+    assert ref_k.tolist() == [0, 0, 0, 50]
+    print("non-finite densities", dtype, flush=True)
+    s = BatchedLOITS(n_nodes, events, base, streams, accept, node_off, dtype=dtype,
+                     tuning=Tuning(threads=256, chunk_events=1024, scan_mode=1))
+    x = torch.from_numpy(x64).to(dtype).cuda()
+    d = torch.from_numpy(d64).to(dtype).cuda()
+    values, kept, st, ex = s.run_forward(x, d, 3, want_bins=True)
+    assert kept.cpu().numpy().tolist() == ref_k.tolist()
+    assert np.array_equal(ex["bins"].cpu().numpy(), ref_b)
+    v = values.cpu().numpy().astype(np.float64)
+    if dtype == torch.float64:
+        assert np.array_equal(v, ref_v)
+    else:
+        assert np.max(np.abs(v - ref_v) / np.abs(ref_v)) <= 1e-5
+    gv = np.linspace(-1.0, 1.0, len(ref_v))
+    ref_g = oracle_port.batched_backward(x64, d64, node_off, n_nodes, events, base, streams, 3,
+                                         gv, accept=accept)
+    gd = torch.empty_like(d)
+    s.backward_raw(x, d, 3, st, torch.from_numpy(gv).to(dtype).cuda(), gd)
+    got = gd.double().cpu().numpy()
+    assert np.array_equal(np.isnan(got), np.isnan(ref_g))
+    ok = ~np.isnan(ref_g)
+    scale = np.abs(ref_g[ok]).max()
+    tol = 1e-10 if dtype == torch.float64 else 1e-4
+    assert np.abs(got[ok] - ref_g[ok]).max() <= tol * scale
+
+
 def test_batched_large_curves_global_tables_match_oracle(oracle_port, hip_device):
     """Curves too large for LDS (global tables) mixed with LDS-resident ones,
     several chunks per curve, a window cut; both precisions."""
