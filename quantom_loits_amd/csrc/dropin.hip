@@ -13,8 +13,10 @@
 //                      (:239-266) in a single pass, tiles ordered by a
 //                      decoupled look-back
 //   row_offsets_kernel openmp/loits_core.cpp:244-262 from the packed slots
-//   ivjp_wave_kernel   curves of at most 64 nodes: one wave per cell, rows in
-//                      order (the accumulation order of
+//   ivjp_wave_kernel   curves of at most 64 nodes: one wave per cell, lane j
+//                      owning bin j, which walks only the rows that touch it
+//                      (per-bin row masks of each 64-row batch), in row order
+//                      (the accumulation order of
 //                      openmp/loits_core.cpp:285-352); ivjp_kernel, one
 //                      thread per cell, for larger curves
 //   curve_vjp_kernel   one thread per curve: CDF and density adjoints (:340-472)
@@ -521,15 +523,34 @@ __global__ void __launch_bounds__(256) ivjp_kernel(VjpParams P0) {
   }
 }
 
-struct RowF {
-  double upx, ax, bx, idx, upq, aq, bq, idq;
-  int32_t jx, jq;
-};
+// First index i in [0, n) with a[i] >= key (n if none) of an ascending array:
+// a 65-way search, one probe per lane per step (wave-uniform result).
+__device__ __forceinline__ int64_t wave_lower_bound(const int64_t* a, int64_t n, int64_t key) {
+  int64_t lo = 0, hi = n;  // the answer lies in [lo, hi]
+  const int lane = lane_id();
+  while (lo < hi) {
+    const int64_t pos = lo + ((hi - lo) * (lane + 1)) / (kWave + 1);  // in [lo, hi), ascending
+    const int c = popc64(ballot(a[pos] < key));  // the probes below key are lanes [0, c)
+    const int64_t below = c > 0 ? wave_lane(pos, c - 1) : lo - 1;
+    const int64_t above = c < kWave ? wave_lane(pos, c) : hi;
+    lo = below + 1;
+    hi = above;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ uint64_t shfl_up1_u64(uint64_t v) {
+  const uint32_t lo = uint32_t(__shfl_up(int(uint32_t(v)), 1, kWave));
+  const uint32_t hi = uint32_t(__shfl_up(int(uint32_t(v >> 32)), 1, kWave));
+  return lane_id() == 0 ? 0ull : (uint64_t(lo) | (uint64_t(hi) << 32));
+}
 
 template <typename Real>
 __global__ void __launch_bounds__(256) ivjp_wave_kernel(VjpParams P0) {
   const VjpParams P = with_device_sizes(P0);
-  __shared__ RowF rows[4 * kWave];
+  __shared__ unsigned long long masks[4 * 2 * kWave];
+  __shared__ int2 offsets[4 * kWave];
+  __shared__ double2 entries[4 * 4 * kWave];
   const loits_hip_shape& s = P.s;
   const int lane = lane_id();
   const int64_t g = int64_t(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
@@ -538,20 +559,17 @@ __global__ void __launch_bounds__(256) ivjp_wave_kernel(VjpParams P0) {
   const int64_t ix = cell / s.ny, iy = cell - ix * s.ny;
   const int64_t ox = ((b * s.ny + iy) * s.nx + ix) * s.kx, bx = (b * s.nx + ix) * s.kx;
   const int64_t oq = ((b * s.nx + ix) * s.ny + iy) * s.kq, bq = (b * s.ny + iy) * s.kq;
-  const int64_t key0 = g * P.nmax, key1 = key0 + P.nmax;
-  int64_t lo0 = 0, hi0 = P.nvalid, lo1 = 0, hi1 = P.nvalid;
-  while (lo0 < hi0 || lo1 < hi1) {  // rows of this cell: [lo0, lo1)
-    if (lo0 < hi0) {
-      const int64_t mid = (lo0 + hi0) >> 1;
-      if (P.packed[mid] < key0) lo0 = mid + 1; else hi0 = mid;
-    }
-    if (lo1 < hi1) {
-      const int64_t mid = (lo1 + hi1) >> 1;
-      if (P.packed[mid] < key1) lo1 = mid + 1; else hi1 = mid;
-    }
-  }
+  const int64_t key0 = g * P.nmax;
+  const int64_t lo0 = wave_lower_bound(P.packed, P.nvalid, key0);  // rows of this cell: [lo0, lo1)
+  const int64_t lo1 = wave_lower_bound(P.packed, P.nvalid, key0 + P.nmax);
   const double* cx = P.cdf_x + ox;
   const double* cq = P.cdf_q + oq;
+  const int w = threadIdx.x >> 6;
+  unsigned long long* mx = masks + w * 2 * kWave;  // rows with interval j, then touching bin j
+  unsigned long long* mq = mx + kWave;
+  int2* offs = offsets + w * kWave;                 // first list entry of bin j (x, q)
+  double2* ex = entries + w * 4 * kWave;            // factor pairs, bin by bin, rows in order
+  double2* eq = ex + 2 * kWave;
   double accx = 0.0, accq = 0.0;
   for (int64_t base = lo0; base < lo1; base += kWave) {
     const int nr = lo1 - base < kWave ? int(lo1 - base) : kWave;
@@ -580,17 +598,48 @@ __global__ void __launch_bounds__(256) ivjp_wave_kernel(VjpParams P0) {
       bqm = -(upq * t * inv_d);
       idq = inv_d;
     }
-    // this wave's rows in LDS; every lane then walks them in row order
-    RowF* rf = rows + (threadIdx.x >> 6) * kWave;
-    rf[lane] = RowF{upx, ax, bxm, idx, upq, aq, bqm, idq, jx, jq};
+    // Lane j holds bin j of x and of q.  Row r adds upx ax to bin jx and
+    // bx idx to bin jx + 1, so bin j takes the rows with interval j or j - 1,
+    // in row order: each row writes its two factor pairs into the lists of
+    // those bins (list position = earlier rows touching the bin), then every
+    // lane sums its own list.
+    mx[lane] = 0ull;
+    mq[lane] = 0ull;
     __builtin_amdgcn_wave_barrier();
-#pragma unroll 4
-    for (int r = 0; r < nr; ++r) {
-      const RowF f = rf[r];
-      if (lane == f.jx) accx = __builtin_fma(f.upx, f.ax, accx);
-      else if (lane == f.jx + 1) accx = __builtin_fma(f.bx, f.idx, accx);
-      if (lane == f.jq) accq = __builtin_fma(f.upq, f.aq, accq);
-      else if (lane == f.jq + 1) accq = __builtin_fma(f.bq, f.idq, accq);
+    if (lane < nr) {
+      atomicOr(&mx[jx], 1ull << lane);
+      atomicOr(&mq[jq], 1ull << lane);
+    }
+    __builtin_amdgcn_wave_barrier();
+    uint64_t tx = mx[lane], tq = mq[lane];
+    tx |= shfl_up1_u64(tx);  // rows touching bin j
+    tq |= shfl_up1_u64(tq);
+    const int len_x = popc64(tx), len_q = popc64(tq);
+    const int off_x = wave_inclusive_scan(len_x) - len_x;  // at most 2 nr entries in all
+    const int off_q = wave_inclusive_scan(len_q) - len_q;
+    __builtin_amdgcn_wave_barrier();
+    mx[lane] = tx;
+    mq[lane] = tq;
+    offs[lane] = make_int2(off_x, off_q);
+    __builtin_amdgcn_wave_barrier();
+    if (lane < nr) {
+      const uint64_t lt = lanemask_lt();
+      const int2 oa = offs[jx], ob = offs[jx + 1], qa = offs[jq], qb = offs[jq + 1];
+      ex[oa.x + popc64(mx[jx] & lt)] = make_double2(upx, ax);
+      ex[ob.x + popc64(mx[jx + 1] & lt)] = make_double2(bxm, idx);
+      eq[qa.y + popc64(mq[jq] & lt)] = make_double2(upq, aq);
+      eq[qb.y + popc64(mq[jq + 1] & lt)] = make_double2(bqm, idq);
+    }
+    __builtin_amdgcn_wave_barrier();
+    for (int i = 0; ballot(i < len_x || i < len_q) != 0ull; ++i) {
+      if (i < len_x) {
+        const double2 e = ex[off_x + i];
+        accx = __builtin_fma(e.x, e.y, accx);
+      }
+      if (i < len_q) {
+        const double2 e = eq[off_q + i];
+        accq = __builtin_fma(e.x, e.y, accq);
+      }
     }
     __builtin_amdgcn_wave_barrier();
   }
