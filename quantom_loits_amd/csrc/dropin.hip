@@ -36,7 +36,7 @@
 namespace qloits {
 namespace {
 
-constexpr int kTileThreads = 256;
+constexpr int kTileThreads = 1024;  // 4096 slots per tile: fewer look-back chains
 constexpr int64_t kTileSlots = 4 * kTileThreads;
 constexpr uint64_t kStatusShift = 62;
 constexpr uint64_t kAggregate = 1ull << kStatusShift;
@@ -197,6 +197,13 @@ __global__ void __launch_bounds__(256) curve_kernel(CurveParams P) {
 // it is written, with all thresholds loaded at once (one cache line or two).
 constexpr int kSmallK = 16;
 
+// interval = clamp(count - 1, 0, K - 2)
+__device__ __forceinline__ int64_t clamp_interval(int64_t count, int64_t K) {
+  int64_t j = count - 1;
+  j = j < 0 ? 0 : j;
+  return j > K - 2 ? K - 2 : j;
+}
+
 // count(k) = #{t : T_t <= k}; interval = clamp(count - 1, 0, K - 2)
 __device__ __forceinline__ int64_t interval_of(const uint32_t* thr, int64_t K, bool mono,
                                                uint32_t key) {
@@ -217,9 +224,7 @@ __device__ __forceinline__ int64_t interval_of(const uint32_t* thr, int64_t K, b
     count = 0;
     for (int64_t t = 0; t < K; ++t) count += thr[t] <= key ? 1 : 0;
   }
-  int64_t j = count - 1;
-  j = j < 0 ? 0 : j;
-  return j > K - 2 ? K - 2 : j;
+  return clamp_interval(count, K);
 }
 
 // ---------------------------------------------------------------------------
@@ -285,6 +290,27 @@ __global__ void __launch_bounds__(kTileThreads) sample_kernel(SampleParams P0) {
   bool mono_x = false, mono_q = false;
   float ufx[4], ufq[4];
   int16_t ix4[4] = {0, 0, 0, 0}, iq4[4] = {0, 0, 0, 0};
+  // Four active slots of one cell with small curves (the usual case): each
+  // threshold is loaded once for the four keys.
+  int cnt_x[4] = {-1, -1, -1, -1}, cnt_q[4] = {-1, -1, -1, -1};
+  if (s.kx <= kSmallK && s.kq <= kSmallK && p0 + 3 < P.total_slots && slot + 3 < P.nmax &&
+      slot + 3 < P.counts[g]) {
+    const int64_t b = g / s.cells, cell = g - b * s.cells;
+    const int64_t ix = cell / s.ny, iy = cell - ix * s.ny;
+    const uint32_t* tx = P.thr_x + ((b * s.ny + iy) * s.nx + ix) * s.kx;
+    const uint32_t* tq = P.thr_q + ((b * s.nx + ix) * s.ny + iy) * s.kq;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) cnt_x[e] = cnt_q[e] = 0;
+#pragma unroll
+    for (int t = 0; t < kSmallK; ++t) {
+      const uint32_t vx = t < s.kx ? tx[t] : kNever, vq = t < s.kq ? tq[t] : kNever;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        cnt_x[e] += vx <= (word_of(wx, e) >> 8) ? 1 : 0;
+        cnt_q[e] += vq <= (word_of(wq, e) >> 8) ? 1 : 0;
+      }
+    }
+  }
 #pragma unroll
   for (int e = 0; e < 4; ++e) {
     if (e > 0 && ++slot == P.nmax) {
@@ -312,11 +338,13 @@ __global__ void __launch_bounds__(kTileThreads) sample_kernel(SampleParams P0) {
     }
     if (slot >= count) continue;
     // x: cpp/loits_core.cpp:183-196
-    const int64_t jx = interval_of(P.thr_x + ox, s.kx, mono_x, kx);
+    const int64_t jx = cnt_x[e] >= 0 ? clamp_interval(cnt_x[e], s.kx)
+                                     : interval_of(P.thr_x + ox, s.kx, mono_x, kx);
     const double ux = double(kx) * 0x1.0p-24;
     vx[e] = __builtin_fma(P.m_x[ox + jx], ux - P.cdf_x[ox + jx], ld<Real>(P.x_bins, bx + jx));
     // q: cpp/loits_core.cpp:219-232
-    const int64_t jq = interval_of(P.thr_q + oq, s.kq, mono_q, kq);
+    const int64_t jq = cnt_q[e] >= 0 ? clamp_interval(cnt_q[e], s.kq)
+                                     : interval_of(P.thr_q + oq, s.kq, mono_q, kq);
     const double uq = double(kq) * 0x1.0p-24;
     vq[e] = __builtin_fma(P.m_q[oq + jq], uq - P.cdf_q[oq + jq], ld<Real>(P.q_bins, bq + jq));
     ix4[e] = int16_t(jx);
@@ -551,6 +579,7 @@ __global__ void __launch_bounds__(256) ivjp_wave_kernel(VjpParams P0) {
   __shared__ unsigned long long masks[4 * 2 * kWave];
   __shared__ int2 offsets[4 * kWave];
   __shared__ double2 entries[4 * 4 * kWave];
+  __shared__ double nodes[4 * 4 * kWave];  // the cell's C_x, x, C_q, q
   const loits_hip_shape& s = P.s;
   const int lane = lane_id();
   const int64_t g = int64_t(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
@@ -562,27 +591,57 @@ __global__ void __launch_bounds__(256) ivjp_wave_kernel(VjpParams P0) {
   const int64_t key0 = g * P.nmax;
   const int64_t lo0 = wave_lower_bound(P.packed, P.nvalid, key0);  // rows of this cell: [lo0, lo1)
   const int64_t lo1 = wave_lower_bound(P.packed, P.nvalid, key0 + P.nmax);
-  const double* cx = P.cdf_x + ox;
-  const double* cq = P.cdf_q + oq;
   const int w = threadIdx.x >> 6;
   unsigned long long* mx = masks + w * 2 * kWave;  // rows with interval j, then touching bin j
   unsigned long long* mq = mx + kWave;
   int2* offs = offsets + w * kWave;                 // first list entry of bin j (x, q)
   double2* ex = entries + w * 4 * kWave;            // factor pairs, bin by bin, rows in order
   double2* eq = ex + 2 * kWave;
+  double* cx = nodes + w * 4 * kWave;
+  double* xn = cx + kWave;
+  double* cq = xn + kWave;
+  double* qn = cq + kWave;
+  if (lane < s.kx) {
+    cx[lane] = P.cdf_x[ox + lane];
+    xn[lane] = ld<Real>(P.x_bins, bx + lane);
+  }
+  if (lane < s.kq) {
+    cq[lane] = P.cdf_q[oq + lane];
+    qn[lane] = ld<Real>(P.q_bins, bq + lane);
+  }
+  __builtin_amdgcn_wave_barrier();
+  // The packed slot and cotangents of a batch's rows are loaded one batch ahead.
+  auto row_of = [&](int64_t base) {
+    const int64_t n = lo1 - base;
+    return base + (lane < n ? lane : 0);
+  };
+  int64_t p = 0;
+  double gx = 0.0, gq = 0.0;
+  if (lo0 < lo1) {
+    const int64_t row = row_of(lo0);
+    p = P.packed[row];
+    gx = ld<Real>(P.grad, 2 * row);
+    gq = ld<Real>(P.grad, 2 * row + 1);
+  }
   double accx = 0.0, accq = 0.0;
   for (int64_t base = lo0; base < lo1; base += kWave) {
     const int nr = lo1 - base < kWave ? int(lo1 - base) : kWave;
-    const int64_t row = base + (lane < nr ? lane : 0);
-    const int64_t p = P.packed[row];
     const int jx = P.interval_x[p], jq = P.interval_q[p];
+    const double ux = double(P.u_x[p]), uq = double(P.u_q[p]);
+    const double gxb = gx, gqb = gq;
+    if (base + kWave < lo1) {
+      const int64_t row = row_of(base + kWave);
+      p = P.packed[row];
+      gx = ld<Real>(P.grad, 2 * row);
+      gq = ld<Real>(P.grad, 2 * row + 1);
+    }
     double upx, ax, bxm, idx, upq, aq, bqm, idq;
     {  // x: the factors of ivjp_row
       const double c0 = cx[jx];
       const double d = cx[jx + 1] - c0 + kEpsilon;
       const double inv_d = 1.0 / d;
-      const double t = double(P.u_x[p]) - c0;
-      upx = ld<Real>(P.grad, 2 * row) * (ld<Real>(P.x_bins, bx + jx + 1) - ld<Real>(P.x_bins, bx + jx));
+      const double t = ux - c0;
+      upx = gxb * (xn[jx + 1] - xn[jx]);
       ax = __builtin_fma(t * inv_d, inv_d, -inv_d);
       bxm = -(upx * t * inv_d);
       idx = inv_d;
@@ -591,9 +650,8 @@ __global__ void __launch_bounds__(256) ivjp_wave_kernel(VjpParams P0) {
       const double c0 = cq[jq];
       const double d = cq[jq + 1] - c0 + kEpsilon;
       const double inv_d = 1.0 / d;
-      const double t = double(P.u_q[p]) - c0;
-      upq = ld<Real>(P.grad, 2 * row + 1) *
-            (ld<Real>(P.q_bins, bq + jq + 1) - ld<Real>(P.q_bins, bq + jq));
+      const double t = uq - c0;
+      upq = gqb * (qn[jq + 1] - qn[jq]);
       aq = __builtin_fma(t * inv_d, inv_d, -inv_d);
       bqm = -(upq * t * inv_d);
       idq = inv_d;
@@ -631,7 +689,14 @@ __global__ void __launch_bounds__(256) ivjp_wave_kernel(VjpParams P0) {
       eq[qb.y + popc64(mq[jq + 1] & lt)] = make_double2(bqm, idq);
     }
     __builtin_amdgcn_wave_barrier();
-    for (int i = 0; ballot(i < len_x || i < len_q) != 0ull; ++i) {
+    int steps = len_x > len_q ? len_x : len_q;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+      const int o = __shfl_xor(steps, d, kWave);
+      steps = o > steps ? o : steps;
+    }
+#pragma unroll 2
+    for (int i = 0; i < steps; ++i) {
       if (i < len_x) {
         const double2 e = ex[off_x + i];
         accx = __builtin_fma(e.x, e.y, accx);
