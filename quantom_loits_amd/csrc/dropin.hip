@@ -567,6 +567,20 @@ __device__ __forceinline__ int64_t wave_lower_bound(const int64_t* a, int64_t n,
   return lo;
 }
 
+// Lane j receives the mask of the lanes r < nr whose v equals j (0 <= v < 64):
+// one ballot per distinct value.
+__device__ __forceinline__ uint64_t rows_with_value(int v, int nr) {
+  const int lane = lane_id();
+  uint64_t left = ballot(lane < nr), mine = 0ull;
+  while (left != 0ull) {
+    const int val = __builtin_amdgcn_readlane(v, __builtin_ctzll(left));
+    const uint64_t m = ballot(v == val) & left;
+    mine = lane == val ? m : mine;
+    left &= ~m;
+  }
+  return mine;
+}
+
 __device__ __forceinline__ uint64_t shfl_up1_u64(uint64_t v) {
   const uint32_t lo = uint32_t(__shfl_up(int(uint32_t(v)), 1, kWave));
   const uint32_t hi = uint32_t(__shfl_up(int(uint32_t(v >> 32)), 1, kWave));
@@ -661,15 +675,7 @@ __global__ void __launch_bounds__(256) ivjp_wave_kernel(VjpParams P0) {
     // in row order: each row writes its two factor pairs into the lists of
     // those bins (list position = earlier rows touching the bin), then every
     // lane sums its own list.
-    mx[lane] = 0ull;
-    mq[lane] = 0ull;
-    __builtin_amdgcn_wave_barrier();
-    if (lane < nr) {
-      atomicOr(&mx[jx], 1ull << lane);
-      atomicOr(&mq[jq], 1ull << lane);
-    }
-    __builtin_amdgcn_wave_barrier();
-    uint64_t tx = mx[lane], tq = mq[lane];
+    uint64_t tx = rows_with_value(jx, nr), tq = rows_with_value(jq, nr);  // interval == j
     tx |= shfl_up1_u64(tx);  // rows touching bin j
     tq |= shfl_up1_u64(tq);
     const int len_x = popc64(tx), len_q = popc64(tq);
@@ -695,21 +701,198 @@ __global__ void __launch_bounds__(256) ivjp_wave_kernel(VjpParams P0) {
       const int o = __shfl_xor(steps, d, kWave);
       steps = o > steps ? o : steps;
     }
-#pragma unroll 2
-    for (int i = 0; i < steps; ++i) {
-      if (i < len_x) {
-        const double2 e = ex[off_x + i];
-        accx = __builtin_fma(e.x, e.y, accx);
+    // four entries per list at a time, their reads issued together (a lane
+    // past the end of its list reads an entry in range and keeps its sum)
+    for (int i0 = 0; i0 < steps; i0 += 4) {
+      double2 fx[4], fq[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int kx_ = off_x + i0 + u, kq_ = off_q + i0 + u;
+        fx[u] = ex[kx_ < 2 * kWave ? kx_ : 2 * kWave - 1];
+        fq[u] = eq[kq_ < 2 * kWave ? kq_ : 2 * kWave - 1];
       }
-      if (i < len_q) {
-        const double2 e = eq[off_q + i];
-        accq = __builtin_fma(e.x, e.y, accq);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const double nx = __builtin_fma(fx[u].x, fx[u].y, accx);
+        const double nq = __builtin_fma(fq[u].x, fq[u].y, accq);
+        accx = i0 + u < len_x ? nx : accx;
+        accq = i0 + u < len_q ? nq : accq;
       }
     }
     __builtin_amdgcn_wave_barrier();
   }
   if (lane < s.kx) P.gcdf_x[ox + lane] = accx;
   if (lane < s.kq) P.gcdf_q[oq + lane] = accq;
+}
+
+// Largest value over the wave.
+__device__ __forceinline__ int64_t wave_max_i64(int64_t v) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) {
+    const uint32_t lo = uint32_t(__shfl_xor(int(uint32_t(uint64_t(v))), d, kWave));
+    const uint32_t hi = uint32_t(__shfl_xor(int(uint32_t(uint64_t(v) >> 32)), d, kWave));
+    const int64_t o = int64_t(uint64_t(lo) | (uint64_t(hi) << 32));
+    v = o > v ? o : v;
+  }
+  return v;
+}
+
+// Node offsets of cell g (x curve, x bins, q curve, q bins).
+struct CellGeo {
+  int64_t ox, bx, oq, bq;
+};
+
+__device__ __forceinline__ CellGeo cell_geo(const loits_hip_shape& s, int64_t g) {
+  const int64_t b = g / s.cells, cell = g - b * s.cells;
+  const int64_t ix = cell / s.ny, iy = cell - ix * s.ny;
+  return CellGeo{((b * s.ny + iy) * s.nx + ix) * s.kx, (b * s.nx + ix) * s.kx,
+                 ((b * s.nx + ix) * s.ny + iy) * s.kq, (b * s.ny + iy) * s.kq};
+}
+
+// Curves with kx + kq <= 32: C = 64 / (kx + kq) cells per wave, so that the
+// sums of ivjp_wave_kernel keep most lanes busy.  Lanes [c R, c R + R)
+// (R = 64 / C) take the rows of cell c, R per round; lane c L + j (L = kx + kq)
+// owns bin j of x (j < kx) or bin j - kx of q of cell c and sums the factor
+// pairs of the rows touching it, in row order (openmp/loits_core.cpp:285-352).
+template <typename Real>
+__global__ void __launch_bounds__(256) ivjp_cells_kernel(VjpParams P0) {
+  const VjpParams P = with_device_sizes(P0);
+  __shared__ unsigned long long touch[4 * kWave];
+  __shared__ int offsets[4 * kWave];
+  __shared__ double2 entries[4 * 4 * kWave];
+  __shared__ double nodes[4 * 2 * kWave];
+  const loits_hip_shape& s = P.s;
+  const int lane = lane_id(), w = threadIdx.x >> 6;
+  const int kx = int(s.kx), L = int(s.kx + s.kq), C = kWave / L, R = kWave / C;
+  const int64_t gcells = s.batch * s.cells;
+  const int64_t g0 = (int64_t(blockIdx.x) * (blockDim.x >> 6) + w) * C;
+  if (g0 >= gcells) return;  // whole waves (no barrier below)
+  unsigned long long* T = touch + w * kWave;  // rows with value j, then touching bin j
+  int* offs = offsets + w * kWave;            // first list entry of bin j
+  double2* ent = entries + w * 4 * kWave;     // factor pairs, bin by bin, rows in order
+  double* nc = nodes + w * 2 * kWave;         // C and x (or q) of the node of bin lane j
+  double* nn = nc + kWave;
+  const int cb = lane / L, bb = lane - cb * L;  // this lane as a bin
+  const int cr = lane / R, ir = lane - cr * R;  // this lane as a row
+  const bool bin_live = cb < C && g0 + cb < gcells;
+  const bool row_lane = cr < C;
+  int64_t lo = 0, hi = 0;  // rows of cell cr
+  for (int c = 0; c < C && g0 + c < gcells; ++c) {
+    const int64_t key = (g0 + c) * P.nmax;
+    const int64_t a = wave_lower_bound(P.packed, P.nvalid, key);
+    const int64_t b = wave_lower_bound(P.packed, P.nvalid, key + P.nmax);
+    if (cr == c) {
+      lo = a;
+      hi = b;
+    }
+  }
+  if (bin_live) {
+    const CellGeo q = cell_geo(s, g0 + cb);
+    if (bb < kx) {
+      nc[lane] = P.cdf_x[q.ox + bb];
+      nn[lane] = ld<Real>(P.x_bins, q.bx + bb);
+    } else {
+      nc[lane] = P.cdf_q[q.oq + bb - kx];
+      nn[lane] = ld<Real>(P.q_bins, q.bq + bb - kx);
+    }
+  }
+  __builtin_amdgcn_wave_barrier();
+  const int64_t rounds = wave_max_i64(row_lane ? (hi - lo + R - 1) / R : 0);
+  const int cl = row_lane ? cr * L : 0;  // first bin lane of this row's cell
+  // the packed slot and cotangents of a round's rows are loaded one round ahead
+  int64_t p = 0;
+  double gx = 0.0, gq = 0.0;
+  if (row_lane && lo + ir < hi) {
+    const int64_t row = lo + ir;
+    p = P.packed[row];
+    gx = ld<Real>(P.grad, 2 * row);
+    gq = ld<Real>(P.grad, 2 * row + 1);
+  }
+  double acc = 0.0;
+  for (int64_t k = 0; k < rounds; ++k) {
+    const int64_t row = lo + k * R + ir;
+    const bool live = row_lane && row < hi;
+    int jx = 0, jq = 0;
+    double ux = 0.0, uq = 0.0;
+    if (live) {
+      jx = P.interval_x[p];
+      jq = P.interval_q[p];
+      ux = double(P.u_x[p]);
+      uq = double(P.u_q[p]);
+    }
+    const double gxb = gx, gqb = gq;
+    if (row_lane && row + R < hi) {
+      p = P.packed[row + R];
+      gx = ld<Real>(P.grad, 2 * (row + R));
+      gq = ld<Real>(P.grad, 2 * (row + R) + 1);
+    }
+    const int la = cl + jx, qa = cl + kx + jq;  // bin lanes of this row's first nodes
+    double upx, ax, bxm, idx, upq, aq, bqm, idq;
+    {  // x: the factors of ivjp_row
+      const double c0 = nc[la];
+      const double d = nc[la + 1] - c0 + kEpsilon;
+      const double inv_d = 1.0 / d;
+      const double t = ux - c0;
+      upx = gxb * (nn[la + 1] - nn[la]);
+      ax = __builtin_fma(t * inv_d, inv_d, -inv_d);
+      bxm = -(upx * t * inv_d);
+      idx = inv_d;
+    }
+    {  // q
+      const double c0 = nc[qa];
+      const double d = nc[qa + 1] - c0 + kEpsilon;
+      const double inv_d = 1.0 / d;
+      const double t = uq - c0;
+      upq = gqb * (nn[qa + 1] - nn[qa]);
+      aq = __builtin_fma(t * inv_d, inv_d, -inv_d);
+      bqm = -(upq * t * inv_d);
+      idq = inv_d;
+    }
+    T[lane] = 0ull;
+    __builtin_amdgcn_wave_barrier();
+    if (live) {
+      atomicOr(&T[la], 1ull << lane);
+      atomicOr(&T[qa], 1ull << lane);
+    }
+    __builtin_amdgcn_wave_barrier();
+    uint64_t t = T[lane];
+    const uint64_t up = shfl_up1_u64(t);
+    t |= (bin_live && bb != 0 && bb != kx) ? up : 0ull;  // rows with value j - 1 of this axis
+    const int len = popc64(t);
+    const int off = wave_inclusive_scan(len) - len;  // at most 4 rows' worth in all
+    __builtin_amdgcn_wave_barrier();
+    T[lane] = t;
+    offs[lane] = off;
+    __builtin_amdgcn_wave_barrier();
+    if (live) {
+      const uint64_t lt = lanemask_lt();
+      ent[offs[la] + popc64(T[la] & lt)] = make_double2(upx, ax);
+      ent[offs[la + 1] + popc64(T[la + 1] & lt)] = make_double2(bxm, idx);
+      ent[offs[qa] + popc64(T[qa] & lt)] = make_double2(upq, aq);
+      ent[offs[qa + 1] + popc64(T[qa + 1] & lt)] = make_double2(bqm, idq);
+    }
+    __builtin_amdgcn_wave_barrier();
+    const int steps = int(wave_max_i64(len));
+    for (int i0 = 0; i0 < steps; i0 += 4) {
+      double2 f[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int e = off + i0 + u;
+        f[u] = ent[e < 4 * kWave ? e : 4 * kWave - 1];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const double nxt = __builtin_fma(f[u].x, f[u].y, acc);
+        acc = i0 + u < len ? nxt : acc;
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+  if (bin_live) {
+    const CellGeo q = cell_geo(s, g0 + cb);
+    if (bb < kx) P.gcdf_x[q.ox + bb] = acc;
+    else P.gcdf_q[q.oq + bb - kx] = acc;
+  }
 }
 
 template <typename Real>
@@ -997,7 +1180,16 @@ int backward_impl(const loits_hip_backward_args* a, const int64_t* dev_nmax,
               a->grad_xsec_q, dev_nmax, dev_nvalid};
   const bool f32 = a->dtype == LOITS_HIP_F32;
   const int64_t gcells = s.batch * s.cells;
-  if (s.kx <= kWave && s.kq <= kWave) {
+  if (s.kx + s.kq <= kWave / 2) {
+    const int64_t per_wave = kWave / (s.kx + s.kq);
+    const int64_t waves = (gcells + per_wave - 1) / per_wave;
+    if (f32)
+      hipLaunchKernelGGL(ivjp_cells_kernel<float>, dim3(blocks_for(waves, 4)), dim3(256), 0, st,
+                         V);
+    else
+      hipLaunchKernelGGL(ivjp_cells_kernel<double>, dim3(blocks_for(waves, 4)), dim3(256), 0, st,
+                         V);
+  } else if (s.kx <= kWave && s.kq <= kWave) {
     if (f32)
       hipLaunchKernelGGL(ivjp_wave_kernel<float>, dim3(blocks_for(gcells, 4)), dim3(256), 0, st,
                          V);
