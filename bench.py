@@ -50,6 +50,9 @@ def parse():
     p.add_argument("--no-extras", action="store_true",
                    help="skip the other workloads (forward at B=1024 N=4096, recomputed "
                         "tables, the 4096-curve ragged batch)")
+    p.add_argument("--dropin-only", action="store_true",
+                   help="only time the reference-API (two-axis) path at the GAN shapes and "
+                        "print those timings")
     p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                    help="only copy a MIB-MiB buffer a few times (the plain-copy kernel that "
                         "scales rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
@@ -248,6 +251,9 @@ def main():
     from quantom_loits_amd.workloads import uniform_family
 
     dev = torch.device("cuda", 0)
+    if args.dropin_only:
+        print(json.dumps({"dropin_two_axis": dropin_workloads(dev, args.steps)}))
+        return
     x_np, d_np = uniform_family(B, N, seed=0, dtype=np.float32)
     x = torch.from_numpy(x_np.reshape(-1)).to(dev)
     d = torch.from_numpy(d_np.reshape(-1)).to(dev)
