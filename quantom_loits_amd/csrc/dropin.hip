@@ -108,7 +108,8 @@ struct CurveParams {
   uint32_t* thr_q;
   double* m_x;
   double* m_q;
-  uint8_t* mono;  // [x curves | q curves]
+  uint8_t* mono;     // [x curves | q curves]
+  uint32_t* unsafe;  // set when some curve is not safe (below)
 };
 
 __device__ __forceinline__ bool norm_term_fused(int64_t i, int64_t nterms) {
@@ -191,6 +192,75 @@ __global__ void __launch_bounds__(256) curve_kernel(CurveParams P) {
     m[off + j] = (ld<Real>(bins, bins_off + j + 1) - ld<Real>(bins, bins_off + j)) * inv_d;
   }
   P.mono[c] = mono ? 1 : 0;
+  // A safe curve gives every slot a finite value in [1e-100, 1e100]: monotone
+  // thresholds put u at or above C_j (T_j <= k means C_j <= u, and T_0 = 0),
+  // ascending nodes make m_j >= 0, so x_j <= fma(m_j, u - C_j, x_j) <=
+  // x_j + m_j.  A cell of two safe curves then keeps every active slot
+  // (cpp/loits_core.cpp:256), and rows follow from the counts alone.
+  const double x0 = ld<Real>(bins, bins_off);
+  bool safe = mono && x0 >= 1e-100 && __builtin_isfinite(cdf[off + k - 1]);
+  for (int64_t j = 0; j + 1 < k; ++j) {
+    const double xj = ld<Real>(bins, bins_off + j), xn = ld<Real>(bins, bins_off + j + 1);
+    const double mj = m[off + j];
+    safe = safe && xn >= xj && __builtin_isfinite(cdf[off + j]) && __builtin_isfinite(mj) &&
+           xj + mj <= 1e100;
+  }
+  if (!safe) atomicOr(P.unsafe, 1u);
+}
+
+// row0[g] = active slots of the cells before g (counts summed in cell
+// order); row0[gcells] = all of them.  One work-group of 1024 threads, 8192
+// cells at a time: coalesced loads into LDS, eight consecutive cells per
+// thread, a block scan, coalesced stores.
+constexpr int kRowPiece = 8192;
+
+__global__ void __launch_bounds__(1024) row_start_kernel(const int64_t* counts, int64_t gcells,
+                                                         int64_t* row0) {
+  __shared__ int64_t buf[kRowPiece];
+  __shared__ int64_t wave_tot[16];
+  const int tid = threadIdx.x, wave = tid >> 6;
+  constexpr int kPer = kRowPiece / 1024;
+  int64_t carry = 0;
+  for (int64_t base = 0; base < gcells; base += kRowPiece) {
+    const int n = gcells - base < kRowPiece ? int(gcells - base) : kRowPiece;
+    int64_t v[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const int i = tid + k * 1024;
+      v[k] = i < n ? counts[base + i] : 0;
+    }
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) buf[tid + k * 1024] = v[k];
+    __syncthreads();
+    int64_t local = 0;
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      v[k] = buf[tid * kPer + k];
+      local += v[k];
+    }
+    const int64_t inc = wave_inclusive_scan(local);
+    if (lane_id() == kWave - 1) wave_tot[wave] = inc;
+    __syncthreads();
+    int64_t run = carry + inc - local, total = 0;
+    for (int w = 0; w < 16; ++w) {
+      run += w < wave ? wave_tot[w] : 0;
+      total += wave_tot[w];
+    }
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      buf[tid * kPer + k] = run;
+      run += v[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const int i = tid + k * 1024;
+      if (i < n) row0[base + i] = buf[i];
+    }
+    carry += total;
+    __syncthreads();
+  }
+  if (tid == 0) row0[gcells] = carry;
 }
 
 // Curves of at most kSmallK nodes: the count of cpp/loits_core.cpp:12-18 as
@@ -254,6 +324,8 @@ struct SampleParams {
   // sync-free form: nmax and active read here; the values above are capacities
   const int64_t* dev_na;
   int64_t active_cap;
+  const uint32_t* unsafe;    // 0: every cell safe, rows from row0 (no look-back)
+  const int64_t* row0;       // [cells + 1] first row of each cell
 };
 
 template <typename Real>
@@ -287,6 +359,7 @@ __global__ void __launch_bounds__(kTileThreads) sample_kernel(SampleParams P0) {
   // cell g and slot of p0 by one division; the next three slots step from there
   int64_t g = p0 / P.nmax, slot = p0 - g * P.nmax;
   int64_t cur_g = -1, count = 0, cxi = 0, cqi = 0, ox = 0, bx = 0, oq = 0, bq = 0;
+  int activebits = 0;
   bool mono_x = false, mono_q = false;
   float ufx[4], ufq[4];
   int16_t ix4[4] = {0, 0, 0, 0}, iq4[4] = {0, 0, 0, 0};
@@ -337,6 +410,7 @@ __global__ void __launch_bounds__(kTileThreads) sample_kernel(SampleParams P0) {
       mono_q = P.mono[nxc + cqi] != 0;
     }
     if (slot >= count) continue;
+    activebits |= 1 << e;
     // x: cpp/loits_core.cpp:183-196
     const int64_t jx = cnt_x[e] >= 0 ? clamp_interval(cnt_x[e], s.kx)
                                      : interval_of(P.thr_x + ox, s.kx, mono_x, kx);
@@ -369,6 +443,11 @@ __global__ void __launch_bounds__(kTileThreads) sample_kernel(SampleParams P0) {
       P.interval_q[p0 + e] = iq4[e];
     }
   }
+  // Every cell safe: every active slot is kept, and the tile's first row is
+  // the count of active slots before it (no look-back).
+  const bool direct = *P.unsafe == 0;
+  if (direct && keepbits != activebits)  // cannot happen for safe curves: report it
+    atomicOr(reinterpret_cast<unsigned long long*>(&P.valid[1]), 4ull);
   // in-tile ordered offsets
   const int cnt = __popc(keepbits);
   const uint64_t lt = lanemask_lt();
@@ -387,7 +466,14 @@ __global__ void __launch_bounds__(kTileThreads) sample_kernel(SampleParams P0) {
   // inclusive prefix (a predecessor without a status yet is waited for).
   // Each status word carries its own value, so relaxed device-scope atomics
   // suffice (no fence: nothing else written by a tile is read in this kernel).
-  if (threadIdx.x < kWave) {
+  if (direct) {
+    if (threadIdx.x == 0) {
+      const int64_t p = tile * kTileSlots, g0 = p / P.nmax, s0 = p - g0 * P.nmax;
+      const int64_t c0 = P.counts[g0];
+      sh_excl = P.row0[g0] + (s0 < c0 ? s0 : c0);
+      if (tile == P.n_tiles - 1) P.valid[0] = P.row0[s.batch * s.cells];
+    }
+  } else if (threadIdx.x < kWave) {
     const int lane = lane_id();
     uint64_t* st = P.tile_status;
     if (tile == 0) {
@@ -978,6 +1064,8 @@ struct FwdWorkspace {
   uint8_t* mono;
   uint64_t* tile_status;
   uint32_t* tile_counter;
+  uint32_t* unsafe;  // in the cleared header after the tile counter
+  int64_t* row0;
   int64_t bytes;
 };
 
@@ -990,6 +1078,7 @@ FwdWorkspace fwd_workspace(const loits_hip_shape& s, int64_t nmax, void* base) {
   FwdWorkspace w{};
   int64_t off = 0;
   w.tile_counter = reinterpret_cast<uint32_t*>(p + off);
+  w.unsafe = w.tile_counter + 1;
   off += 64;
   w.tile_status = reinterpret_cast<uint64_t*>(p + off);
   off += align16(8 * (tiles > 0 ? tiles : 1));
@@ -1003,6 +1092,8 @@ FwdWorkspace fwd_workspace(const loits_hip_shape& s, int64_t nmax, void* base) {
   off += align16(8 * nqe);
   w.mono = p + off;
   off += align16(curves);
+  w.row0 = reinterpret_cast<int64_t*>(p + off);
+  off += align16(8 * (s.batch * s.cells + 1));
   w.bytes = off;
   return w;
 }
@@ -1083,9 +1174,17 @@ int forward_impl(const loits_hip_forward_args* a, const int64_t* dev_na, void* s
   const int64_t tiles = (slots + kTileSlots - 1) / kTileSlots;
   const bool f32 = a->dtype == LOITS_HIP_F32;
 
+  // zero: tile counter, safety flag, look-back status words, valid[0..1]
+  const int64_t zwords = (64 + align16(8 * (tiles > 0 ? tiles : 1))) / 8;
+  hipLaunchKernelGGL(zero_words, dim3(blocks_for(zwords, 256)), dim3(256), 0, st,
+                     reinterpret_cast<uint64_t*>(w.tile_counter), zwords);
+  hipLaunchKernelGGL(zero_words, dim3(1), dim3(64), 0, st,
+                     reinterpret_cast<uint64_t*>(a->valid), int64_t(2));
+  rc = check_hip(hipGetLastError(), "forward zero launch");
+  if (rc != LOITS_HIP_OK) return rc;
   CurveParams C{s,          a->x_bins, a->xsec_x, a->q_bins, a->xsec_q, a->acceptance,
                 a->norm_x,  a->norm_q, a->cdf_x,  a->cdf_q,  w.thr_x,   w.thr_q,
-                w.m_x,      w.m_q,     w.mono};
+                w.m_x,      w.m_q,     w.mono,    w.unsafe};
   const int64_t curves = 2 * s.batch * s.cells;
   if (f32)
     hipLaunchKernelGGL(curve_kernel<float>, dim3(blocks_for(curves, 256)), dim3(256), 0, st, C);
@@ -1093,13 +1192,9 @@ int forward_impl(const loits_hip_forward_args* a, const int64_t* dev_na, void* s
     hipLaunchKernelGGL(curve_kernel<double>, dim3(blocks_for(curves, 256)), dim3(256), 0, st, C);
   rc = check_hip(hipGetLastError(), "forward curve launch");
   if (rc != LOITS_HIP_OK) return rc;
-  // zero: tile counter, look-back status words, valid[0..1]
-  const int64_t zwords = (64 + align16(8 * (tiles > 0 ? tiles : 1))) / 8;
-  hipLaunchKernelGGL(zero_words, dim3(blocks_for(zwords, 256)), dim3(256), 0, st,
-                     reinterpret_cast<uint64_t*>(w.tile_counter), zwords);
-  hipLaunchKernelGGL(zero_words, dim3(1), dim3(64), 0, st,
-                     reinterpret_cast<uint64_t*>(a->valid), int64_t(2));
-  rc = check_hip(hipGetLastError(), "forward zero launch");
+  hipLaunchKernelGGL(row_start_kernel, dim3(1), dim3(1024), 0, st, a->counts,
+                     s.batch * s.cells, w.row0);
+  rc = check_hip(hipGetLastError(), "forward row-start launch");
   if (rc != LOITS_HIP_OK) return rc;
   if (tiles > 0) {
     if (tiles > INT32_MAX) return fail(LOITS_HIP_EUNSUPPORTED, "forward: too many slots");
@@ -1109,7 +1204,7 @@ int forward_impl(const loits_hip_forward_args* a, const int64_t* dev_na, void* s
                    w.thr_q,    w.m_x,      w.m_q,      w.mono,      a->u_x,
                    a->u_q,     a->interval_x,          a->interval_q,
                    a->events,  a->packed,  w.tile_status,           w.tile_counter,
-                   a->valid,   dev_na,     a->active};
+                   a->valid,   dev_na,     a->active,  w.unsafe,    w.row0};
     if (f32)
       hipLaunchKernelGGL(sample_kernel<float>, dim3(unsigned(tiles)), dim3(kTileThreads), 0, st, S);
     else
