@@ -637,10 +637,11 @@ __global__ void __launch_bounds__(256) ivjp_kernel(VjpParams P0) {
   }
 }
 
-// First index i in [0, n) with a[i] >= key (n if none) of an ascending array:
-// a 65-way search, one probe per lane per step (wave-uniform result).
-__device__ __forceinline__ int64_t wave_lower_bound(const int64_t* a, int64_t n, int64_t key) {
-  int64_t lo = 0, hi = n;  // the answer lies in [lo, hi]
+// First index i in [lo, hi) with a[i] >= key (hi if none) of an ascending
+// array: a 65-way search, one probe per lane per step (wave-uniform result).
+__device__ __forceinline__ int64_t wave_lower_bound(const int64_t* a, int64_t lo, int64_t hi,
+                                                    int64_t key) {
+  // the answer lies in [lo, hi]
   const int lane = lane_id();
   while (lo < hi) {
     const int64_t pos = lo + ((hi - lo) * (lane + 1)) / (kWave + 1);  // in [lo, hi), ascending
@@ -689,8 +690,9 @@ __global__ void __launch_bounds__(256) ivjp_wave_kernel(VjpParams P0) {
   const int64_t ox = ((b * s.ny + iy) * s.nx + ix) * s.kx, bx = (b * s.nx + ix) * s.kx;
   const int64_t oq = ((b * s.nx + ix) * s.ny + iy) * s.kq, bq = (b * s.ny + iy) * s.kq;
   const int64_t key0 = g * P.nmax;
-  const int64_t lo0 = wave_lower_bound(P.packed, P.nvalid, key0);  // rows of this cell: [lo0, lo1)
-  const int64_t lo1 = wave_lower_bound(P.packed, P.nvalid, key0 + P.nmax);
+  const int64_t lo0 = wave_lower_bound(P.packed, 0, P.nvalid, key0);  // rows of this cell: [lo0, lo1)
+  const int64_t end = P.nvalid - lo0 < P.nmax ? P.nvalid : lo0 + P.nmax;  // at most nmax rows
+  const int64_t lo1 = wave_lower_bound(P.packed, lo0, end, key0 + P.nmax);
   const int w = threadIdx.x >> 6;
   unsigned long long* mx = masks + w * 2 * kWave;  // rows with interval j, then touching bin j
   unsigned long long* mq = mx + kWave;
@@ -862,15 +864,18 @@ __global__ void __launch_bounds__(256) ivjp_cells_kernel(VjpParams P0) {
   const int cr = lane / R, ir = lane - cr * R;  // this lane as a row
   const bool bin_live = cb < C && g0 + cb < gcells;
   const bool row_lane = cr < C;
-  int64_t lo = 0, hi = 0;  // rows of cell cr
+  // rows of cell cr: [lo, hi); a cell holds at most nmax rows, so each
+  // boundary after the first is searched within nmax of the previous one
+  int64_t lo = 0, hi = 0;
+  int64_t a = wave_lower_bound(P.packed, 0, P.nvalid, g0 * P.nmax);
   for (int c = 0; c < C && g0 + c < gcells; ++c) {
-    const int64_t key = (g0 + c) * P.nmax;
-    const int64_t a = wave_lower_bound(P.packed, P.nvalid, key);
-    const int64_t b = wave_lower_bound(P.packed, P.nvalid, key + P.nmax);
+    const int64_t end = P.nvalid - a < P.nmax ? P.nvalid : a + P.nmax;
+    const int64_t b = wave_lower_bound(P.packed, a, end, (g0 + c + 1) * P.nmax);
     if (cr == c) {
       lo = a;
       hi = b;
     }
+    a = b;
   }
   if (bin_live) {
     const CellGeo q = cell_geo(s, g0 + cb);
