@@ -783,15 +783,9 @@ __global__ void __launch_bounds__(256) ivjp_wave_kernel(VjpParams P0) {
       eq[qb.y + popc64(mq[jq + 1] & lt)] = make_double2(bqm, idq);
     }
     __builtin_amdgcn_wave_barrier();
-    int steps = len_x > len_q ? len_x : len_q;
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) {
-      const int o = __shfl_xor(steps, d, kWave);
-      steps = o > steps ? o : steps;
-    }
     // four entries per list at a time, their reads issued together (a lane
     // past the end of its list reads an entry in range and keeps its sum)
-    for (int i0 = 0; i0 < steps; i0 += 4) {
+    for (int i0 = 0; ballot(i0 < len_x || i0 < len_q) != 0ull; i0 += 4) {
       double2 fx[4], fq[4];
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
@@ -963,8 +957,7 @@ __global__ void __launch_bounds__(256) ivjp_cells_kernel(VjpParams P0) {
       ent[offs[qa + 1] + popc64(T[qa + 1] & lt)] = make_double2(bqm, idq);
     }
     __builtin_amdgcn_wave_barrier();
-    const int steps = int(wave_max_i64(len));
-    for (int i0 = 0; i0 < steps; i0 += 4) {
+    for (int i0 = 0; ballot(i0 < len) != 0ull; i0 += 4) {  // until every list is summed
       double2 f[4];
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
