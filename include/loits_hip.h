@@ -110,7 +110,9 @@ typedef struct loits_hip_forward_args {
   int64_t* packed;        /* [active] */
   int64_t* row_offsets;   /* [B*cells + 1] or NULL */
   void* workspace;        /* loits_hip_forward_workspace_size() bytes */
-  int64_t* valid;         /* device out [2]: kept events, error flag (0 = ok) */
+  int64_t* valid;         /* device out [2]: kept events, error flags (0 = ok; bit 0: a
+                             look-back wait ran out, bit 1: bounded capacities exceeded,
+                             bit 2: a slot of a drop-free curve was dropped) */
 } loits_hip_forward_args;
 
 /* Bytes of device workspace loits_hip_forward needs for this call. */
@@ -119,7 +121,8 @@ int64_t loits_hip_forward_workspace_size(const loits_hip_shape* shape, int64_t n
 /* The reference forward (cpp/bindings.cpp:24-130): densities, CDFs, both
  * Philox streams (2*sequence, 2*sequence+1), intervals, interpolation and the
  * stable compaction of cpp/loits_core.cpp:239-266 (single pass, ordered by a
- * decoupled look-back over tiles).  The CDF chain follows the reference's
+ * decoupled look-back over tiles, or, when every curve is drop-free, by the
+ * active slots before each tile).  The CDF chain follows the reference's
  * sequential order, so norm, cdf, u, intervals, events and packed equal the
  * reference's.  No host synchronisation: valid[0] holds the kept count. */
 int loits_hip_forward(const loits_hip_forward_args* args, void* stream);

@@ -179,8 +179,11 @@ class BoundedLOITS:
 
     def state(self):
         nmax, _, kept, err = self.sizes()
-        if err:
+        if err & 2:
             raise RuntimeError(f"bounded forward failed (error bits {err}); raise the capacities")
+        if err:
+            raise RuntimeError(f"bounded forward failed (error bits {err}): the compaction "
+                               "did not complete")
         B, nx, ny, _, _ = self.shape
         g = B * nx * ny
         return [self.events[:kept], self.norm_x, self.norm_q, self.cdf_x, self.cdf_q,

@@ -160,7 +160,10 @@ std::vector<at::Tensor> forward(at::Tensor x_bins, at::Tensor xsec_x, at::Tensor
     check(loits_hip_forward(&a, stream), "loits_hip_forward");
   }
   at::Tensor host_valid = valid.cpu();
-  if (host_valid[1].item<int64_t>() != 0)
+  const int64_t flags = host_valid[1].item<int64_t>();
+  if (flags & 4)
+    throw std::runtime_error("loits_hip_forward: a slot of a drop-free curve was dropped");
+  if (flags != 0)
     throw std::runtime_error("loits_hip_forward: compaction look-back did not complete");
   const int64_t nvalid = host_valid[0].item<int64_t>();
   RegionGuard region(profile_regions, "loits::forward::state_pack");
