@@ -542,10 +542,16 @@ __global__ void __launch_bounds__(kTileThreads) sample_kernel(SampleParams P0) {
   }
 }
 
+// Drop-free curves: the rows of cell g start at row0[g] (no search).
 __global__ void row_offsets_kernel(const int64_t* packed, const int64_t* valid, int64_t nmax,
-                                   int64_t gcells, int64_t* row_offsets, const int64_t* dev_na) {
+                                   int64_t gcells, int64_t* row_offsets, const int64_t* dev_na,
+                                   const uint32_t* unsafe, const int64_t* row0) {
   const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (g > gcells) return;
+  if (*unsafe == 0) {
+    row_offsets[g] = row0[g];
+    return;
+  }
   if (dev_na) nmax = dev_na[0];
   const int64_t key = g * nmax;
   int64_t lo = 0, hi = valid[0];
@@ -1213,7 +1219,8 @@ int forward_impl(const loits_hip_forward_args* a, const int64_t* dev_na, void* s
   if (a->row_offsets) {
     const int64_t gcells = s.batch * s.cells;
     hipLaunchKernelGGL(row_offsets_kernel, dim3(blocks_for(gcells + 1, 256)), dim3(256), 0, st,
-                       a->packed, a->valid, a->nmax, gcells, a->row_offsets, dev_na);
+                       a->packed, a->valid, a->nmax, gcells, a->row_offsets, dev_na, w.unsafe,
+                       w.row0);
     rc = check_hip(hipGetLastError(), "forward row_offsets launch");
   }
   return rc;
