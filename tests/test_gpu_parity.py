@@ -115,6 +115,47 @@ def test_dropin_float32_inputs_close_to_float64_reference(ext, golden_reference)
     np.testing.assert_allclose(st[0].cpu().numpy().astype(np.float64), ref["events"], rtol=1e-6)
 
 
+@pytest.mark.parametrize("kx,kq,zero_node", [(3, 29, False), (7, 2, False), (5, 12, True),
+                                             (40, 3, False)])
+def test_dropin_uneven_axes_bit_exact_vs_oracle(ext, oracle_port, kx, kq, zero_node):
+    """Different node counts per axis: the multi-cell interpolation VJP
+    (kx + kq <= 32, 64 / (kx + kq) cells per wave), the one-cell-per-wave VJP
+    (40 + 3) and both compaction orders: drop-free curves (rows from the cell
+    counts), and a node at x = 0 plus a NaN density, whose cell's slots are all
+    dropped (look-back order)."""
+    rng = np.random.default_rng(kx * 100 + kq)
+    B, nx, ny = 2, 3, 4
+    xb = np.sort(rng.uniform(0.01, 1.0, (B, nx, kx)), axis=2)
+    if zero_node:
+        xb[0, 1, 0] = 0.0
+    qb = np.sort(rng.uniform(1.0, 50.0, (B, ny, kq)), axis=2)
+    xs = rng.uniform(0.1, 1.0, (B, ny, nx, kx))
+    if zero_node:
+        xs[1, 2, 0, 1] = np.nan
+    qs = rng.uniform(0.1, 1.0, (B, nx, ny, kq))
+    w = rng.uniform(0.5, 1.5, (B, nx, ny))
+    w /= w.sum()
+    acc = rng.uniform(0, 1, (B, nx, ny)) < 0.9
+    n_events, seed, sequence = 60000, 11, 3
+    ref = oracle_port.forward(xb, xs, qb, qs, w, acc, n_events, seed, sequence)
+    print("uneven axes", kx, kq, zero_node, flush=True)
+    args = _to_dev([xb, xs, qb, qs, w, acc], torch.float64)
+    st = ext.forward(*args, n_events, seed, sequence, False)
+    events, norm_x, norm_q, cdf_x, cdf_q, u_x, u_q, ix, iq, packed, row_offsets = st
+    np.testing.assert_array_equal(events.cpu().numpy(), ref["events"])
+    np.testing.assert_array_equal(packed.cpu().numpy(), ref["packed"])
+    np.testing.assert_array_equal(row_offsets.cpu().numpy(), ref["row_offsets"])
+    if zero_node:
+        assert len(ref["packed"]) < int(ref["active"])  # some slots were dropped
+    g = rng.standard_normal(ref["events"].shape)
+    gx_ref, gq_ref = oracle_port.backward(g, xb, xs, qb, qs, acc, ref)
+    gx, gq = ext.backward(torch.from_numpy(g).cuda(), args[0], args[1], args[2], args[3], args[5],
+                          norm_x, norm_q, cdf_x, cdf_q, u_x, u_q, ix, iq, packed, row_offsets,
+                          False)
+    np.testing.assert_array_equal(gx.cpu().numpy(), gx_ref)
+    np.testing.assert_array_equal(gq.cpu().numpy(), gq_ref)
+
+
 def _batched_inputs(g, dtype):
     x = torch.from_numpy(np.ascontiguousarray(g["x"])).to(dtype).cuda().reshape(-1)
     d = torch.from_numpy(np.ascontiguousarray(g["density"])).to(dtype).cuda().reshape(-1)
