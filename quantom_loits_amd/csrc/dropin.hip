@@ -36,8 +36,12 @@
 namespace qloits {
 namespace {
 
-constexpr int kTileThreads = 1024;  // 4096 slots per tile: fewer look-back chains
+// Tiles of the look-back order: 4096 slots (fewer chains); tiles of curves
+// that are all drop-free: 1024 slots (no ordering, several groups per CU).
+constexpr int kTileThreads = 1024;
 constexpr int64_t kTileSlots = 4 * kTileThreads;
+constexpr int kDirectThreads = 256;
+constexpr int64_t kDirectSlots = 4 * kDirectThreads;
 constexpr uint64_t kStatusShift = 62;
 constexpr uint64_t kAggregate = 1ull << kStatusShift;
 constexpr uint64_t kInclusive = 2ull << kStatusShift;
@@ -328,10 +332,20 @@ struct SampleParams {
   const int64_t* row0;       // [cells + 1] first row of each cell
 };
 
-template <typename Real>
-__global__ void __launch_bounds__(kTileThreads) sample_kernel(SampleParams P0) {
+// Launched twice per call: DIRECT (kDirectThreads) does the work when every
+// curve is drop-free, the other (kTileThreads) when some curve is not; each
+// returns at once in the other case.  Drop-free: every active slot is kept,
+// a tile's first row is the count of active slots before it and the
+// work-group index is the tile.  Otherwise tiles are taken from a counter, in
+// the order the decoupled look-back needs.
+template <typename Real, bool DIRECT>
+__global__ void __launch_bounds__(DIRECT ? kDirectThreads : kTileThreads)
+    sample_kernel(SampleParams P0) {
+  constexpr int kThreads = DIRECT ? kDirectThreads : kTileThreads;
+  constexpr int64_t kSlots = 4 * kThreads;
   __shared__ int32_t wave_tot[16];
   __shared__ int64_t sh_tile, sh_excl;
+  if ((*P0.unsafe == 0) != DIRECT) return;  // the other launch's case
   SampleParams P = P0;
   const loits_hip_shape& s = P.s;
   if (P0.dev_na) {  // sizes from the device; the launch covers the capacities
@@ -343,13 +357,15 @@ __global__ void __launch_bounds__(kTileThreads) sample_kernel(SampleParams P0) {
     }
     P.nmax = nm;
     P.total_slots = s.batch * s.cells * nm;
-    P.n_tiles = (P.total_slots + kTileSlots - 1) / kTileSlots;
   }
-  if (threadIdx.x == 0) sh_tile = int64_t(atomicAdd(P.tile_counter, 1u));
+  const int64_t n_tiles = (P.total_slots + kSlots - 1) / kSlots;
+  constexpr bool direct = DIRECT;
+  if (threadIdx.x == 0)
+    sh_tile = direct ? int64_t(blockIdx.x) : int64_t(atomicAdd(P.tile_counter, 1u));
   __syncthreads();
   const int64_t tile = sh_tile;
-  if (tile >= P.n_tiles) return;  // (capacity tiles past the live slots)
-  const int64_t q = tile * kTileThreads + threadIdx.x;  // Philox block of this thread
+  if (tile >= n_tiles) return;  // (capacity tiles past the live slots)
+  const int64_t q = tile * kThreads + threadIdx.x;  // Philox block of this thread
   const Words4 wx = philox4x32_10(uint64_t(q), P.sid_x, P.seed);
   const Words4 wq = philox4x32_10(uint64_t(q), P.sid_q, P.seed);
   double vx[4], vq[4];
@@ -443,9 +459,6 @@ __global__ void __launch_bounds__(kTileThreads) sample_kernel(SampleParams P0) {
       P.interval_q[p0 + e] = iq4[e];
     }
   }
-  // Every cell safe: every active slot is kept, and the tile's first row is
-  // the count of active slots before it (no look-back).
-  const bool direct = *P.unsafe == 0;
   if (direct && keepbits != activebits)  // cannot happen for safe curves: report it
     atomicOr(reinterpret_cast<unsigned long long*>(&P.valid[1]), 4ull);
   // in-tile ordered offsets
@@ -457,7 +470,7 @@ __global__ void __launch_bounds__(kTileThreads) sample_kernel(SampleParams P0) {
   if (lane_id() == 0) wave_tot[wave] = popc64(b0) + 2 * popc64(b1) + 4 * popc64(b2);
   __syncthreads();
   int base = 0, agg = 0;
-  for (int w = 0; w < kTileThreads / 64; ++w) {
+  for (int w = 0; w < kThreads / 64; ++w) {
     agg += wave_tot[w];
     base += w < wave ? wave_tot[w] : 0;
   }
@@ -468,10 +481,10 @@ __global__ void __launch_bounds__(kTileThreads) sample_kernel(SampleParams P0) {
   // suffice (no fence: nothing else written by a tile is read in this kernel).
   if (direct) {
     if (threadIdx.x == 0) {
-      const int64_t p = tile * kTileSlots, g0 = p / P.nmax, s0 = p - g0 * P.nmax;
+      const int64_t p = tile * kSlots, g0 = p / P.nmax, s0 = p - g0 * P.nmax;
       const int64_t c0 = P.counts[g0];
       sh_excl = P.row0[g0] + (s0 < c0 ? s0 : c0);
-      if (tile == P.n_tiles - 1) P.valid[0] = P.row0[s.batch * s.cells];
+      if (tile == n_tiles - 1) P.valid[0] = P.row0[s.batch * s.cells];
     }
   } else if (threadIdx.x < kWave) {
     const int lane = lane_id();
@@ -526,7 +539,7 @@ __global__ void __launch_bounds__(kTileThreads) sample_kernel(SampleParams P0) {
         sh_excl = excl;
       }
     }
-    if (lane == 0 && tile == P.n_tiles - 1) P.valid[0] = sh_excl + agg;
+    if (lane == 0 && tile == n_tiles - 1) P.valid[0] = sh_excl + agg;
   }
   __syncthreads();
   int64_t row = sh_excl + base + pre;
@@ -1209,10 +1222,19 @@ int forward_impl(const loits_hip_forward_args* a, const int64_t* dev_na, void* s
                    a->u_q,     a->interval_x,          a->interval_q,
                    a->events,  a->packed,  w.tile_status,           w.tile_counter,
                    a->valid,   dev_na,     a->active,  w.unsafe,    w.row0};
-    if (f32)
-      hipLaunchKernelGGL(sample_kernel<float>, dim3(unsigned(tiles)), dim3(kTileThreads), 0, st, S);
-    else
-      hipLaunchKernelGGL(sample_kernel<double>, dim3(unsigned(tiles)), dim3(kTileThreads), 0, st, S);
+    const int64_t direct_tiles = (slots + kDirectSlots - 1) / kDirectSlots;
+    if (direct_tiles > INT32_MAX) return fail(LOITS_HIP_EUNSUPPORTED, "forward: too many slots");
+    if (f32) {
+      hipLaunchKernelGGL((sample_kernel<float, true>), dim3(unsigned(direct_tiles)),
+                         dim3(kDirectThreads), 0, st, S);
+      hipLaunchKernelGGL((sample_kernel<float, false>), dim3(unsigned(tiles)), dim3(kTileThreads),
+                         0, st, S);
+    } else {
+      hipLaunchKernelGGL((sample_kernel<double, true>), dim3(unsigned(direct_tiles)),
+                         dim3(kDirectThreads), 0, st, S);
+      hipLaunchKernelGGL((sample_kernel<double, false>), dim3(unsigned(tiles)),
+                         dim3(kTileThreads), 0, st, S);
+    }
     rc = check_hip(hipGetLastError(), "forward sample launch");
     if (rc != LOITS_HIP_OK) return rc;
   }
