@@ -37,10 +37,10 @@ namespace qloits {
 namespace {
 
 // Tiles of the look-back order: 4096 slots (fewer chains); tiles of curves
-// that are all drop-free: 1024 slots (no ordering, several groups per CU).
+// that are all drop-free: one wave of 256 slots (no ordering between tiles).
 constexpr int kTileThreads = 1024;
 constexpr int64_t kTileSlots = 4 * kTileThreads;
-constexpr int kDirectThreads = 256;
+constexpr int kDirectThreads = 64;
 constexpr int64_t kDirectSlots = 4 * kDirectThreads;
 constexpr uint64_t kStatusShift = 62;
 constexpr uint64_t kAggregate = 1ull << kStatusShift;
