@@ -850,8 +850,9 @@ __device__ __forceinline__ CellGeo cell_geo(const loits_hip_shape& s, int64_t g)
                  ((b * s.nx + ix) * s.ny + iy) * s.kq, (b * s.ny + iy) * s.kq};
 }
 
-// Curves with kx + kq <= 32: C = 64 / (kx + kq) cells per wave, so that the
-// sums of ivjp_wave_kernel keep most lanes busy.  Lanes [c R, c R + R)
+// Curves with kx + kq <= 64: C = min(64 / (kx + kq), 2) cells per wave (two
+// measured faster than one or three at 9801 cells of 10 + 10 nodes), one lane
+// per node of either axis.  Lanes [c R, c R + R)
 // (R = 64 / C) take the rows of cell c, R per round; lane c L + j (L = kx + kq)
 // owns bin j of x (j < kx) or bin j - kx of q of cell c and sums the factor
 // pairs of the rows touching it, in row order (openmp/loits_core.cpp:285-352).
@@ -864,7 +865,8 @@ __global__ void __launch_bounds__(256) ivjp_cells_kernel(VjpParams P0) {
   __shared__ double nodes[4 * 2 * kWave];
   const loits_hip_shape& s = P.s;
   const int lane = lane_id(), w = threadIdx.x >> 6;
-  const int kx = int(s.kx), L = int(s.kx + s.kq), C = kWave / L, R = kWave / C;
+  const int kx = int(s.kx), L = int(s.kx + s.kq), C = kWave / L < 2 ? kWave / L : 2;
+  const int R = kWave / C;
   const int64_t gcells = s.batch * s.cells;
   const int64_t g0 = (int64_t(blockIdx.x) * (blockDim.x >> 6) + w) * C;
   if (g0 >= gcells) return;  // whole waves (no barrier below)
@@ -1302,8 +1304,8 @@ int backward_impl(const loits_hip_backward_args* a, const int64_t* dev_nmax,
               a->grad_xsec_q, dev_nmax, dev_nvalid};
   const bool f32 = a->dtype == LOITS_HIP_F32;
   const int64_t gcells = s.batch * s.cells;
-  if (s.kx + s.kq <= kWave / 2) {
-    const int64_t per_wave = kWave / (s.kx + s.kq);
+  if (s.kx + s.kq <= kWave) {
+    const int64_t per_wave = kWave / (s.kx + s.kq) < 2 ? kWave / (s.kx + s.kq) : 2;
     const int64_t waves = (gcells + per_wave - 1) / per_wave;
     if (f32)
       hipLaunchKernelGGL(ivjp_cells_kernel<float>, dim3(blocks_for(waves, 4)), dim3(256), 0, st,

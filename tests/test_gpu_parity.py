@@ -116,13 +116,13 @@ def test_dropin_float32_inputs_close_to_float64_reference(ext, golden_reference)
 
 
 @pytest.mark.parametrize("kx,kq,zero_node", [(3, 29, False), (7, 2, False), (5, 12, True),
-                                             (40, 3, False)])
+                                             (40, 3, False), (60, 10, False)])
 def test_dropin_uneven_axes_bit_exact_vs_oracle(ext, oracle_port, kx, kq, zero_node):
-    """Different node counts per axis: the multi-cell interpolation VJP
-    (kx + kq <= 32, 64 / (kx + kq) cells per wave), the one-cell-per-wave VJP
-    (40 + 3) and both compaction orders: drop-free curves (rows from the cell
-    counts), and a node at x = 0 plus a NaN density, whose cell's slots are all
-    dropped (look-back order)."""
+    """Different node counts per axis: the interpolation VJP with a lane per
+    node (kx + kq <= 64; two cells per wave, or one for 40 + 3), the one with a
+    lane per node pair (60 + 10) and both compaction orders: drop-free curves
+    (rows from the cell counts), and a node at x = 0 plus a NaN density, whose
+    cell's slots are all dropped (look-back order)."""
     rng = np.random.default_rng(kx * 100 + kq)
     B, nx, ny = 2, 3, 4
     xb = np.sort(rng.uniform(0.01, 1.0, (B, nx, kx)), axis=2)
