@@ -1206,9 +1206,9 @@ int forward_impl(const loits_hip_forward_args* a, const int64_t* dev_na, void* s
                 w.m_x,      w.m_q,     w.mono,    w.unsafe};
   const int64_t curves = 2 * s.batch * s.cells;
   if (f32)
-    hipLaunchKernelGGL(curve_kernel<float>, dim3(blocks_for(curves, 256)), dim3(256), 0, st, C);
+    hipLaunchKernelGGL(curve_kernel<float>, dim3(blocks_for(curves, 64)), dim3(64), 0, st, C);
   else
-    hipLaunchKernelGGL(curve_kernel<double>, dim3(blocks_for(curves, 256)), dim3(256), 0, st, C);
+    hipLaunchKernelGGL(curve_kernel<double>, dim3(blocks_for(curves, 64)), dim3(64), 0, st, C);
   rc = check_hip(hipGetLastError(), "forward curve launch");
   if (rc != LOITS_HIP_OK) return rc;
   hipLaunchKernelGGL(row_start_kernel, dim3(1), dim3(1024), 0, st, a->counts,
@@ -1329,10 +1329,10 @@ int backward_impl(const loits_hip_backward_args* a, const int64_t* dev_nmax,
   if (rc != LOITS_HIP_OK) return rc;
   const int64_t curves = 2 * gcells;
   if (f32)
-    hipLaunchKernelGGL(curve_vjp_kernel<float>, dim3(blocks_for(curves, 256)), dim3(256), 0, st,
+    hipLaunchKernelGGL(curve_vjp_kernel<float>, dim3(blocks_for(curves, 64)), dim3(64), 0, st,
                        V);
   else
-    hipLaunchKernelGGL(curve_vjp_kernel<double>, dim3(blocks_for(curves, 256)), dim3(256), 0, st,
+    hipLaunchKernelGGL(curve_vjp_kernel<double>, dim3(blocks_for(curves, 64)), dim3(64), 0, st,
                        V);
   return check_hip(hipGetLastError(), "backward curve launch");
 }
