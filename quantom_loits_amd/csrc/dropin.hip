@@ -345,6 +345,9 @@ __global__ void __launch_bounds__(DIRECT ? kDirectThreads : kTileThreads)
   constexpr int64_t kSlots = 4 * kThreads;
   __shared__ int32_t wave_tot[16];
   __shared__ int64_t sh_tile, sh_excl;
+  // one-wave tiles stage their kept rows so the stores below are coalesced
+  __shared__ Real st_ev[DIRECT ? 2 * kSlots : 1];
+  __shared__ int64_t st_pk[DIRECT ? kSlots : 1];
   if ((*P0.unsafe == 0) != DIRECT) return;  // the other launch's case
   SampleParams P = P0;
   const loits_hip_shape& s = P.s;
@@ -541,9 +544,26 @@ __global__ void __launch_bounds__(DIRECT ? kDirectThreads : kTileThreads)
     }
     if (lane == 0 && tile == n_tiles - 1) P.valid[0] = sh_excl + agg;
   }
+  Real* ev = static_cast<Real*>(P.events);
+  if (direct) {  // one wave: base == 0, the tile's rows are [sh_excl, sh_excl + agg)
+    int r = pre;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      if ((keepbits >> e) & 1) {
+        st_ev[2 * r] = Real(vx[e]);
+        st_ev[2 * r + 1] = Real(vq[e]);
+        st_pk[r] = 4 * q + e;
+        ++r;
+      }
+    }
+    __syncthreads();
+    const int64_t r0 = sh_excl;
+    for (int i = threadIdx.x; i < 2 * agg; i += kThreads) ev[2 * r0 + i] = st_ev[i];
+    for (int i = threadIdx.x; i < agg; i += kThreads) P.packed[r0 + i] = st_pk[i];
+    return;
+  }
   __syncthreads();
   int64_t row = sh_excl + base + pre;
-  Real* ev = static_cast<Real*>(P.events);
 #pragma unroll
   for (int e = 0; e < 4; ++e) {
     if ((keepbits >> e) & 1) {
