@@ -301,6 +301,14 @@ __device__ __forceinline__ int64_t interval_of(const uint32_t* thr, int64_t K, b
   return clamp_interval(count, K);
 }
 
+// n / d for n >= 0, d > 0: a 32-bit division when both fit in 32 bits (every
+// slot and cell index below 2^32), the 64-bit one otherwise.  The 64-bit
+// division is a long VALU sequence and the sampling kernel is issue-bound.
+__device__ __forceinline__ int64_t udiv(int64_t n, int64_t d) {
+  if (((uint64_t(n) | uint64_t(d)) >> 32) == 0) return int64_t(uint32_t(n) / uint32_t(d));
+  return n / d;
+}
+
 // ---------------------------------------------------------------------------
 struct SampleParams {
   loits_hip_shape s;
@@ -369,28 +377,41 @@ __global__ void __launch_bounds__(DIRECT ? kDirectThreads : kTileThreads)
   const int64_t tile = sh_tile;
   if (tile >= n_tiles) return;  // (capacity tiles past the live slots)
   const int64_t q = tile * kThreads + threadIdx.x;  // Philox block of this thread
-  const Words4 wx = philox4x32_10(uint64_t(q), P.sid_x, P.seed);
-  const Words4 wq = philox4x32_10(uint64_t(q), P.sid_q, P.seed);
+  const Words4 wx = philox4x32_10_uniform(uint64_t(q), P.sid_x, P.seed);
+  const Words4 wq = philox4x32_10_uniform(uint64_t(q), P.sid_q, P.seed);
   double vx[4], vq[4];
   int keepbits = 0;
   const int64_t nxc = s.batch * s.ny * s.nx;
   const int64_t p0 = 4 * q;
   // cell g and slot of p0 by one division; the next three slots step from there
-  int64_t g = p0 / P.nmax, slot = p0 - g * P.nmax;
+  int64_t g = udiv(p0, P.nmax), slot = p0 - g * P.nmax;
   int64_t cur_g = -1, count = 0, cxi = 0, cqi = 0, ox = 0, bx = 0, oq = 0, bq = 0;
   int activebits = 0;
   bool mono_x = false, mono_q = false;
   float ufx[4], ufq[4];
   int16_t ix4[4] = {0, 0, 0, 0}, iq4[4] = {0, 0, 0, 0};
+  auto enter_cell = [&](int64_t gg) {  // the curves of cell gg
+    cur_g = gg;
+    count = P.counts[gg];
+    const int64_t b = udiv(gg, s.cells), cell = gg - b * s.cells;
+    const int64_t ix = udiv(cell, s.ny), iy = cell - ix * s.ny;
+    cxi = (b * s.ny + iy) * s.nx + ix;
+    cqi = (b * s.nx + ix) * s.ny + iy;
+    ox = cxi * s.kx;
+    bx = (b * s.nx + ix) * s.kx;
+    oq = cqi * s.kq;
+    bq = (b * s.ny + iy) * s.kq;
+    mono_x = P.mono[cxi] != 0;
+    mono_q = P.mono[nxc + cqi] != 0;
+  };
+  if (p0 < P.total_slots) enter_cell(g);
   // Four active slots of one cell with small curves (the usual case): each
   // threshold is loaded once for the four keys.
   int cnt_x[4] = {-1, -1, -1, -1}, cnt_q[4] = {-1, -1, -1, -1};
   if (s.kx <= kSmallK && s.kq <= kSmallK && p0 + 3 < P.total_slots && slot + 3 < P.nmax &&
-      slot + 3 < P.counts[g]) {
-    const int64_t b = g / s.cells, cell = g - b * s.cells;
-    const int64_t ix = cell / s.ny, iy = cell - ix * s.ny;
-    const uint32_t* tx = P.thr_x + ((b * s.ny + iy) * s.nx + ix) * s.kx;
-    const uint32_t* tq = P.thr_q + ((b * s.nx + ix) * s.ny + iy) * s.kq;
+      slot + 3 < count) {
+    const uint32_t* tx = P.thr_x + ox;
+    const uint32_t* tq = P.thr_q + oq;
 #pragma unroll
     for (int e = 0; e < 4; ++e) cnt_x[e] = cnt_q[e] = 0;
 #pragma unroll
@@ -414,20 +435,7 @@ __global__ void __launch_bounds__(DIRECT ? kDirectThreads : kTileThreads)
     ufx[e] = float(kx) * 0x1.0p-24f;
     ufq[e] = float(kq) * 0x1.0p-24f;
     if (p >= P.total_slots) continue;
-    if (g != cur_g) {
-      cur_g = g;
-      count = P.counts[g];
-      const int64_t b = g / s.cells, cell = g - b * s.cells;
-      const int64_t ix = cell / s.ny, iy = cell - ix * s.ny;
-      cxi = (b * s.ny + iy) * s.nx + ix;
-      cqi = (b * s.nx + ix) * s.ny + iy;
-      ox = cxi * s.kx;
-      bx = (b * s.nx + ix) * s.kx;
-      oq = cqi * s.kq;
-      bq = (b * s.ny + iy) * s.kq;
-      mono_x = P.mono[cxi] != 0;
-      mono_q = P.mono[nxc + cqi] != 0;
-    }
+    if (g != cur_g) enter_cell(g);
     if (slot >= count) continue;
     activebits |= 1 << e;
     // x: cpp/loits_core.cpp:183-196
@@ -484,7 +492,7 @@ __global__ void __launch_bounds__(DIRECT ? kDirectThreads : kTileThreads)
   // suffice (no fence: nothing else written by a tile is read in this kernel).
   if (direct) {
     if (threadIdx.x == 0) {
-      const int64_t p = tile * kSlots, g0 = p / P.nmax, s0 = p - g0 * P.nmax;
+      const int64_t p = tile * kSlots, g0 = udiv(p, P.nmax), s0 = p - g0 * P.nmax;
       const int64_t c0 = P.counts[g0];
       sh_excl = P.row0[g0] + (s0 < c0 ? s0 : c0);
       if (tile == n_tiles - 1) P.valid[0] = P.row0[s.batch * s.cells];
