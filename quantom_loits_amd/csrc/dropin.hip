@@ -933,32 +933,40 @@ __global__ void __launch_bounds__(256) ivjp_cells_kernel(VjpParams P0) {
   __builtin_amdgcn_wave_barrier();
   const int64_t rounds = wave_max_i64(row_lane ? (hi - lo + R - 1) / R : 0);
   const int cl = row_lane ? cr * L : 0;  // first bin lane of this row's cell
-  // the packed slot and cotangents of a round's rows are loaded one round ahead
-  int64_t p = 0;
-  double gx = 0.0, gq = 0.0;
+  // A round's intervals, uniforms and cotangents are loaded one round ahead
+  // and its packed slot two rounds ahead (the gathers at the slot then have
+  // a whole round to arrive).
+  int64_t pn = 0;  // packed slot of the next round's row
+  int jxn = 0, jqn = 0;
+  float uxn = 0.0f, uqn = 0.0f;
+  double gxn = 0.0, gqn = 0.0;
   if (row_lane && lo + ir < hi) {
     const int64_t row = lo + ir;
-    p = P.packed[row];
-    gx = ld<Real>(P.grad, 2 * row);
-    gq = ld<Real>(P.grad, 2 * row + 1);
+    const int64_t p = P.packed[row];
+    jxn = P.interval_x[p];
+    jqn = P.interval_q[p];
+    uxn = P.u_x[p];
+    uqn = P.u_q[p];
+    gxn = ld<Real>(P.grad, 2 * row);
+    gqn = ld<Real>(P.grad, 2 * row + 1);
+    if (row + R < hi) pn = P.packed[row + R];
   }
   double acc = 0.0;
   for (int64_t k = 0; k < rounds; ++k) {
     const int64_t row = lo + k * R + ir;
     const bool live = row_lane && row < hi;
-    int jx = 0, jq = 0;
-    double ux = 0.0, uq = 0.0;
-    if (live) {
-      jx = P.interval_x[p];
-      jq = P.interval_q[p];
-      ux = double(P.u_x[p]);
-      uq = double(P.u_q[p]);
-    }
-    const double gxb = gx, gqb = gq;
+    const int jx = live ? jxn : 0, jq = live ? jqn : 0;
+    const double ux = live ? double(uxn) : 0.0, uq = live ? double(uqn) : 0.0;
+    const double gxb = gxn, gqb = gqn;
     if (row_lane && row + R < hi) {
-      p = P.packed[row + R];
-      gx = ld<Real>(P.grad, 2 * (row + R));
-      gq = ld<Real>(P.grad, 2 * (row + R) + 1);
+      const int64_t p = pn;
+      jxn = P.interval_x[p];
+      jqn = P.interval_q[p];
+      uxn = P.u_x[p];
+      uqn = P.u_q[p];
+      gxn = ld<Real>(P.grad, 2 * (row + R));
+      gqn = ld<Real>(P.grad, 2 * (row + R) + 1);
+      if (row + 2 * R < hi) pn = P.packed[row + 2 * R];
     }
     const int la = cl + jx, qa = cl + kx + jq;  // bin lanes of this row's first nodes
     double upx, ax, bxm, idx, upq, aq, bqm, idq;
