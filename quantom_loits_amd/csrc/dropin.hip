@@ -890,7 +890,7 @@ __global__ void __launch_bounds__(256) ivjp_cells_kernel(VjpParams P0) {
   __shared__ unsigned long long touch[4 * kWave];
   __shared__ int offsets[4 * kWave];
   __shared__ double2 entries[4 * 4 * kWave];
-  __shared__ double nodes[4 * 2 * kWave];
+  __shared__ double nodes[4 * 3 * kWave];
   const loits_hip_shape& s = P.s;
   const int lane = lane_id(), w = threadIdx.x >> 6;
   const int kx = int(s.kx), L = int(s.kx + s.kq), C = kWave / L < 2 ? kWave / L : 2;
@@ -901,8 +901,10 @@ __global__ void __launch_bounds__(256) ivjp_cells_kernel(VjpParams P0) {
   unsigned long long* T = touch + w * kWave;  // rows with value j, then touching bin j
   int* offs = offsets + w * kWave;            // first list entry of bin j
   double2* ent = entries + w * 4 * kWave;     // factor pairs, bin by bin, rows in order
-  double* nc = nodes + w * 2 * kWave;         // C and x (or q) of the node of bin lane j
-  double* nn = nc + kWave;
+  // node of bin lane j: (C_j, 1 / (C_{j+1} - C_j + eps)) and x_{j+1} - x_j,
+  // the factors every row of interval j uses (evaluated once per node)
+  double2* cd = reinterpret_cast<double2*>(nodes + w * 3 * kWave);
+  double* dxs = nodes + w * 3 * kWave + 2 * kWave;
   const int cb = lane / L, bb = lane - cb * L;  // this lane as a bin
   const int cr = lane / R, ir = lane - cr * R;  // this lane as a row
   const bool bin_live = cb < C && g0 + cb < gcells;
@@ -922,13 +924,19 @@ __global__ void __launch_bounds__(256) ivjp_cells_kernel(VjpParams P0) {
   }
   if (bin_live) {
     const CellGeo q = cell_geo(s, g0 + cb);
-    if (bb < kx) {
-      nc[lane] = P.cdf_x[q.ox + bb];
-      nn[lane] = ld<Real>(P.x_bins, q.bx + bb);
-    } else {
-      nc[lane] = P.cdf_q[q.oq + bb - kx];
-      nn[lane] = ld<Real>(P.q_bins, q.bq + bb - kx);
+    const bool xa = bb < kx;
+    const int j = xa ? bb : bb - kx, kk = xa ? kx : int(s.kq);
+    const double* cdf = xa ? P.cdf_x + q.ox : P.cdf_q + q.oq;
+    const void* bins = xa ? P.x_bins : P.q_bins;
+    const int64_t boff = xa ? q.bx : q.bq;
+    const double c0 = cdf[j];
+    double inv_d = 0.0, dx = 0.0;
+    if (j + 1 < kk) {
+      inv_d = 1.0 / (cdf[j + 1] - c0 + kEpsilon);
+      dx = ld<Real>(bins, boff + j + 1) - ld<Real>(bins, boff + j);
     }
+    cd[lane] = make_double2(c0, inv_d);
+    dxs[lane] = dx;
   }
   __builtin_amdgcn_wave_barrier();
   const int64_t rounds = wave_max_i64(row_lane ? (hi - lo + R - 1) / R : 0);
@@ -971,21 +979,19 @@ __global__ void __launch_bounds__(256) ivjp_cells_kernel(VjpParams P0) {
     const int la = cl + jx, qa = cl + kx + jq;  // bin lanes of this row's first nodes
     double upx, ax, bxm, idx, upq, aq, bqm, idq;
     {  // x: the factors of ivjp_row
-      const double c0 = nc[la];
-      const double d = nc[la + 1] - c0 + kEpsilon;
-      const double inv_d = 1.0 / d;
+      const double2 ci = cd[la];
+      const double c0 = ci.x, inv_d = ci.y;
       const double t = ux - c0;
-      upx = gxb * (nn[la + 1] - nn[la]);
+      upx = gxb * dxs[la];
       ax = __builtin_fma(t * inv_d, inv_d, -inv_d);
       bxm = -(upx * t * inv_d);
       idx = inv_d;
     }
     {  // q
-      const double c0 = nc[qa];
-      const double d = nc[qa + 1] - c0 + kEpsilon;
-      const double inv_d = 1.0 / d;
+      const double2 ci = cd[qa];
+      const double c0 = ci.x, inv_d = ci.y;
       const double t = uq - c0;
-      upq = gqb * (nn[qa + 1] - nn[qa]);
+      upq = gqb * dxs[qa];
       aq = __builtin_fma(t * inv_d, inv_d, -inv_d);
       bqm = -(upq * t * inv_d);
       idq = inv_d;
