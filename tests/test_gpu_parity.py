@@ -156,6 +156,41 @@ def test_dropin_uneven_axes_bit_exact_vs_oracle(ext, oracle_port, kx, kq, zero_n
     np.testing.assert_array_equal(gq.cpu().numpy(), gq_ref)
 
 
+@pytest.mark.parametrize("n_events", [1000000, 10000000])
+def test_dropin_gan_shape_bit_exact_vs_oracle(ext, oracle_port, n_events):
+    """The reference's GAN shape on the reference-API path: 99 x 99 cells of
+    10 nodes per axis, float64, a random acceptance mask.  Every curve is
+    drop-free here, so this covers the one-wave sampling tiles whose slots
+    span cell boundaries, rows placed from the cell counts, and the VJP with
+    two cells per wave over about 100 (1e6) or 1000 (1e7) rows per cell."""
+    rng = np.random.default_rng(7)
+    grid, k = 99, 10
+    xb = np.sort(rng.uniform(0.001, 0.999, (1, grid, k)), axis=2)
+    qb = np.sort(rng.uniform(1.0, 100.0, (1, grid, k)), axis=2)
+    xs = rng.uniform(0.1, 1.0, (1, grid, grid, k))
+    qs = rng.uniform(0.1, 1.0, (1, grid, grid, k))
+    w = rng.uniform(0.5, 1.5, (1, grid, grid))
+    w /= w.sum()
+    acc = rng.uniform(0, 1, (1, grid, grid)) < 0.95
+    seed, sequence = 42, 5
+    ref = oracle_port.forward(xb, xs, qb, qs, w, acc, n_events, seed, sequence)
+    assert len(ref["packed"]) == int(ref["active"])  # drop-free
+    print("gan shape", n_events, flush=True)
+    args = _to_dev([xb, xs, qb, qs, w, acc], torch.float64)
+    st = ext.forward(*args, n_events, seed, sequence, False)
+    events, norm_x, norm_q, cdf_x, cdf_q, u_x, u_q, ix, iq, packed, row_offsets = st
+    np.testing.assert_array_equal(packed.cpu().numpy(), ref["packed"])
+    np.testing.assert_array_equal(events.cpu().numpy(), ref["events"])
+    np.testing.assert_array_equal(row_offsets.cpu().numpy(), ref["row_offsets"])
+    g = rng.standard_normal(ref["events"].shape)
+    gx_ref, gq_ref = oracle_port.backward(g, xb, xs, qb, qs, acc, ref)
+    gx, gq = ext.backward(torch.from_numpy(g).cuda(), args[0], args[1], args[2], args[3], args[5],
+                          norm_x, norm_q, cdf_x, cdf_q, u_x, u_q, ix, iq, packed, row_offsets,
+                          False)
+    np.testing.assert_array_equal(gx.cpu().numpy(), gx_ref)
+    np.testing.assert_array_equal(gq.cpu().numpy(), gq_ref)
+
+
 def _batched_inputs(g, dtype):
     x = torch.from_numpy(np.ascontiguousarray(g["x"])).to(dtype).cuda().reshape(-1)
     d = torch.from_numpy(np.ascontiguousarray(g["density"])).to(dtype).cuda().reshape(-1)
