@@ -346,9 +346,11 @@ struct SampleParams {
 // a tile's first row is the count of active slots before it and the
 // work-group index is the tile.  Otherwise tiles are taken from a counter, in
 // the order the decoupled look-back needs.
+// Five waves per SIMD: the register count fits 96 without spilling (98 left
+// four).
 template <typename Real, bool DIRECT>
 __global__ void __launch_bounds__(DIRECT ? kDirectThreads : kTileThreads)
-    sample_kernel(SampleParams P0) {
+    __attribute__((amdgpu_waves_per_eu(5))) sample_kernel(SampleParams P0) {
   constexpr int kThreads = DIRECT ? kDirectThreads : kTileThreads;
   constexpr int64_t kSlots = 4 * kThreads;
   __shared__ int32_t wave_tot[16];
