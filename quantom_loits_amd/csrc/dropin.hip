@@ -838,9 +838,9 @@ __global__ void __launch_bounds__(256) ivjp_wave_kernel(VjpParams P0) {
       double2 fx[4], fq[4];
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        const int kx_ = off_x + i0 + u, kq_ = off_q + i0 + u;
-        fx[u] = ex[kx_ < 2 * kWave ? kx_ : 2 * kWave - 1];
-        fq[u] = eq[kq_ < 2 * kWave ? kq_ : 2 * kWave - 1];
+        // past the end of a list: entry 0, one broadcast address
+        fx[u] = ex[i0 + u < len_x ? off_x + i0 + u : 0];
+        fq[u] = eq[i0 + u < len_q ? off_q + i0 + u : 0];
       }
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
@@ -1026,8 +1026,9 @@ __global__ void __launch_bounds__(256) ivjp_cells_kernel(VjpParams P0) {
       double2 f[4];
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        const int e = off + i0 + u;
-        f[u] = ent[e < 4 * kWave ? e : 4 * kWave - 1];
+        // a lane past its list's end reads entry 0 with the others: one
+        // broadcast address instead of a distinct (unused) entry per lane
+        f[u] = ent[i0 + u < len ? off + i0 + u : 0];
       }
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
