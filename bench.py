@@ -155,7 +155,10 @@ def extra_workloads(dev, steps):
 def dropin_workloads(dev, steps):
     """The reference-API path (LOITS(backend="hip") bindings, both axes) at the
     reference's own GAN shapes: grid g x g cells of K = 10 nodes per axis, one
-    theory instance; forward + backward, the forward's two host reads included."""
+    theory instance; forward + backward, the forward's two host reads included.
+    The cotangent (all ones) is a view of a buffer filled once per shape: at
+    most n_events rows are kept, and refilling 160 MB per step would time
+    the harness, not the path."""
     import torch
     from quantom_loits_amd import load_extension
 
@@ -173,11 +176,13 @@ def dropin_workloads(dev, steps):
         acc = rng.uniform(0, 1, (1, grid, grid)) < 0.95
         args = [torch.from_numpy(a).to(dev) for a in (xb, xs, qb, qs, w)]
         acc_t = torch.from_numpy(acc).to(dev)
+        ones = torch.ones(n_events, 2, dtype=torch.float64, device=dev)
 
         def step():
             st = ext.forward(args[0], args[1], args[2], args[3], args[4], acc_t, n_events, SEED,
                              SEQUENCE, False)
-            g = torch.ones_like(st[0])
+            kept = st[0].shape[0]
+            g = ones[:kept] if kept <= n_events else torch.ones_like(st[0])
             ext.backward(g, args[0], args[1], args[2], args[3], acc_t, *st[1:], False)
             return st[0].shape[0]
 
