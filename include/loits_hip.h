@@ -164,6 +164,11 @@ typedef struct loits_hip_backward_args {
   void* grad_xsec_x;        /* like xsec_x */
   void* grad_xsec_q;        /* like xsec_q */
   void* workspace;          /* loits_hip_backward_workspace_size() bytes */
+  /* Optional, NULL to find them from `packed`: the rows of cell g are
+   * [row_offsets[g], row_offsets[g + 1]) (batch * cells + 1 entries), as the
+   * forward writes them and the reference's VJP reads them
+   * (openmp/loits_core.cpp:307, :341).  Ranges are clamped to [0, nvalid). */
+  const int64_t* row_offsets;
 } loits_hip_backward_args;
 
 int64_t loits_hip_backward_workspace_size(const loits_hip_shape* shape, int64_t nmax);

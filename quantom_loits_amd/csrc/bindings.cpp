@@ -181,7 +181,11 @@ std::vector<at::Tensor> backward(at::Tensor grad_events, at::Tensor x_bins, at::
   // rest = ([row_offsets,] profile_regions [, stream])
   std::vector<py::object> tail;
   for (auto item : rest) tail.push_back(py::reinterpret_borrow<py::object>(item));
-  if (!tail.empty() && py::hasattr(tail.front(), "dtype")) tail.erase(tail.begin());  // row_offsets
+  at::Tensor row_offsets;
+  if (!tail.empty() && py::hasattr(tail.front(), "dtype")) {
+    row_offsets = tail.front().cast<at::Tensor>();
+    tail.erase(tail.begin());
+  }
   const bool profile_regions = !tail.empty() && tail[0].cast<bool>();
   py::object stream_obj = tail.size() > 1 ? tail[1] : py::none();
   RegionGuard total(profile_regions, "loits::backward");
@@ -229,6 +233,11 @@ std::vector<at::Tensor> backward(at::Tensor grad_events, at::Tensor x_bins, at::
     a.grad_xsec_x = grad_x.data_ptr();
     a.grad_xsec_q = grad_q.data_ptr();
     a.workspace = workspace.data_ptr();
+    // the forward's row offsets give each cell's rows without a search
+    if (row_offsets.defined() && row_offsets.is_cuda() &&
+        row_offsets.scalar_type() == at::kLong && row_offsets.is_contiguous() &&
+        row_offsets.numel() == s.batch * s.cells + 1 && row_offsets.device() == xsec_x.device())
+      a.row_offsets = row_offsets.data_ptr<int64_t>();
     check(loits_hip_backward(&a, stream), "loits_hip_backward");
   }
   RegionGuard region(profile_regions, "loits::backward::state_pack");

@@ -633,7 +633,18 @@ struct VjpParams {
   // sync-free form: nmax and nvalid read here
   const int64_t* dev_nmax;
   const int64_t* dev_nvalid;
+  const int64_t* row_offsets;  // optional: rows of cell g (else searched in packed)
 };
+
+// Rows [lo, hi) of cell g from row_offsets, clamped to [0, nvalid).
+__device__ __forceinline__ void cell_rows(const VjpParams& P, int64_t g, int64_t& lo,
+                                          int64_t& hi) {
+  int64_t a = P.row_offsets[g], b = P.row_offsets[g + 1];
+  a = a < 0 ? 0 : (a > P.nvalid ? P.nvalid : a);
+  b = b < a ? a : (b > P.nvalid ? P.nvalid : b);
+  lo = a;
+  hi = b;
+}
 
 __device__ __forceinline__ VjpParams with_device_sizes(const VjpParams& P0) {
   VjpParams P = P0;
@@ -738,10 +749,15 @@ __global__ void __launch_bounds__(256) ivjp_wave_kernel(VjpParams P0) {
   const int64_t ix = cell / s.ny, iy = cell - ix * s.ny;
   const int64_t ox = ((b * s.ny + iy) * s.nx + ix) * s.kx, bx = (b * s.nx + ix) * s.kx;
   const int64_t oq = ((b * s.nx + ix) * s.ny + iy) * s.kq, bq = (b * s.ny + iy) * s.kq;
-  const int64_t key0 = g * P.nmax;
-  const int64_t lo0 = wave_lower_bound(P.packed, 0, P.nvalid, key0);  // rows of this cell: [lo0, lo1)
-  const int64_t end = P.nvalid - lo0 < P.nmax ? P.nvalid : lo0 + P.nmax;  // at most nmax rows
-  const int64_t lo1 = wave_lower_bound(P.packed, lo0, end, key0 + P.nmax);
+  int64_t lo0, lo1;  // rows of this cell: [lo0, lo1)
+  if (P.row_offsets) {
+    cell_rows(P, g, lo0, lo1);
+  } else {
+    const int64_t key0 = g * P.nmax;
+    lo0 = wave_lower_bound(P.packed, 0, P.nvalid, key0);
+    const int64_t end = P.nvalid - lo0 < P.nmax ? P.nvalid : lo0 + P.nmax;  // at most nmax rows
+    lo1 = wave_lower_bound(P.packed, lo0, end, key0 + P.nmax);
+  }
   const int w = threadIdx.x >> 6;
   unsigned long long* mx = masks + w * 2 * kWave;  // rows with interval j, then touching bin j
   unsigned long long* mq = mx + kWave;
@@ -914,15 +930,19 @@ __global__ void __launch_bounds__(256) ivjp_cells_kernel(VjpParams P0) {
   // rows of cell cr: [lo, hi); a cell holds at most nmax rows, so each
   // boundary after the first is searched within nmax of the previous one
   int64_t lo = 0, hi = 0;
-  int64_t a = wave_lower_bound(P.packed, 0, P.nvalid, g0 * P.nmax);
-  for (int c = 0; c < C && g0 + c < gcells; ++c) {
-    const int64_t end = P.nvalid - a < P.nmax ? P.nvalid : a + P.nmax;
-    const int64_t b = wave_lower_bound(P.packed, a, end, (g0 + c + 1) * P.nmax);
-    if (cr == c) {
-      lo = a;
-      hi = b;
+  if (P.row_offsets) {
+    if (row_lane && g0 + cr < gcells) cell_rows(P, g0 + cr, lo, hi);
+  } else {
+    int64_t a = wave_lower_bound(P.packed, 0, P.nvalid, g0 * P.nmax);
+    for (int c = 0; c < C && g0 + c < gcells; ++c) {
+      const int64_t end = P.nvalid - a < P.nmax ? P.nvalid : a + P.nmax;
+      const int64_t b = wave_lower_bound(P.packed, a, end, (g0 + c + 1) * P.nmax);
+      if (cr == c) {
+        lo = a;
+        hi = b;
+      }
+      a = b;
     }
-    a = b;
   }
   if (bin_live) {
     const CellGeo q = cell_geo(s, g0 + cb);
@@ -1347,6 +1367,7 @@ int backward_impl(const loits_hip_backward_args* a, const int64_t* dev_nmax,
               a->cdf_q,  a->u_x,       a->u_q,     a->interval_x,  a->interval_q,
               a->packed, gcdf_x,       gcdf_q,     grho,           a->grad_xsec_x,
               a->grad_xsec_q, dev_nmax, dev_nvalid};
+  V.row_offsets = a->row_offsets;
   const bool f32 = a->dtype == LOITS_HIP_F32;
   const int64_t gcells = s.batch * s.cells;
   if (s.kx + s.kq <= kWave) {
