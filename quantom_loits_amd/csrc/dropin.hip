@@ -6,19 +6,25 @@
 //   curve_kernel       one thread per x- or q-curve: the reference's sequential
 //                      norm / rho / CDF chain (:55-165), integer thresholds and
 //                      per-bin slopes m_j (:193-194)
-//   sample_kernel      one tile of 4 * 256 padded slots per work-group: both
-//                      Philox streams for every padded slot (:39-53,
+//   row_start_kernel   first row of every cell: the scan of the cell counts
+//   sample_kernel      launched twice, one launch doing the work: both Philox
+//                      streams for every padded slot (:39-53,
 //                      cpp/bindings.cpp:79-92), interval + interpolation for
-//                      active slots (:167-237), and the stable compaction
-//                      (:239-266) in a single pass, tiles ordered by a
+//                      active slots (:167-237) and the stable compaction
+//                      (:239-266) in one pass.  When every curve is drop-free
+//                      (no active slot can be dropped) one-wave tiles of 256
+//                      slots place their rows from the cell counts; otherwise
+//                      1024-thread tiles of 4096 slots are ordered by a
 //                      decoupled look-back
-//   row_offsets_kernel openmp/loits_core.cpp:244-262 from the packed slots
-//   ivjp_wave_kernel   curves of at most 64 nodes: one wave per cell, lane j
-//                      owning bin j, which walks only the rows that touch it
-//                      (per-bin row masks of each 64-row batch), in row order
-//                      (the accumulation order of
-//                      openmp/loits_core.cpp:285-352); ivjp_kernel, one
-//                      thread per cell, for larger curves
+//   row_offsets_kernel openmp/loits_core.cpp:244-262 (the cell counts' scan
+//                      when drop-free, else from the packed slots)
+//   ivjp_cells_kernel  kx + kq <= 64: a lane per node of either axis, up to
+//                      two cells per wave; each lane sums only the rows that
+//                      touch its node, in row order (the accumulation order
+//                      of openmp/loits_core.cpp:285-352), from per-node lists
+//                      built per batch of rows; ivjp_wave_kernel (one cell per
+//                      wave, kx, kq <= 64) and ivjp_kernel (one thread per
+//                      cell) for larger curves
 //   curve_vjp_kernel   one thread per curve: CDF and density adjoints (:340-472)
 // The *_bounded entry points read nmax, active and the kept count on the
 // device instead of taking them from the host.
