@@ -67,15 +67,26 @@ __global__ void __launch_bounds__(1024) alloc_kernel(const void* weights, int64_
   __shared__ int64_t smax[16], ssum[16];
   const double scale = double(n_events);
   int64_t mx = 0, sum = 0;
-  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
-    const double v = __builtin_fabs(ld<Real>(weights, i) * scale);
-    int64_t c;
-    if (!(v == v)) c = 0;
-    else if (v >= 4611686018427387904.0) c = int64_t(1) << 62;
-    else c = static_cast<int64_t>(v);
-    counts[i] = c;
-    mx = c > mx ? c : mx;
-    sum += c;
+  constexpr int kU = 4;  // weights loaded per thread at a time (latencies overlap)
+  for (int64_t i0 = threadIdx.x; i0 < n; i0 += int64_t(kU) * blockDim.x) {
+    double v[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const int64_t i = i0 + int64_t(u) * blockDim.x;
+      v[u] = i < n ? __builtin_fabs(ld<Real>(weights, i) * scale) : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const int64_t i = i0 + int64_t(u) * blockDim.x;
+      if (i >= n) continue;
+      int64_t c;
+      if (!(v[u] == v[u])) c = 0;
+      else if (v[u] >= 4611686018427387904.0) c = int64_t(1) << 62;
+      else c = static_cast<int64_t>(v[u]);
+      counts[i] = c;
+      mx = c > mx ? c : mx;
+      sum += c;
+    }
   }
   for (int d = 32; d >= 1; d >>= 1) {
     const int64_t om = __shfl_xor(mx, d, kWave), os = __shfl_xor(sum, d, kWave);
