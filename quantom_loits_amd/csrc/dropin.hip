@@ -137,6 +137,100 @@ __device__ __forceinline__ bool norm_term_fused(int64_t i, int64_t nterms) {
   return nterms <= 3 || ((nterms & 1) && i == nterms - 1);
 }
 
+// The chain of one curve.  R > 0: its k <= R densities and nodes are loaded
+// into registers first (every load in flight at once) and the CDF and slopes
+// stay there for the checks; R == 0 reads them from memory.  The same
+// expressions in the same order either way.
+constexpr int kRegNodes = 16;
+
+template <typename Real, int R>
+__device__ __forceinline__ void curve_chain(const CurveParams& P, int64_t c, const void* xsec,
+                                            int64_t off, const void* bins, int64_t bins_off,
+                                            int64_t k, double acc, double* norm_out, double* cdf,
+                                            uint32_t* thr, double* m) {
+  constexpr int A = R > 0 ? R : 1;
+  double xr[A], br[A], cr[A], mr[A];
+  if constexpr (R > 0) {
+#pragma unroll
+    for (int t = 0; t < R; ++t) {
+      xr[t] = t < k ? ld<Real>(xsec, off + t) : 0.0;
+      br[t] = t < k ? ld<Real>(bins, bins_off + t) : 0.0;
+    }
+  }
+  auto X = [&](int64_t t) -> double {
+    if constexpr (R > 0) return xr[t]; else return ld<Real>(xsec, off + t);
+  };
+  auto Bn = [&](int64_t t) -> double {
+    if constexpr (R > 0) return br[t]; else return ld<Real>(bins, bins_off + t);
+  };
+  auto Cd = [&](int64_t t) -> double {
+    if constexpr (R > 0) return cr[t]; else return cdf[off + t];
+  };
+  auto Ms = [&](int64_t t) -> double {
+    if constexpr (R > 0) return mr[t]; else return m[off + t];
+  };
+  const int64_t kl = R > 0 ? R : k;  // loop bound, a constant with registers
+  // norm: cpp/loits_core.cpp:73-80 (fusion pattern of the reference build)
+  double nrm = 0.0;
+#pragma unroll
+  for (int64_t t = 1; t < kl; ++t) {
+    if (t >= k) break;
+    const double h = 0.5 * (Bn(t) - Bn(t - 1));
+    const double sum = X(t - 1) + X(t);
+    nrm = norm_term_fused(t - 1, k - 1) ? __builtin_fma(h, sum, nrm) : nrm + h * sum;
+  }
+  *norm_out = nrm;
+  const double inv = 1.0 / nrm;
+  // CDF: cpp/loits_core.cpp:130-135
+  double cum = 0.0, rho_prev = X(0) * inv, c_last = 0.0;
+  cdf[off] = 0.0;
+  if constexpr (R > 0) cr[0] = 0.0;
+  uint32_t prev = 0u;
+  bool mono = true;
+  thr[off] = 0u;
+#pragma unroll
+  for (int64_t t = 1; t < kl; ++t) {
+    if (t >= k) break;
+    const double rho = X(t) * inv;
+    cum = __builtin_fma(0.5 * (Bn(t) - Bn(t - 1)), rho_prev + rho, cum);
+    const double cv = cum * acc;
+    cdf[off + t] = cv;
+    if constexpr (R > 0) cr[t] = cv;
+    c_last = cv;
+    const uint32_t tv = threshold_of(cv);
+    thr[off + t] = tv;
+    mono = mono && tv >= prev;
+    prev = tv;
+    rho_prev = rho;
+  }
+  // slopes: cpp/loits_core.cpp:192-194
+#pragma unroll
+  for (int64_t j = 0; j + 1 < kl; ++j) {
+    if (j + 1 >= k) break;
+    const double inv_d = 1.0 / (Cd(j + 1) - Cd(j) + kEpsilon);
+    const double mj = (Bn(j + 1) - Bn(j)) * inv_d;
+    m[off + j] = mj;
+    if constexpr (R > 0) mr[j] = mj;
+  }
+  P.mono[c] = mono ? 1 : 0;
+  // A safe curve gives every slot a finite value in [1e-100, 1e100]: monotone
+  // thresholds put u at or above C_j (T_j <= k means C_j <= u, and T_0 = 0),
+  // ascending nodes make m_j >= 0, so x_j <= fma(m_j, u - C_j, x_j) <=
+  // x_j + m_j.  A cell of two safe curves then keeps every active slot
+  // (cpp/loits_core.cpp:256), and rows follow from the counts alone.
+  const double x0 = Bn(0);
+  bool safe = mono && x0 >= 1e-100 && __builtin_isfinite(c_last);  // c_last = C_{k-1}
+#pragma unroll
+  for (int64_t j = 0; j + 1 < kl; ++j) {
+    if (j + 1 >= k) break;
+    const double xj = Bn(j), xn = Bn(j + 1);
+    const double mj = Ms(j);
+    safe = safe && xn >= xj && __builtin_isfinite(Cd(j)) && __builtin_isfinite(mj) &&
+           xj + mj <= 1e100;
+  }
+  if (!safe) atomicOr(P.unsafe, 1u);
+}
+
 template <typename Real>
 __global__ void __launch_bounds__(256) curve_kernel(CurveParams P) {
   const loits_hip_shape& s = P.s;
@@ -180,53 +274,10 @@ __global__ void __launch_bounds__(256) curve_kernel(CurveParams P) {
   }
   const int64_t off = cc * k;
   const double acc = P.acc[(b * s.nx + ix) * s.ny + iy] ? 1.0 : 0.0;
-  // norm: cpp/loits_core.cpp:73-80 (fusion pattern of the reference build)
-  double nrm = 0.0;
-  for (int64_t t = 1; t < k; ++t) {
-    const double h = 0.5 * (ld<Real>(bins, bins_off + t) - ld<Real>(bins, bins_off + t - 1));
-    const double sum = ld<Real>(xsec, off + t - 1) + ld<Real>(xsec, off + t);
-    nrm = norm_term_fused(t - 1, k - 1) ? __builtin_fma(h, sum, nrm) : nrm + h * sum;
-  }
-  norm[cc] = nrm;
-  const double inv = 1.0 / nrm;
-  // CDF: cpp/loits_core.cpp:130-135
-  double cum = 0.0, rho_prev = ld<Real>(xsec, off) * inv;
-  cdf[off] = 0.0;
-  uint32_t prev = 0u;
-  bool mono = true;
-  thr[off] = 0u;
-  for (int64_t t = 1; t < k; ++t) {
-    const double rho = ld<Real>(xsec, off + t) * inv;
-    cum = __builtin_fma(0.5 * (ld<Real>(bins, bins_off + t) - ld<Real>(bins, bins_off + t - 1)),
-                        rho_prev + rho, cum);
-    const double cv = cum * acc;
-    cdf[off + t] = cv;
-    const uint32_t tv = threshold_of(cv);
-    thr[off + t] = tv;
-    mono = mono && tv >= prev;
-    prev = tv;
-    rho_prev = rho;
-  }
-  // slopes: cpp/loits_core.cpp:192-194
-  for (int64_t j = 0; j + 1 < k; ++j) {
-    const double inv_d = 1.0 / (cdf[off + j + 1] - cdf[off + j] + kEpsilon);
-    m[off + j] = (ld<Real>(bins, bins_off + j + 1) - ld<Real>(bins, bins_off + j)) * inv_d;
-  }
-  P.mono[c] = mono ? 1 : 0;
-  // A safe curve gives every slot a finite value in [1e-100, 1e100]: monotone
-  // thresholds put u at or above C_j (T_j <= k means C_j <= u, and T_0 = 0),
-  // ascending nodes make m_j >= 0, so x_j <= fma(m_j, u - C_j, x_j) <=
-  // x_j + m_j.  A cell of two safe curves then keeps every active slot
-  // (cpp/loits_core.cpp:256), and rows follow from the counts alone.
-  const double x0 = ld<Real>(bins, bins_off);
-  bool safe = mono && x0 >= 1e-100 && __builtin_isfinite(cdf[off + k - 1]);
-  for (int64_t j = 0; j + 1 < k; ++j) {
-    const double xj = ld<Real>(bins, bins_off + j), xn = ld<Real>(bins, bins_off + j + 1);
-    const double mj = m[off + j];
-    safe = safe && xn >= xj && __builtin_isfinite(cdf[off + j]) && __builtin_isfinite(mj) &&
-           xj + mj <= 1e100;
-  }
-  if (!safe) atomicOr(P.unsafe, 1u);
+  if (k <= kRegNodes)
+    curve_chain<Real, kRegNodes>(P, c, xsec, off, bins, bins_off, k, acc, norm + cc, cdf, thr, m);
+  else
+    curve_chain<Real, 0>(P, c, xsec, off, bins, bins_off, k, acc, norm + cc, cdf, thr, m);
 }
 
 // row0[g] = active slots of the cells before g (counts summed in cell
