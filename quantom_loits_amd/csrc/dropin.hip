@@ -1133,6 +1133,75 @@ __global__ void __launch_bounds__(256) ivjp_cells_kernel(VjpParams P0) {
   }
 }
 
+// CDF and density adjoints of one curve.  R > 0: the curve's k <= R values
+// are loaded into registers first and the per-node gradient stays there
+// (gr_mem is not touched); R == 0 keeps it in gr_mem.  The same expressions
+// in the same order either way.
+template <typename Real, int R>
+__device__ __forceinline__ void curve_vjp_chain(const double* gc, const void* bins,
+                                                int64_t bins_off, const void* xsec, int64_t off,
+                                                int64_t k, double acc, double n, double* gr_mem,
+                                                Real* gx) {
+  constexpr int A = R > 0 ? R : 1;
+  double gcr[A], br[A], xr[A], grr[A];
+  if constexpr (R > 0) {
+#pragma unroll
+    for (int t = 0; t < R; ++t) {
+      gcr[t] = t < k ? gc[t] : 0.0;
+      br[t] = t < k ? ld<Real>(bins, bins_off + t) : 0.0;
+      xr[t] = t < k ? ld<Real>(xsec, off + t) : 0.0;
+      grr[t] = 0.0;
+    }
+  } else {
+    for (int64_t t = 0; t < k; ++t) gr_mem[t] = 0.0;
+  }
+  auto GC = [&](int64_t t) -> double {
+    if constexpr (R > 0) return gcr[t]; else return gc[t];
+  };
+  auto Bn = [&](int64_t t) -> double {
+    if constexpr (R > 0) return br[t]; else return ld<Real>(bins, bins_off + t);
+  };
+  auto X = [&](int64_t t) -> double {
+    if constexpr (R > 0) return xr[t]; else return ld<Real>(xsec, off + t);
+  };
+  auto GR = [&](int64_t t) -> double& {
+    if constexpr (R > 0) return grr[t]; else return gr_mem[t];
+  };
+  const int64_t kl = R > 0 ? R : k;  // loop bound, a constant with registers
+  // CDF adjoint: cpp/loits_core.cpp:355-365
+  double suffix = 0.0;
+#pragma unroll
+  for (int64_t t = kl - 1; t > 0; --t) {
+    if (t > k - 1) continue;
+    suffix += GC(t);
+    const double h = 0.5 * (Bn(t) - Bn(t - 1)) * acc;
+    GR(t - 1) = __builtin_fma(h, suffix, GR(t - 1));
+    GR(t) = __builtin_fma(h, suffix, GR(t));
+  }
+  // density adjoint: cpp/loits_core.cpp:416-431 (grad_norm fusion pattern of
+  // the reference build: only an odd trailing term is fused)
+  const double inv_n = 1.0 / n;
+  const double inv_n2 = inv_n * inv_n;
+  double grad_norm = 0.0;
+#pragma unroll
+  for (int64_t t = 0; t < kl; ++t) {
+    if (t >= k) break;
+    const double term = GR(t) * X(t);
+    grad_norm = ((k & 1) && t == k - 1) ? __builtin_fma(-term, inv_n2, grad_norm)
+                                        : grad_norm - term * inv_n2;
+  }
+  double prev = GR(0) * inv_n;
+#pragma unroll
+  for (int64_t t = 1; t < kl; ++t) {
+    if (t >= k) break;
+    const double h = 0.5 * (Bn(t) - Bn(t - 1));
+    const double cur = GR(t) * inv_n;
+    gx[t - 1] = Real(__builtin_fma(h, grad_norm, prev));
+    prev = __builtin_fma(h, grad_norm, cur);
+  }
+  gx[k - 1] = Real(prev);
+}
+
 template <typename Real>
 __global__ void __launch_bounds__(256) curve_vjp_kernel(VjpParams P) {
   const loits_hip_shape& s = P.s;
@@ -1175,35 +1244,11 @@ __global__ void __launch_bounds__(256) curve_vjp_kernel(VjpParams P) {
   }
   const int64_t off = cc * k;
   const double acc = P.acc[(b * s.nx + ix) * s.ny + iy] ? 1.0 : 0.0;
-  // CDF adjoint: cpp/loits_core.cpp:355-365
-  for (int64_t t = 0; t < k; ++t) gr[t] = 0.0;
-  double suffix = 0.0;
-  for (int64_t t = k - 1; t > 0; --t) {
-    suffix += gc[t];
-    const double h = 0.5 * (ld<Real>(bins, bins_off + t) - ld<Real>(bins, bins_off + t - 1)) * acc;
-    gr[t - 1] = __builtin_fma(h, suffix, gr[t - 1]);
-    gr[t] = __builtin_fma(h, suffix, gr[t]);
-  }
-  // density adjoint: cpp/loits_core.cpp:416-431 (grad_norm fusion pattern of
-  // the reference build: only an odd trailing term is fused)
-  const double n = norm[cc];
-  const double inv_n = 1.0 / n;
-  const double inv_n2 = inv_n * inv_n;
-  double grad_norm = 0.0;
   Real* gx = static_cast<Real*>(out) + off;
-  for (int64_t t = 0; t < k; ++t) {
-    const double term = gr[t] * ld<Real>(xsec, off + t);
-    grad_norm = ((k & 1) && t == k - 1) ? __builtin_fma(-term, inv_n2, grad_norm)
-                                        : grad_norm - term * inv_n2;
-  }
-  double prev = gr[0] * inv_n;
-  for (int64_t t = 1; t < k; ++t) {
-    const double h = 0.5 * (ld<Real>(bins, bins_off + t) - ld<Real>(bins, bins_off + t - 1));
-    const double cur = gr[t] * inv_n;
-    gx[t - 1] = Real(__builtin_fma(h, grad_norm, prev));
-    prev = __builtin_fma(h, grad_norm, cur);
-  }
-  gx[k - 1] = Real(prev);
+  if (k <= kRegNodes)
+    curve_vjp_chain<Real, kRegNodes>(gc, bins, bins_off, xsec, off, k, acc, norm[cc], gr, gx);
+  else
+    curve_vjp_chain<Real, 0>(gc, bins, bins_off, xsec, off, k, acc, norm[cc], gr, gx);
 }
 
 int64_t align16(int64_t v) { return (v + 15) & ~int64_t(15); }
