@@ -236,8 +236,9 @@ def cpu_baseline():
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 "
-                                   "(the batched path shards by instances, see DESIGN.md)"}))
+        print(json.dumps({"error": "the benchmark runs on one GPU; --gpus must be 1 "
+                                   "(across GPUs the batched path would shard by instances, "
+                                   "whole densities per GPU)"}))
         raise SystemExit(2)
     import torch
 
