@@ -142,6 +142,23 @@ int loits_hip_forward(const loits_hip_forward_args* args, void* stream);
 int loits_hip_forward_bounded(const loits_hip_forward_args* args, const int64_t* nmax_active,
                               void* stream);
 
+/* Bytes of the device scratch array behind loits_hip_forward_noalloc (one
+ * copy per GPU the library runs on, held until the process ends). */
+#define LOITS_HIP_NOALLOC_SCRATCH_BYTES (16 * 1024 * 1024)
+
+/* Allocation-free form of loits_hip_forward: identical outputs, but the
+ * workspace is the library's own static device scratch
+ * (LOITS_HIP_NOALLOC_SCRATCH_BYTES per GPU), so the call makes no
+ * hipMalloc/hipFree and no allocating library call.  args->workspace must be
+ * NULL; a shape whose workspace (loits_hip_forward_workspace_size) exceeds
+ * the scratch is rejected with LOITS_HIP_EINVAL.  Every call on one GPU
+ * shares that GPU's scratch and clears what must start at zero itself, so
+ * calls that overlap on one GPU (concurrent streams or threads) race and must
+ * be serialised by the caller.  No reference counterpart: it is for callers
+ * that cannot allocate (HIP graph capture, tight serving loops, harnesses that
+ * forbid allocation). */
+int loits_hip_forward_noalloc(const loits_hip_forward_args* args, void* stream);
+
 typedef struct loits_hip_backward_args {
   loits_hip_shape shape;
   int dtype;

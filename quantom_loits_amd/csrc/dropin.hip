@@ -33,6 +33,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <mutex>
 #include <string>
 
 #include "capi_internal.hpp"
@@ -1440,6 +1441,81 @@ int loits_hip_forward_bounded(const loits_hip_forward_args* a, const int64_t* nm
                               void* stream) {
   if (!nmax_active) return fail(LOITS_HIP_EINVAL, "forward_bounded: nmax_active is NULL");
   return forward_impl(a, nmax_active, stream);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Allocation-free forward.  Its workspace is `scratch`, a file-scope device
+// array: every GPU that loads the library holds one copy until the process
+// ends.  Host code never passes `scratch` itself to a kernel (on the host the
+// name is only a stand-in for the symbol); the device address of the current
+// GPU's copy comes from hipGetSymbolAddress, once per device, and is kept.
+namespace qloits {
+namespace {
+
+constexpr int64_t kNoAllocScratchBytes = LOITS_HIP_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[kMaxScratchDevices] = {};
+
+// Device address of `scratch` on the GPU that runs `st` (the current device
+// for the legacy default stream).
+int scratch_address(hipStream_t st, void** out) {
+  int dev = -1;
+  int rc = check_hip(st ? hipStreamGetDevice(st, &dev) : hipGetDevice(&dev),
+                     "forward_noalloc: device of the stream");
+  if (rc != LOITS_HIP_OK) return rc;
+  if (dev < 0 || dev >= kMaxScratchDevices)
+    return fail(LOITS_HIP_EUNSUPPORTED, "forward_noalloc: device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    int cur = -1;
+    rc = check_hip(hipGetDevice(&cur), "forward_noalloc: hipGetDevice");
+    if (rc != LOITS_HIP_OK) return rc;
+    if (cur != dev) {
+      rc = check_hip(hipSetDevice(dev), "forward_noalloc: hipSetDevice");
+      if (rc != LOITS_HIP_OK) return rc;
+    }
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != dev) (void)hipSetDevice(cur);
+    rc = check_hip(e, "forward_noalloc: hipGetSymbolAddress(scratch)");
+    if (rc != LOITS_HIP_OK) return rc;
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return LOITS_HIP_OK;
+}
+
+}  // namespace
+}  // namespace qloits
+
+extern "C" {
+
+int loits_hip_forward_noalloc(const loits_hip_forward_args* a, void* stream) {
+  if (!a) return fail(LOITS_HIP_EINVAL, "forward_noalloc: args is NULL");
+  if (a->workspace)
+    return fail(LOITS_HIP_EINVAL,
+                "forward_noalloc: workspace must be NULL (the library's own scratch is used)");
+  int rc = check_shape(a->shape);
+  if (rc != LOITS_HIP_OK) return rc;
+  if (a->nmax < 0) return fail(LOITS_HIP_EINVAL, "forward_noalloc: bad nmax");
+  const int64_t need = fwd_workspace(a->shape, a->nmax, nullptr).bytes;
+  if (need > kNoAllocScratchBytes)
+    return fail(LOITS_HIP_EINVAL,
+                "forward_noalloc: this shape needs " + std::to_string(need) +
+                    " bytes of workspace, more than the " +
+                    std::to_string(kNoAllocScratchBytes) +
+                    "-byte scratch of the allocation-free entry point; use loits_hip_forward");
+  void* ws = nullptr;
+  rc = scratch_address(static_cast<hipStream_t>(stream), &ws);
+  if (rc != LOITS_HIP_OK) return rc;
+  loits_hip_forward_args b = *a;
+  b.workspace = ws;
+  return forward_impl(&b, nullptr, stream);
 }
 
 int64_t loits_hip_backward_workspace_size(const loits_hip_shape* shape, int64_t nmax) {
