@@ -37,6 +37,11 @@ void check(int status, const char* what) {
     throw std::runtime_error(std::string(what) + ": " + loits_hip_last_error());
 }
 
+void require_device(const at::Tensor& t, const char* name) {
+  if (!t.is_cuda()) throw std::invalid_argument(std::string(name) + " must be a GPU tensor");
+  if (!t.is_contiguous()) throw std::invalid_argument(std::string(name) + " must be contiguous");
+}
+
 void* stream_of(const py::object& stream, const at::Tensor& like) {
   if (!stream.is_none()) return reinterpret_cast<void*>(stream.cast<uintptr_t>());
   py::object cuda = py::module_::import("torch").attr("cuda");
@@ -44,9 +49,48 @@ void* stream_of(const py::object& stream, const at::Tensor& like) {
   return reinterpret_cast<void*>(cuda.attr("current_stream")(dev).attr("cuda_stream").cast<uintptr_t>());
 }
 
-void require_device(const at::Tensor& t, const char* name) {
-  if (!t.is_cuda()) throw std::invalid_argument(std::string(name) + " must be a GPU tensor");
-  if (!t.is_contiguous()) throw std::invalid_argument(std::string(name) + " must be contiguous");
+// Makes an explicitly passed HIP stream torch's current stream for the
+// lifetime of the object, so that the temporaries a binding allocates are
+// ordered on that stream by the caching allocator and the host reads of
+// sizes (.cpu()) wait for the kernels that write them.  No-op for None.
+class StreamScope {
+ public:
+  StreamScope(const py::object& stream, const at::Tensor& like) {
+    if (stream.is_none()) return;
+    py::object cuda = py::module_::import("torch").attr("cuda");
+    py::object ext = cuda.attr("ExternalStream")(stream.cast<uintptr_t>(),
+                                                 py::arg("device") = like.device().index());
+    ctx_ = cuda.attr("stream")(ext);
+    ctx_.attr("__enter__")();
+  }
+  ~StreamScope() {
+    if (ctx_) ctx_.attr("__exit__")(py::none(), py::none(), py::none());
+  }
+  StreamScope(const StreamScope&) = delete;
+  StreamScope& operator=(const StreamScope&) = delete;
+
+ private:
+  py::object ctx_;
+};
+
+// Throws std::invalid_argument (ValueError in Python) unless `t` is a
+// contiguous GPU tensor of dtype `dt` with `numel` elements (numel < 0: any).
+void require_tensor(const at::Tensor& t, const char* name, at::ScalarType dt, int64_t numel) {
+  require_device(t, name);
+  if (t.scalar_type() != dt)
+    throw std::invalid_argument(std::string(name) + " has dtype " +
+                                std::string(c10::toString(t.scalar_type())) + ", expected " +
+                                std::string(c10::toString(dt)));
+  if (numel >= 0 && t.numel() != numel)
+    throw std::invalid_argument(std::string(name) + " has " + std::to_string(t.numel()) +
+                                " elements, expected " + std::to_string(numel));
+}
+
+void require_min(const at::Tensor& t, const char* name, int64_t numel) {
+  if (t.numel() < numel)
+    throw std::invalid_argument(std::string(name) + " has " + std::to_string(t.numel()) +
+                                " elements, fewer than the " + std::to_string(numel) +
+                                " required");
 }
 
 int dtype_code(const at::Tensor& t, const char* name) {
@@ -80,6 +124,10 @@ std::vector<at::Tensor> forward(at::Tensor x_bins, at::Tensor xsec_x, at::Tensor
   if (xsec_x.numel() != s.batch * s.ny * s.nx * s.kx || xsec_q.numel() != s.batch * s.nx * s.ny * s.kq ||
       weights.numel() != s.batch * s.cells || acceptance.numel() != s.batch * s.cells)
     throw std::invalid_argument("theory output shapes do not match x_bins / q_bins");
+  for (auto* p : {&x_bins, &q_bins, &xsec_q, &weights, &acceptance})
+    if (p->device() != xsec_x.device())
+      throw std::invalid_argument("all theory outputs must be on one GPU");
+  StreamScope scope(stream_obj, xsec_x);
   void* stream = stream_of(stream_obj, xsec_x);
   auto i64 = xsec_x.options().dtype(at::kLong);
   auto f64 = xsec_x.options().dtype(at::kDouble);
@@ -194,6 +242,33 @@ std::vector<at::Tensor> backward(at::Tensor grad_events, at::Tensor x_bins, at::
   const int dt = dtype_code(xsec_x, "xsec_x");
   if (grad_events.scalar_type() != xsec_x.scalar_type()) grad_events = grad_events.to(xsec_x.scalar_type());
   const loits_hip_shape s = shape_of(x_bins, q_bins);
+  const int64_t gcells = s.batch * s.cells;
+  const at::ScalarType rt = xsec_x.scalar_type();
+  require_tensor(x_bins, "x_bins", rt, -1);
+  require_tensor(q_bins, "q_bins", rt, -1);
+  require_tensor(xsec_x, "xsec_x", rt, s.batch * s.ny * s.nx * s.kx);
+  require_tensor(xsec_q, "xsec_q", rt, s.batch * s.nx * s.ny * s.kq);
+  if (acceptance.scalar_type() != at::kBool || acceptance.numel() != gcells)
+    throw std::invalid_argument("acceptance must be a bool tensor [B, nx, ny]");
+  require_tensor(norm_x, "norm_x", at::kDouble, gcells);
+  require_tensor(norm_q, "norm_q", at::kDouble, gcells);
+  require_tensor(cdf_x, "cdf_x", at::kDouble, xsec_x.numel());
+  require_tensor(cdf_q, "cdf_q", at::kDouble, xsec_q.numel());
+  if (u_x.dim() != 3 || u_x.size(0) != s.batch || u_x.size(1) != s.cells)
+    throw std::invalid_argument("u_x must be [B, cells, nmax]");
+  const int64_t slots = u_x.numel();
+  require_tensor(u_x, "u_x", at::kFloat, slots);
+  require_tensor(u_q, "u_q", at::kFloat, slots);
+  require_tensor(interval_x, "interval_x", at::kShort, slots);
+  require_tensor(interval_q, "interval_q", at::kShort, slots);
+  require_tensor(packed, "packed", at::kLong, -1);
+  if (grad_events.dim() != 2 || grad_events.size(1) != 2 || grad_events.size(0) < packed.size(0))
+    throw std::invalid_argument("grad_events must be [n, 2] with a row for every packed event");
+  for (const at::Tensor* p : {&grad_events, &x_bins, &q_bins, &xsec_q, &norm_x, &norm_q, &cdf_x,
+                              &cdf_q, &u_x, &u_q, &interval_x, &interval_q, &packed})
+    if (p->device() != xsec_x.device())
+      throw std::invalid_argument("backward: every tensor must be on xsec_x's GPU");
+  StreamScope scope(stream_obj, xsec_x);
   void* stream = stream_of(stream_obj, xsec_x);
   at::Tensor grad_x = at::empty(xsec_x.sizes(), xsec_x.options());
   at::Tensor grad_q = at::empty(xsec_q.sizes(), xsec_q.options());
@@ -262,6 +337,34 @@ void forward_bounded(at::Tensor x_bins, at::Tensor xsec_x, at::Tensor q_bins, at
   const int dt = dtype_code(xsec_x, "xsec_x");
   const loits_hip_shape s = shape_of(x_bins, q_bins);
   const int64_t gcells = s.batch * s.cells;
+  const at::ScalarType rt = xsec_x.scalar_type();
+  require_tensor(x_bins, "x_bins", rt, -1);
+  require_tensor(q_bins, "q_bins", rt, -1);
+  require_tensor(xsec_x, "xsec_x", rt, s.batch * s.ny * s.nx * s.kx);
+  require_tensor(xsec_q, "xsec_q", rt, s.batch * s.nx * s.ny * s.kq);
+  require_tensor(weights, "weights", rt, gcells);
+  require_tensor(events, "events", rt, -1);
+  if (acceptance.scalar_type() != at::kBool || acceptance.numel() != gcells)
+    throw std::invalid_argument("acceptance must be a bool tensor [B, nx, ny]");
+  require_tensor(norm_x, "norm_x", at::kDouble, gcells);
+  require_tensor(norm_q, "norm_q", at::kDouble, gcells);
+  require_tensor(cdf_x, "cdf_x", at::kDouble, xsec_x.numel());
+  require_tensor(cdf_q, "cdf_q", at::kDouble, xsec_q.numel());
+  require_tensor(counts, "counts", at::kLong, -1);
+  require_tensor(na, "na", at::kLong, -1);
+  require_tensor(u_x, "u_x", at::kFloat, -1);
+  require_tensor(u_q, "u_q", at::kFloat, -1);
+  require_tensor(interval_x, "interval_x", at::kShort, -1);
+  require_tensor(interval_q, "interval_q", at::kShort, -1);
+  require_tensor(packed, "packed", at::kLong, -1);
+  require_tensor(row_offsets, "row_offsets", at::kLong, -1);
+  require_tensor(workspace, "workspace", at::kByte, -1);
+  require_tensor(valid, "valid", at::kLong, -1);
+  for (const at::Tensor* p : {&x_bins, &q_bins, &xsec_q, &weights, &acceptance, &counts, &na,
+                              &norm_x, &norm_q, &cdf_x, &cdf_q, &u_x, &u_q, &interval_x,
+                              &interval_q, &events, &packed, &row_offsets, &workspace, &valid})
+    if (p->device() != xsec_x.device())
+      throw std::invalid_argument("bounded forward: every tensor must be on xsec_x's GPU");
   const int64_t nmax_cap = gcells > 0 ? u_x.numel() / gcells : 0;
   const int64_t active_cap = packed.numel();
   if (u_q.numel() < gcells * nmax_cap || interval_x.numel() < gcells * nmax_cap ||
@@ -323,6 +426,36 @@ void backward_bounded(at::Tensor grad_events, at::Tensor x_bins, at::Tensor xsec
     throw std::invalid_argument("bounded backward: gradients must have xsec_x's dtype");
   const loits_hip_shape s = shape_of(x_bins, q_bins);
   const int64_t gcells = s.batch * s.cells;
+  const at::ScalarType rt = xsec_x.scalar_type();
+  require_tensor(x_bins, "x_bins", rt, -1);
+  require_tensor(q_bins, "q_bins", rt, -1);
+  require_tensor(xsec_x, "xsec_x", rt, s.batch * s.ny * s.nx * s.kx);
+  require_tensor(xsec_q, "xsec_q", rt, s.batch * s.nx * s.ny * s.kq);
+  if (acceptance.scalar_type() != at::kBool || acceptance.numel() != gcells)
+    throw std::invalid_argument("acceptance must be a bool tensor [B, nx, ny]");
+  require_tensor(norm_x, "norm_x", at::kDouble, gcells);
+  require_tensor(norm_q, "norm_q", at::kDouble, gcells);
+  require_tensor(cdf_x, "cdf_x", at::kDouble, xsec_x.numel());
+  require_tensor(cdf_q, "cdf_q", at::kDouble, xsec_q.numel());
+  const int64_t nmax_cap = gcells > 0 ? u_x.numel() / gcells : 0;
+  require_tensor(u_x, "u_x", at::kFloat, -1);
+  require_tensor(u_q, "u_q", at::kFloat, -1);
+  require_tensor(interval_x, "interval_x", at::kShort, -1);
+  require_tensor(interval_q, "interval_q", at::kShort, -1);
+  require_min(u_q, "u_q", gcells * nmax_cap);
+  require_min(interval_x, "interval_x", gcells * nmax_cap);
+  require_min(interval_q, "interval_q", gcells * nmax_cap);
+  require_tensor(packed, "packed", at::kLong, -1);
+  require_tensor(na, "na", at::kLong, -1);
+  require_tensor(valid, "valid", at::kLong, -1);
+  require_min(na, "na", 2);
+  require_min(valid, "valid", 2);
+  require_tensor(workspace, "workspace", at::kByte, -1);
+  for (const at::Tensor* p : {&grad_events, &x_bins, &q_bins, &xsec_q, &acceptance, &norm_x,
+                              &norm_q, &cdf_x, &cdf_q, &u_x, &u_q, &interval_x, &interval_q,
+                              &packed, &na, &valid, &grad_x, &grad_q, &workspace})
+    if (p->device() != xsec_x.device())
+      throw std::invalid_argument("bounded backward: every tensor must be on xsec_x's GPU");
   if (grad_events.numel() < 2 * packed.numel() || grad_x.numel() != xsec_x.numel() ||
       grad_q.numel() != xsec_q.numel())
     throw std::invalid_argument("bounded backward: gradient buffers have the wrong size");
@@ -360,6 +493,8 @@ void backward_bounded(at::Tensor grad_events, at::Tensor x_bins, at::Tensor xsec
 
 at::Tensor philox_fill(int64_t count, uint64_t seed, uint64_t stream_id, int64_t device,
                        py::object stream_obj) {
+  at::Tensor probe = at::empty({0}, at::TensorOptions().dtype(at::kFloat).device(at::kCUDA, device));
+  StreamScope scope(stream_obj, probe);
   at::Tensor out = at::empty({count}, at::TensorOptions().dtype(at::kFloat).device(at::kCUDA, device));
   check(loits_hip_philox_fill(out.data_ptr<float>(), count, seed, stream_id,
                               stream_of(stream_obj, out)),
@@ -369,6 +504,8 @@ at::Tensor philox_fill(int64_t count, uint64_t seed, uint64_t stream_id, int64_t
 
 at::Tensor philox_words(int64_t first_block, int64_t blocks, uint64_t stream_id, uint64_t seed,
                         int64_t device, py::object stream_obj) {
+  at::Tensor probe = at::empty({0}, at::TensorOptions().dtype(at::kInt).device(at::kCUDA, device));
+  StreamScope scope(stream_obj, probe);
   at::Tensor out = at::empty({blocks * 4}, at::TensorOptions().dtype(at::kInt).device(at::kCUDA, device));
   check(loits_hip_philox_words(reinterpret_cast<uint32_t*>(out.data_ptr<int32_t>()), first_block,
                                blocks, stream_id, seed, stream_of(stream_obj, out)),
@@ -384,6 +521,7 @@ at::Tensor soft_histogram(at::Tensor values, at::Tensor row_start, double lo, do
   if (values.scalar_type() != at::kFloat || row_start.scalar_type() != at::kLong)
     throw std::invalid_argument("values must be float32 and row_start int64");
   const int64_t n = row_start.numel() - 1;
+  StreamScope scope(stream_obj, values);
   at::Tensor hist = at::empty({n > 0 ? n : 0, bins}, values.options());
   check(loits_hip_soft_histogram(values.data_ptr<float>(), row_start.data_ptr<int64_t>(), n,
                                  values.numel(), float(lo), float(hi), int(bins), float(width),
@@ -403,6 +541,7 @@ at::Tensor soft_histogram_backward(at::Tensor values, at::Tensor row_start, doub
     throw std::invalid_argument("values and grad_hist must be float32, row_start int64");
   const int64_t n = row_start.numel() - 1;
   if (grad_hist.numel() != n * bins) throw std::invalid_argument("grad_hist must be [n, bins]");
+  StreamScope scope(stream_obj, values);
   at::Tensor grad = at::empty_like(values);
   check(loits_hip_soft_histogram_backward(values.data_ptr<float>(), row_start.data_ptr<int64_t>(),
                                           n, values.numel(), float(lo), float(hi), int(bins),

@@ -45,6 +45,21 @@ __device__ int64_t instance_of(const int64_t* row_start, int64_t n, int64_t r) {
   return a;
 }
 
+// Bin position t of a value and the first bin c0 = floor(t) of its taps.
+// NaN values (and so NaN events) take no part: false.  t is clamped to
+// [-(kTaps + 1), bins + kTaps + 1] first, so the float-to-int conversion is
+// defined for every finite or infinite t; a value that far outside has no tap
+// on a bin either way.
+__device__ __forceinline__ bool tap_origin(float v, const HistParams& P, float* t, int* c0) {
+  float u = (v - P.lo) * P.inv_bw;
+  if (u != u) return false;
+  const float lim_lo = -float(kTaps + 1), lim_hi = float(P.bins + kTaps + 1);
+  u = u < lim_lo ? lim_lo : (u > lim_hi ? lim_hi : u);
+  *t = u;
+  *c0 = int(__builtin_floorf(u));
+  return true;
+}
+
 __global__ void zero_hist(float* h, int64_t n) {
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i < n) h[i] = 0.0f;
@@ -66,8 +81,9 @@ __global__ void __launch_bounds__(kHistThreads) hist_kernel(HistParams P) {
   int64_t inst = inst0;
   for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
     while (inst + 1 < P.n_instances && P.row_start[inst + 1] <= r) ++inst;
-    const float t = (P.values[r] - P.lo) * P.inv_bw;
-    const int c0 = int(__builtin_floorf(t));
+    float t;
+    int c0;
+    if (!tap_origin(P.values[r], P, &t, &c0)) continue;
 #pragma unroll
     for (int k = -kTaps; k <= kTaps; ++k) {
       const int c = c0 + k;
@@ -99,8 +115,12 @@ __global__ void __launch_bounds__(kHistThreads) hist_backward_kernel(HistParams 
   int64_t inst = sh_inst0;
   for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
     while (inst + 1 < P.n_instances && P.row_start[inst + 1] <= r) ++inst;
-    const float t = (P.values[r] - P.lo) * P.inv_bw;
-    const int c0 = int(__builtin_floorf(t));
+    float t;
+    int c0;
+    if (!tap_origin(P.values[r], P, &t, &c0)) {
+      P.grad_values[r] = 0.0f;
+      continue;
+    }
     const float* gh = P.grad_hist + inst * P.bins;
     float acc = 0.0f;
 #pragma unroll
