@@ -231,6 +231,10 @@ typedef struct loits_hip_tuning {
   int64_t chunk_events;     /* events per work-group (rounded up to 4*threads) */
   int32_t scan_mode;        /* 0 = parallel CDF scan, 1 = the reference's sequential chain */
   int32_t persistent;       /* reserved, must be 0 */
+  int32_t table_format;     /* 0 = cell tables for float32 curves of <= 4096 nodes when
+                               chunk_events <= 2^18 (guide tables otherwise), 1 = guide
+                               tables for every curve */
+  int32_t reserved;         /* must be 0 */
 } loits_hip_tuning;
 
 typedef struct loits_hip_layout {
@@ -252,8 +256,13 @@ typedef struct loits_hip_layout {
   int32_t range_instances;  /* instances whose VJP splits the bins into ranges of at
                                most 4096 (tables and histograms too large for LDS);
                                grid_backward counts one work item per chunk and range */
-  int64_t backward_plain;   /* work items of the instances without ranges (the first
-                               launch of the VJP; the ranged items follow) */
+  int32_t cell_instances;   /* instances on cell tables (table_format 0) */
+  int32_t reserved;
+  int64_t backward_plain;   /* work items of the instances without ranges (the cell-table
+                               items, then the guide-table items; the ranged items follow) */
+  int64_t cell_chunks;      /* forward work-groups of the cell-table instances (their own
+                               launch; the other chunks take a second one) */
+  int64_t backward_cell;    /* VJP work items of the cell-table instances */
 } loits_hip_layout;
 
 /* Host only: validate `instances` and compute sizes for `dtype`. */

@@ -73,7 +73,7 @@ def plannable(n_nodes, events, dtype, tuning: Tuning) -> bool:
     try:
         ext.BatchedPlan(torch.from_numpy(table), DTYPE_CODE[dtype], tuning.threads,
                         tuning.guide_shift, tuning.chunk_events, tuning.scan_mode,
-                        tuning.persistent)
+                        tuning.persistent, tuning.table_format)
     except (ValueError, RuntimeError):
         return False
     return True

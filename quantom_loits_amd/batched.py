@@ -32,6 +32,7 @@ class Tuning:
     chunk_events: int = 1 << 18
     scan_mode: int = 0        # 0 parallel CDF scan, 1 the reference's sequential chain
     persistent: int = 0
+    table_format: int = 0     # 0 cell tables where they apply, 1 guide tables only
 
 
 class BatchedLOITS:
@@ -75,7 +76,8 @@ class BatchedLOITS:
         ext = load_extension()
         t = self.tuning
         self.plan = ext.BatchedPlan(torch.from_numpy(table), DTYPE_CODE[dtype], t.threads,
-                                    t.guide_shift, t.chunk_events, t.scan_mode, t.persistent)
+                                    t.guide_shift, t.chunk_events, t.scan_mode, t.persistent,
+                                    t.table_format)
         self.layout = self.plan.layout()
         self.desc = self.plan.descriptors().to(self.device)
         self.n_instances = n

@@ -27,6 +27,7 @@
 #include <cstdint>
 
 #include "batched.hpp"
+#include "fast_tables.hpp"
 #include "loits_device.hpp"
 
 #ifndef QLOITS_TRIP
@@ -178,39 +179,14 @@ __device__ __forceinline__ LdsTables<Real> lds_tables_of(uint8_t* lds, const Ins
   return t;
 }
 
-// GLOBAL: srt, coef and cdf are built in place in the global blob (instances
-// whose LDS image does not fit); LDS then holds the crowded-cell list and the
-// cell counts only.
-template <typename Real, bool SEQUENTIAL, bool GLOBAL = false>
-__device__ void build_tables(const BatchedParams& P, int inst, const InstanceDesc& in,
-                             uint8_t* lds, bool writer, BuildShared& sh) {
-  using Coef = typename CoefOf<Real>::type;
-  const int64_t n = in.n_nodes;
-  const Real* x = static_cast<const Real*>(P.x) + in.node_off;
-  const Real* s = static_cast<const Real*>(P.density) + in.node_off;
-  LdsTables<Real> L;
-  uint32_t* hist;  // per-cell threshold counts (step 4)
-  if constexpr (GLOBAL) {
-    const Tables<Real> T = tables_of<Real>(P.tables, in);
-    L.srt = T.srt;
-    L.guide = T.guide;
-    L.coef = T.coef;
-    L.cdf = T.cdf;
-    L.crowd = reinterpret_cast<uint2*>(lds);
-    hist = reinterpret_cast<uint32_t*>(lds + ((int64_t(1) << in.guide_log2) / 4) * 8);
-  } else {
-    L = lds_tables_of<Real>(lds, in);
-    hist = L.guide + (int64_t(1) << in.guide_log2);  // the level-2 region, free until step 6
-  }
-  const double acc = in.accept ? 1.0 : 0.0;
+// Trapezoid norm and CDF of one curve into cdf[0..n) (LDS or global):
+// the reference's sequential chain (cpp/loits_core.cpp:73-80 then :130-135) or
+// a blocked parallel scan; sh.norm and sh.inv receive the norm and 1 / norm.
+// Ends with a work-group barrier.
+template <typename Real, bool SEQUENTIAL>
+__device__ void build_norm_cdf(const Real* x, const Real* s, int64_t n, double acc, double* cdf,
+                               BuildShared& sh) {
   const int tid = threadIdx.x, nt = blockDim.x;
-  if (tid == 0) {
-    sh.bits = 0xFu;
-    sh.assigned = 0;
-    sh.used = 0;
-  }
-
-  // 1. norm and CDF
   if (SEQUENTIAL) {
     // The reference's own order: cpp/loits_core.cpp:73-80 then :130-135.
     if (tid == 0) {
@@ -223,11 +199,11 @@ __device__ void build_tables(const BatchedParams& P, int inst, const InstanceDes
       }
       const double inv = 1.0 / nrm;
       double cum = 0.0, rho_prev = double(s[0]) * inv;
-      L.cdf[0] = 0.0;
+      cdf[0] = 0.0;
       for (int64_t t = 1; t < n; ++t) {
         const double rho = double(s[t]) * inv;
         cum = __builtin_fma(0.5 * (double(x[t]) - double(x[t - 1])), rho_prev + rho, cum);
-        L.cdf[t] = cum * acc;
+        cdf[t] = cum * acc;
         rho_prev = rho;
       }
       sh.norm = nrm;
@@ -270,11 +246,11 @@ __device__ void build_tables(const BatchedParams& P, int inst, const InstanceDes
         const int64_t t = t0 - 1 + q;
         if (t >= t1) continue;
         if (t == 0) {
-          L.cdf[0] = 0.0;
+          cdf[0] = 0.0;
           continue;
         }
         run += terms[q - 1];
-        L.cdf[t] = run * acc;
+        cdf[t] = run * acc;
       }
     } else {
       double local = 0.0;
@@ -290,12 +266,12 @@ __device__ void build_tables(const BatchedParams& P, int inst, const InstanceDes
       run = block_exclusive_scan(part, sh.scratch, &total);
       for (int64_t t = t0; t < t1; ++t) {
         if (t == 0) {
-          L.cdf[0] = 0.0;
+          cdf[0] = 0.0;
           continue;
         }
         run += 0.5 * (double(x[t]) - double(x[t - 1])) *
                (double(s[t - 1]) * inv + double(s[t]) * inv);
-        L.cdf[t] = run * acc;
+        cdf[t] = run * acc;
       }
     }
     if (tid == 0) {
@@ -304,6 +280,42 @@ __device__ void build_tables(const BatchedParams& P, int inst, const InstanceDes
     }
   }
   __syncthreads();
+}
+
+// GLOBAL: srt, coef and cdf are built in place in the global blob (instances
+// whose LDS image does not fit); LDS then holds the crowded-cell list and the
+// cell counts only.
+template <typename Real, bool SEQUENTIAL, bool GLOBAL = false>
+__device__ void build_tables(const BatchedParams& P, int inst, const InstanceDesc& in,
+                             uint8_t* lds, bool writer, BuildShared& sh) {
+  using Coef = typename CoefOf<Real>::type;
+  const int64_t n = in.n_nodes;
+  const Real* x = static_cast<const Real*>(P.x) + in.node_off;
+  const Real* s = static_cast<const Real*>(P.density) + in.node_off;
+  LdsTables<Real> L;
+  uint32_t* hist;  // per-cell threshold counts (step 4)
+  if constexpr (GLOBAL) {
+    const Tables<Real> T = tables_of<Real>(P.tables, in);
+    L.srt = T.srt;
+    L.guide = T.guide;
+    L.coef = T.coef;
+    L.cdf = T.cdf;
+    L.crowd = reinterpret_cast<uint2*>(lds);
+    hist = reinterpret_cast<uint32_t*>(lds + ((int64_t(1) << in.guide_log2) / 4) * 8);
+  } else {
+    L = lds_tables_of<Real>(lds, in);
+    hist = L.guide + (int64_t(1) << in.guide_log2);  // the level-2 region, free until step 6
+  }
+  const double acc = in.accept ? 1.0 : 0.0;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  if (tid == 0) {
+    sh.bits = 0xFu;
+    sh.assigned = 0;
+    sh.used = 0;
+  }
+
+  // 1. norm and CDF
+  build_norm_cdf<Real, SEQUENTIAL>(x, s, n, acc, L.cdf, sh);
 
   // 2. thresholds, per-bin interpolation data, curve properties
   const int64_t pw = pow2_ceil(n);
@@ -537,6 +549,257 @@ __device__ void build_tables(const BatchedParams& P, int inst, const InstanceDes
   }
 }
 
+// Ascending bitonic sort of a[0, pw) in LDS by the work-group (pw a power of two).
+__device__ void bitonic_sort_u32(uint32_t* a, int64_t pw) {
+  const int tid = threadIdx.x, nt = blockDim.x;
+  for (int64_t k = 2; k <= pw; k <<= 1) {
+    for (int64_t j = k >> 1; j > 0; j >>= 1) {
+      for (int64_t i = tid; i < pw; i += nt) {
+        const int64_t ixj = i ^ j;
+        if (ixj > i) {
+          const uint32_t x = a[i], y = a[ixj];
+          const bool up = (i & k) == 0;
+          if ((x > y) == up) {
+            a[i] = y;
+            a[ixj] = x;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// Cell tables of a float32 curve (fast_tables.hpp) built by one work-group in
+// LDS (layout fast_lds_forward): norm and CDF, thresholds and per-bin
+// interpolation data, the sorted splits, the level-1 cell words and the level-2
+// words of crowded cells.  A `writer` work-group publishes the CDF, the
+// thresholds and the cell words (fast_blob_layout) for the VJP.
+// cdf_copy (global, n doubles, private to the work-group) receives the CDF for
+// the float64 re-evaluation of values near a keep boundary.
+template <bool SEQUENTIAL>
+__device__ void build_fast(const BatchedParams& P, int inst, const InstanceDesc& in, uint8_t* lds,
+                           bool writer, BuildShared& sh, double* cdf_copy) {
+  const int64_t n = in.n_nodes;
+  const int g = in.fast_log2, s = 24 - g;
+  const uint32_t S = 1u << s;
+  const int64_t G = int64_t(1) << g;
+  const int l2cap = fast_l2_cap(n);
+  const FastLdsForward L = fast_lds_forward(n, g);
+  uint32_t* cells = reinterpret_cast<uint32_t*>(lds + L.cells);
+  double* cdf = reinterpret_cast<double*>(lds + L.cells);  // until step 4
+  uint2* list = reinterpret_cast<uint2*>(lds + L.list);
+  float4* bins = reinterpret_cast<float4*>(lds + L.bins);
+  uint32_t* srt = reinterpret_cast<uint32_t*>(lds + L.srt);
+  const float* x = static_cast<const float*>(P.x) + in.node_off;
+  const float* d = static_cast<const float*>(P.density) + in.node_off;
+  const double acc = in.accept ? 1.0 : 0.0;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  if (tid == 0) {
+    sh.bits = 0xFu;
+    sh.assigned = 0;
+    sh.used = 0;
+  }
+
+  // 1. norm and CDF
+  build_norm_cdf<float, SEQUENTIAL>(x, d, n, acc, cdf, sh);
+
+  // 2. thresholds, per-bin {M, A, T}, curve properties (as build_tables)
+  const int64_t pw = pow2_ceil(n);
+  uint32_t bits = 0xFu;
+  for (int64_t t = tid; t < pad4(pw); t += nt) {
+    if (t >= n) {
+      srt[t] = kNever;
+      if (t < pad4(n)) bins[t] = make_float4(0.f, 0.f, 0.f, 0.f);
+      continue;
+    }
+    const double c = cdf[t];
+    const uint32_t thr = threshold_of(c);
+    srt[t] = thr;
+    if (t > 0 && thr < threshold_of(cdf[t - 1])) bits &= ~1u;
+    const double xt = double(x[t]);
+    const double ax = __builtin_fabs(xt);
+    if (!(__builtin_isfinite(c) && ax >= 1e-30 && ax <= 1e29)) bits &= ~2u;
+    if (!(xt > 0.0)) bits &= ~4u;
+    if (!(xt < 0.0)) bits &= ~8u;
+    float4 b = make_float4(0.f, 0.f, __uint_as_float(thr), 0.f);
+    if (t + 1 < n) {
+      const double inv_d = 1.0 / (cdf[t + 1] - c + kEpsilon);  // cpp/loits_core.cpp:193
+      const double m = (double(x[t + 1]) - xt) * inv_d;        // :194
+      if (!__builtin_isfinite(m)) bits &= ~2u;
+      const double r = double(thr) * 0x1.0p-24 - c;
+      b.x = static_cast<float>(m * 0x1.0p-24);
+      b.y = static_cast<float>(__builtin_fma(m, r, xt));
+    }
+    bins[t] = b;
+  }
+  {
+    uint32_t wb = 0u;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) wb |= ballot(!((bits >> q) & 1u)) == 0ull ? (1u << q) : 0u;
+    if (lane_id() == 0 && wb != 0xFu) atomicAnd(&sh.bits, wb);
+  }
+  __syncthreads();
+  const uint32_t all_bits = sh.bits;
+  const bool mono = (all_bits & 1u) != 0;
+  const bool safe = (all_bits & 2u) != 0 && (all_bits & 12u) != 0;
+
+  // 3. publish the CDF and each bin's own threshold (before step 4 reuses the
+  //    CDF's LDS), then sort the thresholds when the CDF is not monotone
+  const FastBlobLayout B = fast_blob_layout(n, g);
+  uint8_t* blob = P.tables + in.table_off;
+  if (cdf_copy)
+    for (int64_t t = tid; t < n; t += nt) cdf_copy[t] = cdf[t];
+  if (writer) {
+    double* gcdf = reinterpret_cast<double*>(blob + B.cdf);
+    uint32_t* gown = reinterpret_cast<uint32_t*>(blob + B.own);
+    for (int64_t t = tid; t < n; t += nt) {
+      gcdf[t] = cdf[t];
+      gown[t] = srt[t];
+      if (P.cdf_out) P.cdf_out[in.node_off + t] = cdf[t];
+    }
+  }
+  __syncthreads();
+  if (!mono) bitonic_sort_u32(srt, pw);
+
+  // 4. per-cell counts of the splits S_1 .. S_{n-2}: all (bits 0-15) and those
+  //    at the cell's first key (bits 16-31); splits >= 2^24 lie in no cell
+  for (int64_t c = tid; c < G; c += nt) cells[c] = 0u;
+  __syncthreads();
+  for (int64_t t = 1 + tid; t <= n - 2; t += nt) {
+    const uint32_t key = srt[t];
+    if (key < (1u << 24)) atomicAdd(&cells[key >> s], 1u + ((key & (S - 1u)) == 0u ? 0x10000u : 0u));
+  }
+  __syncthreads();
+
+  // 5. level-1 words.  Wave w owns the cells [w CW, (w + 1) CW), CW = G / nwaves,
+  //    in rows of 64 consecutive cells (lane l takes cell 64 r + l of the range),
+  //    so every LDS access is conflict-free; the prefix over cells in order is a
+  //    wave scan per row (skipped for rows without splits) plus the totals of the
+  //    waves before.  Scanned per cell: splits | level-2 words asked for << 16
+  //    (each total stays below 2^16: at most n - 2 splits and 4 (n - 2) words).
+  //    A crowded cell's list entry sits at its first level-2 word / 4 (every
+  //    crowded cell asks for at least 4 words).
+  {
+    const int nwaves = nt >> 6, wave = tid >> 6, lane = lane_id();
+    const int64_t CW = G / nwaves;  // a multiple of 64 (G >= 4096, nwaves <= 16)
+    const int rows = int(CW / kWave);
+    uint32_t* cw = cells + int64_t(wave) * CW + lane;
+    auto scan_value = [&](uint32_t v, int* f) -> uint32_t {
+      const uint32_t all = v & 0xFFFFu, inside = all - (v >> 16);
+      *f = inside >= 2u ? guide_fanout_log2(int(inside), s) : 0;
+      return all + (inside >= 2u ? (1u << *f) << 16 : 0u);
+    };
+    uint32_t tot = 0u;
+    for (int r = 0; r < rows; ++r) {
+      int f;
+      tot += scan_value(cw[r * kWave], &f);
+    }
+    tot = wave_sum(tot);
+    uint32_t* wtot = reinterpret_cast<uint32_t*>(sh.scratch);  // one word per wave
+    for (int i = tid; i < l2cap / 4; i += nt) list[i] = make_uint2(0xFFFFFFFFu, 0u);
+    if (lane == 0) wtot[wave] = tot;
+    __syncthreads();
+    uint32_t run = 0u;
+    for (int w2 = 0; w2 < wave; ++w2) run += wtot[w2];
+    for (int r = 0; r < rows; ++r) {
+      const uint32_t v = cw[r * kWave];
+      const uint32_t lo_key = (uint32_t(wave) * uint32_t(CW) + uint32_t(r * kWave + lane)) << s;
+      if (ballot(v != 0u) == 0ull) {  // no split in these 64 cells
+        cw[r * kWave] = fast_direct(run & 0xFFFFu, S);
+        continue;
+      }
+      int f;
+      const uint32_t a = scan_value(v, &f);
+      const uint32_t inc = wave_inclusive_scan(a);
+      const uint32_t ex = run + inc - a;
+      const uint32_t all = v & 0xFFFFu, zero = v >> 16, inside = all - zero;
+      const uint32_t jl = (ex & 0xFFFFu) + zero;  // splits <= L
+      const uint32_t woff = ex >> 16;
+      uint32_t w;
+      if (inside < 2u) {
+        w = fast_direct(jl, inside == 1u ? srt[jl + 1] - lo_key : S);
+      } else if (woff + (1u << f) <= uint32_t(l2cap)) {
+        w = kFastMulti | (uint32_t(f) << 24) | uint32_t(G + woff);
+        list[woff >> 2] = make_uint2(lo_key >> s, jl | (woff << 16));
+        atomicMax(&sh.used, int(woff + (1u << f)));
+      } else {
+        w = kFastMulti | kFastSearch | jl | ((inside < 0x7FFFu ? inside : 0x7FFFu) << 15);
+      }
+      cw[r * kWave] = w;
+      run += wave_lane(inc, kWave - 1);
+    }
+  }
+  __syncthreads();
+
+  // 6. level-2 words of the crowded cells: per-sub-cell counts of the splits
+  //    strictly inside the cell (all | at the sub-cell's first key << 16),
+  //    scattered by the splits, then one wave scan per crowded cell
+  {
+    const int used = sh.used, R = (used + 3) / 4;  // list slots in use (some empty)
+    for (int i = tid; i < used; i += nt) cells[G + i] = 0u;
+    __syncthreads();
+    for (int64_t t = 1 + tid; t <= n - 2; t += nt) {
+      const uint32_t key = srt[t];
+      if (key >= (1u << 24)) continue;
+      const uint32_t w = cells[key >> s];
+      const uint32_t off = key & (S - 1u);
+      if ((w & (kFastMulti | kFastSearch)) != kFastMulti || off == 0u) continue;
+      const int ss = s - int((w >> 24) & 0xFu);
+      atomicAdd(&cells[(w & 0xFFFFFu) + (off >> ss)],
+                1u + ((off & ((1u << ss) - 1u)) == 0u ? 0x10000u : 0u));
+    }
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6, lane = lane_id();
+    for (int e = wave; e < R; e += nwaves) {
+      const uint2 ent = list[e];
+      if (ent.x == 0xFFFFFFFFu) continue;  // (wave-uniform)
+      const uint32_t c = ent.x, wo = ent.y >> 16, jl = ent.y & 0xFFFFu;
+      const int f = int((cells[c] >> 24) & 0xFu);
+      const int ss = s - f;
+      const uint32_t S2 = 1u << ss;
+      const int nsub = 1 << f, per = (nsub + kWave - 1) / kWave;
+      const int q0 = lane * per, q1 = q0 + per < nsub ? q0 + per : nsub;
+      uint32_t local = 0u;
+      for (int q = q0; q < q1; ++q) local += cells[G + wo + q] & 0xFFFFu;
+      uint32_t run = jl + wave_inclusive_scan(local) - local;
+      for (int q = q0; q < q1; ++q) {
+        const uint32_t v = cells[G + wo + q];
+        const uint32_t all = v & 0xFFFFu, zero = v >> 16, in2 = all - zero;
+        const uint32_t jl2 = run + zero;
+        const uint32_t klo = (c << s) + (uint32_t(q) << ss);
+        cells[G + wo + q] = in2 <= 1u
+                                ? fast_direct(jl2, in2 == 1u ? srt[jl2 + 1] - klo : S2)
+                                : kFastMulti | kFastSearch | jl2 | (in2 << 15);
+        run += all;
+      }
+    }
+  }
+  if (tid == 0) {
+    const bool last_full = srt[n - 1] >= 16777214u;
+    sh.flags = kFastTables | (mono ? kMonotone : 0u) |
+               ((mono && safe && acc == 1.0 && last_full) ? kSafe : 0u);
+  }
+  __syncthreads();
+
+  // 7. publish the sorted splits and the cell words in use
+  if (writer) {
+    lds_copy(blob + B.srt, srt, pad4(n) * 4);
+    lds_copy(blob + B.cells, cells, pad4(G + sh.used) * 4);
+    if (tid == 0) {
+      TableHeader* hdr = reinterpret_cast<TableHeader*>(blob);
+      hdr->norm = sh.norm;
+      hdr->inv_n = sh.inv;
+      hdr->flags = sh.flags;
+      hdr->n = static_cast<uint32_t>(n);
+      hdr->guide_log2 = static_cast<uint32_t>(g);
+      hdr->level2_words = static_cast<uint32_t>(sh.used);
+      if (P.norm_out) P.norm_out[inst] = sh.norm;
+      P.done[inst] = 0u;
+    }
+  }
+}
+
 // Standalone build: one work-group per instance.  Before sampling it builds
 // the global-table instances (only_global); for the VJP's recompute path it
 // rebuilds every instance's blob.
@@ -547,7 +810,11 @@ __global__ void __launch_bounds__(1024) tables_kernel(BatchedParams P, int only_
   const InstanceDesc in = P.inst[blockIdx.x];
   if (in.global_tables)
     build_tables<Real, SEQUENTIAL, true>(P, blockIdx.x, in, lds, true, sh);
-  else if (!only_global)
+  else if (only_global)
+    return;
+  else if (sizeof(Real) == 4 && in.fast_log2 > 0)
+    build_fast<SEQUENTIAL>(P, blockIdx.x, in, lds, true, sh, nullptr);
+  else
     build_tables<Real, SEQUENTIAL, false>(P, blockIdx.x, in, lds, true, sh);
 }
 
@@ -730,17 +997,109 @@ __device__ __forceinline__ int next_segment(int* counter) {
 // One chunk's sampling loop over tables in LDS or, for global-table instances,
 // a guide in LDS and the per-bin arrays in global memory (forceinline keeps
 // each call's address spaces known to the compiler).
+// Interval search and values of a trip's keys over the two-level guide tables
+// (find_bins, slot_values); returns the keep bits.
 template <typename Real, bool WINDOW>
-__device__ __forceinline__ void sample_segment(const BatchedParams& P, int c, const ChunkDesc& ch,
-                                               const InstanceDesc& in, int w,
-                                               const uint32_t* s_guide, const uint32_t* s_srt,
-                                               const typename CoefOf<Real>::type* s_coef,
-                                               const uint32_t* s_thr, const double* cdf,
-                                               bool safe_curve, int64_t* sh_kept) {
-  const int n = static_cast<int>(in.n_nodes);
-  const int shift = 24 - in.guide_log2;
-  const Real* x_g = static_cast<const Real*>(P.x) + in.node_off;
+struct GuideLookup {
+  const uint32_t* guide;
+  const uint32_t* srt;
+  const typename CoefOf<Real>::type* coef;
+  const uint32_t* thr;
+  const double* cdf;  // global
+  const Real* x;      // global
+  int shift, n;
+  bool safe;
 
+  template <int NK>
+  __device__ __forceinline__ int operator()(const uint32_t (&k)[NK], int (&j)[NK], Real (&v)[NK],
+                                            const BatchedParams& P) const {
+    find_bins<NK>(k, guide, srt, shift, n, j);
+    return slot_values<Real, WINDOW, NK>(k, j, coef, thr, cdf, x, P, safe, v);
+  }
+};
+
+// Bins of a trip's keys from the cell tables of fast_tables.hpp: one cell word
+// per key; the level-2 step or bisection only at key positions where some lane
+// of the wave needs it (each position's branch is wave-uniform).
+template <int NK>
+__device__ __forceinline__ void cell_bins(const uint32_t (&k)[NK], const uint32_t* cells,
+                                          const uint32_t* srt, int s, int (&j)[NK]) {
+  const uint32_t mask = (1u << s) - 1u;
+  uint32_t w[NK];
+#pragma unroll
+  for (int e = 0; e < NK; ++e) w[e] = cells[k[e] >> s];
+#pragma unroll
+  for (int e = 0; e < NK; ++e) j[e] = fast_direct_bin(w[e], k[e] & mask);
+#pragma unroll
+  for (int e = 0; e < NK; ++e)
+    if (ballot((w[e] & kFastMulti) != 0u) != 0ull)
+      if ((w[e] & kFastMulti) != 0u) j[e] = fast_multi_bin(w[e], k[e], s, cells, srt);
+}
+
+// The same over the cell tables of fast_tables.hpp (float32): one cell word per
+// key, the level-2 step only at key positions where some lane needs it, then
+// one float4 {M, A, T} read per key.
+template <bool WINDOW>
+struct CellLookup {
+  const uint32_t* cells;
+  const uint32_t* srt;
+  const float4* bins;
+  const double* cdf;  // global
+  const float* x;     // global
+  int s;
+  bool safe;
+
+  template <int NK>
+  __device__ __forceinline__ int operator()(const uint32_t (&k)[NK], int (&j)[NK], float (&v)[NK],
+                                            const BatchedParams& P) const {
+    cell_bins<NK>(k, cells, srt, s, j);
+    float4 b[NK];
+#pragma unroll
+    for (int e = 0; e < NK; ++e) b[e] = bins[j[e]];
+    if (!WINDOW && safe) {  // every value of this curve is kept (kSafe)
+#pragma unroll
+      for (int e = 0; e < NK; ++e)
+        v[e] = __builtin_fmaf(
+            b[e].x, static_cast<float>(static_cast<int32_t>(k[e] - __float_as_uint(b[e].z))),
+            b[e].y);
+      return (1 << NK) - 1;
+    }
+    const float lo = static_cast<float>(P.win_lo), hi = static_cast<float>(P.win_hi);
+    int keep = 0, unsafe = 0;
+#pragma unroll
+    for (int e = 0; e < NK; ++e) {
+      v[e] = __builtin_fmaf(
+          b[e].x, static_cast<float>(static_cast<int32_t>(k[e] - __float_as_uint(b[e].z))),
+          b[e].y);
+      const float a = __builtin_fabsf(v[e]);
+      bool ok = a >= 1e-30f && a <= 1e30f;
+      bool in = true;
+      if (WINDOW) {
+        const float margin = 2e-6f * a;
+        ok = ok && __builtin_fabsf(v[e] - lo) > margin && __builtin_fabsf(v[e] - hi) > margin;
+        in = v[e] >= lo && v[e] <= hi;
+      }
+      keep |= (ok && in) ? (1 << e) : 0;
+      unsafe |= ok ? 0 : (1 << e);
+    }
+    if (__builtin_expect(unsafe != 0, 0)) {
+#pragma unroll 1
+      for (int e = 0; e < NK; ++e) {
+        if ((unsafe >> e) & 1) {
+          const double v64 = interp_reference(cdf, x, j[e], k[e]);
+          v[e] = static_cast<float>(v64);
+          keep |= keep_value<WINDOW>(v64, P) ? (1 << e) : 0;
+        }
+      }
+    }
+    return keep;
+  }
+};
+
+template <typename Real, bool WINDOW, typename Lookup>
+__device__ __forceinline__ void sample_segment(const BatchedParams& P, int c, const ChunkDesc& ch,
+                                               const InstanceDesc& in, int w, const Lookup& lookup,
+                                               int64_t* sh_kept) {
   const int nw = P.segs, lane = lane_id();  // segment w of nw in this chunk
   const Segment sg = segment_of(in.counter_base, ch.slot_lo, ch.slot_hi, w, nw);
   const int64_t seg = int64_t(c) * nw + w;
@@ -809,6 +1168,60 @@ __device__ __forceinline__ void sample_segment(const BatchedParams& P, int c, co
     running += total;
   };
 
+  // A curve whose every value is kept (kSafe, no window): slot 4 i + e of the
+  // segment is row 4 i + e - off0, so rows need no keep bits, ballots or masks;
+  // whole blocks go out as one 16-byte store when the rows are aligned.
+  if (!WINDOW && lookup.safe) {
+    const bool vec = off0 == 0u && (base_row & 3) == 0;
+    for (uint32_t i0 = 0; i0 < nblk; i0 += 64 * kTrip) {
+      uint32_t k[4 * kTrip];
+#pragma unroll
+      for (int h = 0; h < kTrip; ++h) {
+        const Words4 wd =
+            philox4x32_10_uniform(sg.qa + i0 + 64 * h + uint32_t(lane), in.stream, P.seed);
+        k[4 * h] = wd.w0 >> 8;
+        k[4 * h + 1] = wd.w1 >> 8;
+        k[4 * h + 2] = wd.w2 >> 8;
+        k[4 * h + 3] = wd.w3 >> 8;
+      }
+      int j[4 * kTrip];
+      Real v[4 * kTrip];
+      lookup(k, j, v, P);
+#pragma unroll
+      for (int h = 0; h < kTrip; ++h) {
+        const uint32_t i = i0 + 64 * h + uint32_t(lane);
+        if (vec && 4 * i + 4 <= span) {
+          if constexpr (sizeof(Real) == 4) {
+            *reinterpret_cast<float4*>(out + 4 * i) =
+                make_float4(v[4 * h], v[4 * h + 1], v[4 * h + 2], v[4 * h + 3]);
+          } else {
+            reinterpret_cast<double2*>(out + 4 * i)[0] = make_double2(v[4 * h], v[4 * h + 1]);
+            reinterpret_cast<double2*>(out + 4 * i)[1] = make_double2(v[4 * h + 2], v[4 * h + 3]);
+          }
+          if (bins_out)
+            *reinterpret_cast<int4*>(bins_out + 4 * i) =
+                make_int4(j[4 * h], j[4 * h + 1], j[4 * h + 2], j[4 * h + 3]);
+        } else {
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const uint32_t r = 4 * i + e;
+            if (i < nblk && r >= off0 && r < span) {
+              out[r - off0] = v[4 * h + e];
+              if (bins_out) bins_out[r - off0] = j[4 * h + e];
+            }
+          }
+        }
+      }
+    }
+    if (lane == 0) {
+      running = int64_t(span - off0);
+      P.seg_kept[seg] = running;
+      sh_kept[w] = running;
+      P.seg_drop[seg] = first_drop;
+    }
+    return;
+  }
+
   // kTrip iterations per trip: their Philox, search and interpolation chains
   // are independent, which hides LDS latency at four waves per SIMD.
   int it = 0;
@@ -824,10 +1237,8 @@ __device__ __forceinline__ void sample_segment(const BatchedParams& P, int c, co
       k[4 * h + 3] = wd.w3 >> 8;
     }
     int j[4 * kTrip];
-    find_bins<4 * kTrip>(k, s_guide, s_srt, shift, n, j);
     Real v[4 * kTrip];
-    const int keep =
-        slot_values<Real, WINDOW, 4 * kTrip>(k, j, s_coef, s_thr, cdf, x_g, P, safe_curve, v);
+    const int keep = lookup(k, j, v, P);
 #pragma unroll
     for (int h = 0; h < kTrip; ++h)
       if (h == 0 || i0 + 64 * h < nblk)
@@ -840,39 +1251,8 @@ __device__ __forceinline__ void sample_segment(const BatchedParams& P, int c, co
   }
 }
 
-template <typename Real, bool WINDOW, bool SEQUENTIAL>
-__global__ void __launch_bounds__(1024) sample_kernel(BatchedParams P) {
-  extern __shared__ __align__(16) uint8_t lds[];
-  __shared__ BuildShared sh;
-  __shared__ int sh_next;  // next unclaimed segment (read after the barriers below)
-  __shared__ int64_t sh_kept[kWave];  // kept events per segment (at most 64 per chunk)
-  const int c = blockIdx.x;
-  const ChunkDesc ch = P.chunks[c];
-  const InstanceDesc in = P.inst[ch.instance];
-  const int64_t nb = pad4(in.n_nodes);
-  if (threadIdx.x == 0) sh_next = 0;
-  if (in.global_tables) {  // built beforehand by tables_kernel; stage the guide
-    const Tables<Real> T = tables_of<Real>(P.tables, in);
-    lds_copy(lds, T.guide, guide_bytes(T, in));
-    if (threadIdx.x == 0) sh.flags = T.hdr->flags;
-    __syncthreads();
-    const uint32_t* thr = (sh.flags & kMonotone) != 0 || sizeof(Real) != 4
-                              ? T.srt
-                              : coef_f32_view(T.coef, nb).thr;
-    for (int sgi = next_segment(&sh_next); sgi < P.segs; sgi = next_segment(&sh_next))
-      sample_segment<Real, WINDOW>(P, c, ch, in, sgi, reinterpret_cast<const uint32_t*>(lds),
-                                   T.srt, T.coef, thr, T.cdf, (sh.flags & kSafe) != 0, sh_kept);
-  } else {
-    build_tables<Real, SEQUENTIAL>(P, ch.instance, in, lds, ch.index == 0, sh);
-    const LdsTables<Real> L = lds_tables_of<Real>(lds, in);
-    const uint32_t* thr = (sh.flags & kMonotone) != 0 || sizeof(Real) != 4
-                              ? L.srt
-                              : coef_f32_view(L.coef, nb).thr;
-    for (int sgi = next_segment(&sh_next); sgi < P.segs; sgi = next_segment(&sh_next))
-      sample_segment<Real, WINDOW>(P, c, ch, in, sgi, L.guide, L.srt, L.coef, thr, L.cdf,
-                                   (sh.flags & kSafe) != 0, sh_kept);
-  }
-  // rows of the chunk's segments relative to the chunk, and the chunk's total
+// Rows of a chunk's segments relative to the chunk, and the chunk's total.
+__device__ __forceinline__ void chunk_rows(const BatchedParams& P, int c, const int64_t* sh_kept) {
   __syncthreads();
   if (threadIdx.x < kWave) {
     const int lane = lane_id();
@@ -887,6 +1267,76 @@ __global__ void __launch_bounds__(1024) sample_kernel(BatchedParams P) {
     if (lane < P.segs) P.seg_row[int64_t(c) * P.segs + lane] = inc - v;
     if (lane == kWave - 1) P.chunk_kept[c] = inc;
   }
+}
+
+// Sampling of the guide-table instances: one work-group per chunk (the chunks
+// after the cell-table ones in fwd_order); waves take segments from a counter.
+template <typename Real, bool WINDOW, bool SEQUENTIAL>
+__global__ void __launch_bounds__(1024) sample_kernel(BatchedParams P) {
+  extern __shared__ __align__(16) uint8_t lds[];
+  __shared__ BuildShared sh;
+  __shared__ int sh_next;  // next unclaimed segment (read after the barriers below)
+  __shared__ int64_t sh_kept[kWave];  // kept events per segment (at most 64 per chunk)
+  const int c = P.fwd_order[P.n_cell_chunks + blockIdx.x];
+  const ChunkDesc ch = P.chunks[c];
+  const InstanceDesc in = P.inst[ch.instance];
+  const int64_t nb = pad4(in.n_nodes);
+  if (threadIdx.x == 0) sh_next = 0;
+  const Real* x_g = static_cast<const Real*>(P.x) + in.node_off;
+  if (in.global_tables) {  // built beforehand by tables_kernel; stage the guide
+    const Tables<Real> T = tables_of<Real>(P.tables, in);
+    lds_copy(lds, T.guide, guide_bytes(T, in));
+    if (threadIdx.x == 0) sh.flags = T.hdr->flags;
+    __syncthreads();
+    const uint32_t* thr = (sh.flags & kMonotone) != 0 || sizeof(Real) != 4
+                              ? T.srt
+                              : coef_f32_view(T.coef, nb).thr;
+    const GuideLookup<Real, WINDOW> lk{reinterpret_cast<const uint32_t*>(lds), T.srt, T.coef, thr,
+                                       T.cdf, x_g, 24 - in.guide_log2, int(in.n_nodes),
+                                       (sh.flags & kSafe) != 0};
+    for (int sgi = next_segment(&sh_next); sgi < P.segs; sgi = next_segment(&sh_next))
+      sample_segment<Real, WINDOW>(P, c, ch, in, sgi, lk, sh_kept);
+  } else {
+    build_tables<Real, SEQUENTIAL>(P, ch.instance, in, lds, ch.index == 0, sh);
+    const LdsTables<Real> L = lds_tables_of<Real>(lds, in);
+    const uint32_t* thr = (sh.flags & kMonotone) != 0 || sizeof(Real) != 4
+                              ? L.srt
+                              : coef_f32_view(L.coef, nb).thr;
+    const GuideLookup<Real, WINDOW> lk{L.guide, L.srt, L.coef, thr, L.cdf, x_g,
+                                       24 - in.guide_log2, int(in.n_nodes),
+                                       (sh.flags & kSafe) != 0};
+    for (int sgi = next_segment(&sh_next); sgi < P.segs; sgi = next_segment(&sh_next))
+      sample_segment<Real, WINDOW>(P, c, ch, in, sgi, lk, sh_kept);
+  }
+  chunk_rows(P, c, sh_kept);
+}
+
+// Sampling of the float32 cell-table instances (fast_tables.hpp): one
+// work-group per chunk (the first n_cell_chunks of fwd_order).  A kernel of
+// its own, so that neither path's registers limit the other's occupancy.
+template <bool WINDOW, bool SEQUENTIAL>
+__global__ void __launch_bounds__(1024) sample_cell_kernel(BatchedParams P) {
+  extern __shared__ __align__(16) uint8_t lds[];
+  __shared__ BuildShared sh;
+  __shared__ int sh_next;
+  __shared__ int64_t sh_kept[kWave];
+  const int c = P.fwd_order[blockIdx.x];
+  const ChunkDesc ch = P.chunks[c];
+  const InstanceDesc in = P.inst[ch.instance];
+  if (threadIdx.x == 0) sh_next = 0;
+  // the chunk's VJP partial sums are free during the forward: they hold this
+  // group's copy of the CDF
+  double* cdf_copy = P.partials + ch.partial_off;
+  build_fast<SEQUENTIAL>(P, ch.instance, in, lds, ch.index == 0, sh, cdf_copy);
+  const FastLdsForward L = fast_lds_forward(in.n_nodes, in.fast_log2);
+  const CellLookup<WINDOW> lk{reinterpret_cast<const uint32_t*>(lds + L.cells),
+                              reinterpret_cast<const uint32_t*>(lds + L.srt),
+                              reinterpret_cast<const float4*>(lds + L.bins), cdf_copy,
+                              static_cast<const float*>(P.x) + in.node_off, 24 - in.fast_log2,
+                              (sh.flags & kSafe) != 0};
+  for (int sgi = next_segment(&sh_next); sgi < P.segs; sgi = next_segment(&sh_next))
+    sample_segment<float, WINDOW>(P, c, ch, in, sgi, lk, sh_kept);
+  chunk_rows(P, c, sh_kept);
 }
 
 // ---------------------------------------------------------------------------
@@ -964,15 +1414,13 @@ __global__ void __launch_bounds__(1024) pack_kernel(BatchedParams P, Real* packe
 // LDS.  gC (cpp/loits_core.cpp:294-301 summed per bin), the CDF adjoint
 // (:355-365) and the density adjoint (:416-431) with blocked parallel scans.
 template <typename Real, int MAXPER>
-__device__ void vjp_tail(const BatchedParams& P, const InstanceDesc& in, const Tables<Real>& T,
-                         const double* h0, const double* h1, Real* grad_density,
+__device__ void vjp_tail(const BatchedParams& P, const InstanceDesc& in, const double* cdf,
+                         double inv_n, const double* h0, const double* h1, Real* grad_density,
                          double* scratch) {
   const int64_t n = in.n_nodes;
   const Real* x = static_cast<const Real*>(P.x) + in.node_off;
   const Real* s = static_cast<const Real*>(P.density) + in.node_off;
-  const double* cdf = T.cdf;
   const double acc = in.accept ? 1.0 : 0.0;
-  const double inv_n = T.hdr->inv_n;
   const int nt = blockDim.x;
   const int64_t per = (n + nt - 1) / nt;
   const int64_t t0 = threadIdx.x * per;
@@ -1416,7 +1864,7 @@ __device__ __forceinline__ void vjp_body(const BatchedParams& P, const Real* gra
   }
 
   if (in.n_chunks == 1) {
-    vjp_tail<Real, MAXPER>(P, in, T, h0, h1, grad_density, scratch);
+    vjp_tail<Real, MAXPER>(P, in, T.cdf, T.hdr->inv_n, h0, h1, grad_density, scratch);
     return;
   }
   // multi-chunk instance: publish partial sums; the last chunk reduces them in order
@@ -1449,7 +1897,309 @@ __device__ __forceinline__ void vjp_body(const BatchedParams& P, const Real* gra
   }
   if (tid == 0) P.done[ch.instance] = 0u;
   __syncthreads();
-  vjp_tail<Real, MAXPER>(P, in, T, h0, h1, grad_density, scratch);
+  vjp_tail<Real, MAXPER>(P, in, T.cdf, T.hdr->inv_n, h0, h1, grad_density, scratch);
+}
+
+// VJP of a float32 curve with cell tables (fast_tables.hpp): one work-group per
+// chunk.  The work-group stages the cell words and sorted splits, regenerates
+// its keys and finds their bins as the forward did, and sums per bin
+//   a0 = sum q,  a1 = sum q k,   q = round(g 2^e), |q| < 2^kFastQBits
+// as 64-bit integers (exact, order-independent): with the bin's own threshold
+// T_j, sum g (u - C_j) = 2^-24 sum g (k - T_j) + (T_j 2^-24 - C_j) sum g, and
+// sum q (k - T_j) = a1 - T_j a0 exactly.  2^e is set from the largest |g| the
+// work-group probes first, two bits below the limit; a cotangent beyond it (or
+// not finite) goes to float64 sums of g and g (k - T_j) in the chunk's
+// partials.  Then the interpolation adjoint per bin, the CDF and density
+// adjoints (vjp_tail), in the last chunk of the instance to finish.
+template <int MAXPER>
+__device__ void fast_vjp_body(const BatchedParams& P, const float* grad, float* grad_density,
+                              uint8_t* lds, int c) {
+  __shared__ double scratch[16];
+  __shared__ int sh_last;
+  __shared__ uint32_t sh_gmax;
+  __shared__ int sh_outl;
+  __shared__ int sh_next;
+  const ChunkDesc ch = P.chunks[c];
+  const InstanceDesc in = P.inst[ch.instance];
+  const int n = static_cast<int>(in.n_nodes);
+  const int g = in.fast_log2, s = 24 - g;
+  const int64_t G = int64_t(1) << g;
+  const uint8_t* blob = P.tables + in.table_off;
+  const TableHeader* hdr = reinterpret_cast<const TableHeader*>(blob);
+  const FastBlobLayout B = fast_blob_layout(n, g);
+  const FastLdsVjp L = fast_lds_vjp(n, g);
+  const double* cdf = reinterpret_cast<const double*>(blob + B.cdf);
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int probe_wave = __builtin_amdgcn_readfirstlane(tid >> 6), probe_nwv = nt >> 6;
+  ScaleProbe probe;
+  probe.load(P, grad, c, probe_wave, probe_nwv, lane_id());
+
+  uint32_t* cells = reinterpret_cast<uint32_t*>(lds + L.cells);
+  uint32_t* srt = reinterpret_cast<uint32_t*>(lds + L.srt);
+  uint64_t* a0 = reinterpret_cast<uint64_t*>(lds + L.a0);
+  uint64_t* a1 = a0 + kFastMaxNodes;  // one address, an immediate offset for the second add
+  const int64_t nb = pad4(n);
+  double* outl = P.partials + ch.partial_off;  // [2 nb]: float64 sums, then partials
+  lds_copy(cells, blob + B.cells, pad4(G + int64_t(hdr->level2_words)) * 4);
+  lds_copy(srt, blob + B.srt, nb * 4);
+  for (int64_t i = tid; i < nb; i += nt) {
+    a0[i] = 0ull;
+    a1[i] = 0ull;
+  }
+  for (int64_t i = tid; i < 2 * nb; i += nt) outl[i] = 0.0;
+  if (tid == 0) {
+    sh_gmax = 0u;
+    sh_outl = 0;
+    sh_next = 0;
+  }
+  const bool mono = (hdr->flags & kMonotone) != 0;
+  const uint32_t* own = mono ? srt : reinterpret_cast<const uint32_t*>(blob + B.own);
+  const int nw = P.segs, lane = lane_id();
+  {
+    uint32_t m_all = probe.max(grad, lane);
+    for (int s0 = probe_wave + 4 * probe_nwv; s0 < nw; s0 += 4 * probe_nwv) {
+      probe.load(P, grad, c, s0, probe_nwv, lane);
+      const uint32_t m = probe.max(grad, lane);
+      m_all = m > m_all ? m : m_all;
+    }
+    __syncthreads();  // the counters above are initialised
+    if (lane == 0 && m_all != 0u) atomicMax(&sh_gmax, m_all);
+  }
+  __syncthreads();
+  float scale;
+  int scale_log2 = 0;
+  {
+    const uint32_t gm = sh_gmax;
+    const int ex = int(gm >> 23);  // |g| < 2^(ex - 126)
+    if (ex == 255) {
+      scale = __builtin_nanf("");  // inf / NaN cotangents: every value takes the float64 sums
+    } else {
+      scale_log2 = (kFastQBits - 2) - (ex - 126);
+      scale_log2 = scale_log2 > 126 ? 126 : scale_log2;
+      scale = __builtin_ldexpf(1.0f, scale_log2);
+    }
+  }
+  constexpr float kQLimit = float(1 << kFastQBits);
+
+  for (int w = next_segment(&sh_next); w < nw; w = next_segment(&sh_next)) {
+    const Segment sg = segment_of(in.counter_base, ch.slot_lo, ch.slot_hi, w, nw);
+    const int64_t seg = int64_t(c) * nw + w;
+    const int first_drop = P.seg_drop[seg];
+    const int64_t row0 = P.chunk_row[c] + P.seg_row[seg];
+    const int64_t per_wave = int64_t(blocks_per_wave(uint64_t(chunk_blocks(in, ch)), nw) / 64);
+    const uint64_t* mask = P.keepmask + ch.mask_off + int64_t(w) * per_wave * 4;
+    const uint32_t nblk = uint32_t(sg.qb - sg.qa);
+    const uint32_t off0 = uint32_t(sg.p_lo - 4 * sg.qa);
+    const uint32_t span = uint32_t(sg.p_hi - 4 * sg.qa);
+    int64_t running = int64_t(4 * 64 * int64_t(first_drop)) - int64_t(off0);
+    running = running < 0 ? 0 : running;
+
+    auto rows_of = [&](int it, uint32_t i0, int& keepbits, int64_t& row) {
+      const uint32_t i = i0 + uint32_t(lane);
+      const bool full_iter = (i0 > 0 || off0 == 0) && 4 * (i0 + 64) <= span;
+      keepbits = 0xF;
+      if (it < first_drop) {
+        if (!full_iter) {
+          keepbits = 0;
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const uint32_t r = 4 * i + e;
+            keepbits |= (i < nblk && r >= off0 && r < span) ? (1 << e) : 0;
+          }
+        }
+        const int32_t r0 = int32_t(4 * i) - int32_t(off0);
+        row = row0 + (r0 < 0 ? 0 : r0);
+      } else {
+        const uint64_t* mw = mask + int64_t(it) * 4;
+        keepbits = 0;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) keepbits |= int((mw[e] >> lane) & 1ull) << e;
+        int total;
+        row = row0 + running + wave_scan_counts(__popc(keepbits), &total);
+        running += total;
+      }
+    };
+
+    auto add = [&](int j, uint32_t k, float gv) {
+      const float gs = gv * scale;
+      if (__builtin_expect(__builtin_fabsf(gs) < kQLimit, 1)) {
+        const int q = static_cast<int>(__builtin_rintf(gs));
+        atomicAdd(reinterpret_cast<unsigned long long*>(a0 + j),
+                  static_cast<unsigned long long>(static_cast<int64_t>(q)));
+        atomicAdd(reinterpret_cast<unsigned long long*>(a1 + j),
+                  static_cast<unsigned long long>(static_cast<int64_t>(q) * int64_t(k)));
+      } else {
+        sh_outl = 1;
+        unsafeAtomicAdd(outl + j, double(gv));
+        unsafeAtomicAdd(outl + nb + j,
+                        double(gv) * double(static_cast<int32_t>(k - own[j])));
+      }
+    };
+
+    if (first_drop == int(sg.iters)) {
+      // every slot of the segment was kept: the cotangent of slot 4 i + e is row
+      // base + 4 i + e; whole aligned blocks load as one 16-byte read
+      const int64_t base = row0 - int64_t(off0);
+      const bool vec = off0 == 0u && (base & 3) == 0;
+      for (uint32_t i0 = 0; i0 < nblk; i0 += 64 * kTrip) {
+        uint32_t k[4 * kTrip];
+#pragma unroll
+        for (int h = 0; h < kTrip; ++h) {
+          const Words4 wd =
+              philox4x32_10_uniform(sg.qa + i0 + 64 * h + uint32_t(lane), in.stream, P.seed);
+          k[4 * h] = wd.w0 >> 8;
+          k[4 * h + 1] = wd.w1 >> 8;
+          k[4 * h + 2] = wd.w2 >> 8;
+          k[4 * h + 3] = wd.w3 >> 8;
+        }
+        float gv[4 * kTrip];
+        int live[kTrip];
+#pragma unroll
+        for (int h = 0; h < kTrip; ++h) {
+          const uint32_t i = i0 + 64 * h + uint32_t(lane);
+          if (vec && 4 * i + 4 <= span) {
+            const float4 g4 = *reinterpret_cast<const float4*>(grad + base + 4 * int64_t(i));
+            gv[4 * h] = g4.x;
+            gv[4 * h + 1] = g4.y;
+            gv[4 * h + 2] = g4.z;
+            gv[4 * h + 3] = g4.w;
+            live[h] = 0xF;
+          } else {
+            live[h] = 0;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+              const uint32_t r = 4 * i + e;
+              const bool ok = i < nblk && r >= off0 && r < span;
+              live[h] |= ok ? (1 << e) : 0;
+              gv[4 * h + e] = ok ? grad[base + int64_t(r)] : 0.0f;
+            }
+          }
+        }
+        int j[4 * kTrip];
+        cell_bins<4 * kTrip>(k, cells, srt, s, j);
+#pragma unroll
+        for (int h = 0; h < kTrip; ++h) {
+          if (live[h] == 0xF) {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) add(j[4 * h + e], k[4 * h + e], gv[4 * h + e]);
+          } else {
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+              if ((live[h] >> e) & 1) add(j[4 * h + e], k[4 * h + e], gv[4 * h + e]);
+          }
+        }
+      }
+      continue;
+    }
+
+    int it = 0;
+    for (uint32_t i0 = 0; i0 < nblk; i0 += 64 * kTrip, it += kTrip) {
+      uint32_t k[4 * kTrip];
+#pragma unroll
+      for (int h = 0; h < kTrip; ++h) {
+        const Words4 wd =
+            philox4x32_10_uniform(sg.qa + i0 + 64 * h + uint32_t(lane), in.stream, P.seed);
+        k[4 * h] = wd.w0 >> 8;
+        k[4 * h + 1] = wd.w1 >> 8;
+        k[4 * h + 2] = wd.w2 >> 8;
+        k[4 * h + 3] = wd.w3 >> 8;
+      }
+      int kb[kTrip];
+      int64_t rw[kTrip];
+#pragma unroll
+      for (int h = 0; h < kTrip; ++h) {
+        kb[h] = 0;
+        rw[h] = 0;
+        if (h == 0 || i0 + 64 * h < nblk) rows_of(it + h, i0 + 64 * h, kb[h], rw[h]);
+      }
+      int j[4 * kTrip];
+      cell_bins<4 * kTrip>(k, cells, srt, s, j);
+      float gv[4 * kTrip];
+#pragma unroll
+      for (int h = 0; h < kTrip; ++h) {
+        if (kb[h] == 0xF && (rw[h] & 3) == 0) {
+          const float4 g4 = *reinterpret_cast<const float4*>(grad + rw[h]);
+          gv[4 * h] = g4.x;
+          gv[4 * h + 1] = g4.y;
+          gv[4 * h + 2] = g4.z;
+          gv[4 * h + 3] = g4.w;
+        } else {
+          int64_t r = rw[h];
+#pragma unroll
+          for (int e = 0; e < 4; ++e) gv[4 * h + e] = ((kb[h] >> e) & 1) ? grad[r++] : 0.0f;
+        }
+      }
+#pragma unroll
+      for (int h = 0; h < kTrip; ++h) {
+        if (kb[h] == 0xF) {
+#pragma unroll
+          for (int e = 0; e < 4; ++e) add(j[4 * h + e], k[4 * h + e], gv[4 * h + e]);
+        } else {
+#pragma unroll
+          for (int e = 0; e < 4; ++e)
+            if ((kb[h] >> e) & 1) add(j[4 * h + e], k[4 * h + e], gv[4 * h + e]);
+        }
+      }
+    }
+  }
+  __syncthreads();
+
+  // integers (and float64 sums) -> h0 = sum g, h1 = sum g (u - C_j), in place
+  double* h0 = reinterpret_cast<double*>(a0);
+  double* h1 = reinterpret_cast<double*>(a1);
+  {
+    const double inv_scale = __builtin_ldexp(1.0, -scale_log2);
+    const bool any_outl = sh_outl != 0;
+    for (int64_t i = tid; i < nb; i += nt) {
+      double v0 = 0.0, v1 = 0.0;
+      if (i < n - 1) {
+        const uint64_t A0 = a0[i], A1 = a1[i];
+        const uint32_t T = own[i];
+        const double C = cdf[i];
+        const int64_t d = static_cast<int64_t>(A1 - uint64_t(T) * A0);  // sum q (k - T_j)
+        v0 = double(static_cast<int64_t>(A0)) * inv_scale + (any_outl ? load_l2(outl + i) : 0.0);
+        v1 = (double(d) * inv_scale + (any_outl ? load_l2(outl + nb + i) : 0.0)) * 0x1.0p-24 +
+             (double(T) * 0x1.0p-24 - C) * v0;
+      }
+      h0[i] = v0;
+      h1[i] = v1;
+    }
+  }
+  __syncthreads();
+  if (in.n_chunks == 1) {
+    vjp_tail<float, MAXPER>(P, in, cdf, hdr->inv_n, h0, h1, grad_density, scratch);
+    return;
+  }
+  // multi-chunk instance: publish partial sums; the last chunk reduces them in order
+  {
+    double* part = P.partials + ch.partial_off;
+    for (int64_t i = tid; i < n; i += nt) {
+      part[i] = h0[i];
+      part[nb + i] = h1[i];
+    }
+  }
+  __syncthreads();
+  if (tid == 0) {
+    __threadfence();
+    const uint32_t old = atomicAdd(&P.done[ch.instance], 1u);
+    sh_last = old == uint32_t(in.n_chunks - 1);
+    if (sh_last) __threadfence();
+  }
+  __syncthreads();
+  if (!sh_last) return;
+  for (int64_t i = tid; i < n; i += nt) {
+    double sa = 0.0, sb = 0.0;
+    for (int cc = in.first_chunk; cc < in.first_chunk + in.n_chunks; ++cc) {
+      const double* pp = P.partials + P.chunks[cc].partial_off;
+      sa += __builtin_nontemporal_load(pp + i);
+      sb += __builtin_nontemporal_load(pp + nb + i);
+    }
+    h0[i] = sa;
+    h1[i] = sb;
+  }
+  if (tid == 0) P.done[ch.instance] = 0u;
+  __syncthreads();
+  vjp_tail<float, MAXPER>(P, in, cdf, hdr->inv_n, h0, h1, grad_density, scratch);
 }
 
 // Exclusive prefix maximum over the work-group (wmax: 16 words of LDS).
@@ -1789,7 +2539,7 @@ __device__ void vjp_range_body(const BatchedParams& P, const Real* grad, Real* g
   }
   if (arrivals == 1u) {
     __syncthreads();
-    vjp_tail<Real, MAXPER>(P, in, T, d0, d1, grad_density, scratch);
+    vjp_tail<Real, MAXPER>(P, in, T.cdf, T.hdr->inv_n, d0, d1, grad_density, scratch);
     return;
   }
   // one device-scope fence per work-group; the last of the instance's
@@ -1817,19 +2567,28 @@ __device__ void vjp_range_body(const BatchedParams& P, const Real* grad, Real* g
   }
   if (tid == 0) P.done[ch.instance] = 0u;
   __syncthreads();
-  vjp_tail<Real, MAXPER>(P, in, T, h0, h1, grad_density, scratch);
+  vjp_tail<Real, MAXPER>(P, in, T.cdf, T.hdr->inv_n, h0, h1, grad_density, scratch);
 }
 
-// VJP of the instances without bin ranges: one work-group per chunk.
+// VJP of the guide-table instances without bin ranges: one work-group per chunk.
 template <typename Real, int MAXPER>
 __global__ void __launch_bounds__(1024) vjp_kernel(BatchedParams P, const Real* grad,
                                                    Real* grad_density) {
   extern __shared__ __align__(16) uint8_t lds[];
-  const int c = P.vjp_items[blockIdx.x].x;
+  const int c = P.vjp_items[P.n_vjp_cell + blockIdx.x].x;
   if (P.inst[P.chunks[c].instance].global_tables)
     vjp_body<Real, MAXPER, true>(P, grad, grad_density, lds, c);
   else
     vjp_body<Real, MAXPER, false>(P, grad, grad_density, lds, c);
+}
+
+// VJP of the float32 cell-table instances: one work-group per chunk (the first
+// n_vjp_cell items).
+template <int MAXPER>
+__global__ void __launch_bounds__(1024) vjp_cell_kernel(BatchedParams P, const float* grad,
+                                                        float* grad_density) {
+  extern __shared__ __align__(16) uint8_t lds[];
+  fast_vjp_body<MAXPER>(P, grad, grad_density, lds, P.vjp_items[blockIdx.x].x);
 }
 
 // VJP of the instances with bin ranges: one work-group per (chunk, range).
@@ -1865,6 +2624,22 @@ hipError_t launch_tables(const BatchedParams& P, int threads, bool sequential, i
 template <typename Real>
 hipError_t launch_sample(const BatchedParams& P, int threads, bool sequential, int64_t lds,
                          hipStream_t stream) {
+  if constexpr (sizeof(Real) == 4) {
+    if (P.n_cell_chunks > 0) {
+      auto kern = P.use_window ? (sequential ? sample_cell_kernel<true, true>
+                                             : sample_cell_kernel<true, false>)
+                               : (sequential ? sample_cell_kernel<false, true>
+                                             : sample_cell_kernel<false, false>);
+      hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+      if (e != hipSuccess) return e;
+      hipLaunchKernelGGL(kern, dim3(unsigned(P.n_cell_chunks)), dim3(threads), size_t(lds),
+                         stream, P);
+      e = hipGetLastError();
+      if (e != hipSuccess) return e;
+    }
+  }
+  if (P.n_chunks - P.n_cell_chunks == 0) return hipSuccess;
   auto kern = P.use_window ? (sequential ? sample_kernel<Real, true, true>
                                          : sample_kernel<Real, true, false>)
                            : (sequential ? sample_kernel<Real, false, true>
@@ -1872,7 +2647,8 @@ hipError_t launch_sample(const BatchedParams& P, int threads, bool sequential, i
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(kern, dim3(P.n_chunks), dim3(threads), size_t(lds), stream, P);
+  hipLaunchKernelGGL(kern, dim3(unsigned(P.n_chunks - P.n_cell_chunks)), dim3(threads),
+                     size_t(lds), stream, P);
   return hipGetLastError();
 }
 
@@ -1892,10 +2668,25 @@ hipError_t launch_pack(const BatchedParams& P, int threads, void* packed, int32_
 template <typename Real>
 hipError_t launch_vjp(const BatchedParams& P, int threads, int64_t lds, int maxper,
                       const void* grad, void* grad_density, hipStream_t stream) {
-  // plain items, then bin-range items (independent instances: either order)
+  // cell-table items, other plain items, then bin-range items (independent
+  // instances: any order)
 #define QL_VJP_CASE(MP)                                                                      \
   if (maxper <= MP) {                                                                        \
-    const int64_t grids[2] = {P.n_vjp_plain, P.n_vjp_items - P.n_vjp_plain};                 \
+    if constexpr (sizeof(Real) == 4) {                                                       \
+      if (P.n_vjp_cell > 0) {                                                                \
+        auto kern = vjp_cell_kernel<MP>;                                                     \
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),             \
+                                           hipFuncAttributeMaxDynamicSharedMemorySize,      \
+                                           int(lds));                                        \
+        if (e != hipSuccess) return e;                                                       \
+        hipLaunchKernelGGL(kern, dim3(unsigned(P.n_vjp_cell)), dim3(threads), size_t(lds),   \
+                           stream, P, static_cast<const float*>(grad),                       \
+                           static_cast<float*>(grad_density));                               \
+        e = hipGetLastError();                                                               \
+        if (e != hipSuccess) return e;                                                       \
+      }                                                                                      \
+    }                                                                                        \
+    const int64_t grids[2] = {P.n_vjp_plain - P.n_vjp_cell, P.n_vjp_items - P.n_vjp_plain};  \
     for (int kind = 0; kind < 2; ++kind) {                                                   \
       if (grids[kind] == 0) continue;                                                        \
       auto kern = kind == 0 ? vjp_kernel<Real, MP> : vjp_range_kernel<Real, MP>;             \

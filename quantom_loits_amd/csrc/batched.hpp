@@ -12,8 +12,11 @@ namespace qloits {
 struct BatchedParams {
   const InstanceDesc* inst;
   const ChunkDesc* chunks;
-  const int2* vjp_items;  // VJP work items {chunk, bin range}: plain items, then ranged
+  const int2* vjp_items;  // VJP work items {chunk, bin range}: cell-table items, other plain
+                          // items, then ranged
+  const int32_t* fwd_order;  // chunks of the cell-table instances first, then the others
   int64_t n_instances, n_chunks, n_vjp_items, n_vjp_plain;
+  int64_t n_cell_chunks, n_vjp_cell;
   const void* x;
   const void* density;
   uint8_t* tables;

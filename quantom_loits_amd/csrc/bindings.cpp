@@ -558,7 +558,7 @@ class BatchedPlan {
   // instances: CPU int64 [n, 8] = node_off, n_nodes, events, out_off,
   // counter_base, stream, accept, 0 (counter/stream as two's-complement int64)
   BatchedPlan(at::Tensor instances, int64_t dtype, int64_t threads, int64_t guide_shift,
-              int64_t chunk_events, int64_t scan_mode, int64_t persistent) {
+              int64_t chunk_events, int64_t scan_mode, int64_t persistent, int64_t table_format) {
     instances = instances.contiguous();
     if (instances.device().is_cuda() || instances.scalar_type() != at::kLong ||
         instances.dim() != 2 || instances.size(1) != 8)
@@ -582,6 +582,7 @@ class BatchedPlan {
     tuning_.chunk_events = chunk_events;
     tuning_.scan_mode = int32_t(scan_mode);
     tuning_.persistent = int32_t(persistent);
+    tuning_.table_format = int32_t(table_format);
     check(loits_hip_batched_layout(inst_.data(), n, int(dtype), &tuning_, &layout_),
           "loits_hip_batched_layout");
     desc_host_ = at::empty({layout_.desc_bytes}, at::TensorOptions().dtype(at::kByte));
@@ -607,6 +608,7 @@ class BatchedPlan {
     d["partial_doubles"] = layout_.partial_doubles;
     d["global_instances"] = layout_.global_instances;
     d["range_instances"] = layout_.range_instances;
+    d["cell_instances"] = layout_.cell_instances;
     d["grid_backward"] = layout_.grid_backward;
     d["backward_plain"] = layout_.backward_plain;
     d["dtype"] = layout_.dtype;
@@ -748,7 +750,7 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
         py::arg("row_start"), py::arg("lo"), py::arg("hi"), py::arg("bins"), py::arg("width"),
         py::arg("grad_hist"), py::arg("stream") = py::none());
   py::class_<BatchedPlan>(m, "BatchedPlan")
-      .def(py::init<at::Tensor, int64_t, int64_t, int64_t, int64_t, int64_t, int64_t>())
+      .def(py::init<at::Tensor, int64_t, int64_t, int64_t, int64_t, int64_t, int64_t, int64_t>())
       .def("layout", &BatchedPlan::layout)
       .def("descriptors", &BatchedPlan::descriptors)
       .def("descriptors_for_streams", &BatchedPlan::descriptors_for_streams)

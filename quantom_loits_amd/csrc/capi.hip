@@ -11,6 +11,7 @@
 
 #include "batched.hpp"
 #include "capi_internal.hpp"
+#include "fast_tables.hpp"
 #include "loits_device.hpp"
 #include "loits_hip.h"
 
@@ -77,7 +78,9 @@ struct VjpItem {
 struct Plan {
   std::vector<InstanceDesc> inst;
   std::vector<ChunkDesc> chunks;
-  std::vector<VjpItem> items;  // VJP work items: one per chunk, then one per (chunk, range)
+  std::vector<VjpItem> items;  // VJP work items: one per chunk (cell-table instances first),
+                               // then one per (chunk, range)
+  std::vector<int32_t> fwd_order;  // chunks of the cell-table instances first
   int64_t n_plain = 0;
   loits_hip_layout layout;
 };
@@ -95,6 +98,8 @@ int make_plan(const loits_hip_instance* instances, int64_t n, int dtype,
     return fail(LOITS_HIP_EINVAL, "batched: guide_shift must be in [0, 3]");
   if (tuning->persistent != 0)
     return fail(LOITS_HIP_EINVAL, "batched: persistent is reserved and must be 0");
+  if (tuning->table_format != 0 && tuning->table_format != 1)
+    return fail(LOITS_HIP_EINVAL, "batched: table_format must be 0 or 1");
   const int64_t per_iter = int64_t(4) * threads;
   int64_t chunk_events = tuning->chunk_events;
   if (chunk_events <= 0 || chunk_events > (int64_t(1) << 30)) chunk_events = int64_t(1) << 30;
@@ -112,11 +117,14 @@ int make_plan(const loits_hip_instance* instances, int64_t n, int dtype,
   L.persistent = tuning->persistent;
   int64_t out = 0, table_off = 0, mask_words = 0, partial_doubles = 0;
   int64_t lds_f = 0, lds_b = 0, lds_t = 0;
-  int32_t n_global = 0, n_ranged = 0;
+  int32_t n_global = 0, n_ranged = 0, n_cell = 0;
   std::vector<VjpItem> ranged;  // appended after the plain items
+  std::vector<VjpItem> guide_items;  // plain items of guide-table instances (after the cell items)
+  std::vector<int32_t> guide_chunks;
   plan->inst.resize(size_t(n));
   plan->chunks.clear();
   plan->items.clear();
+  plan->fwd_order.clear();
   for (int64_t i = 0; i < n; ++i) {
     const loits_hip_instance& s = instances[i];
     if (s.n_nodes < 2 || s.n_nodes > 32767)
@@ -138,47 +146,69 @@ int make_plan(const loits_hip_instance* instances, int64_t n, int dtype,
     d.counter_base = s.counter_base;
     d.stream = s.stream;
     d.accept = s.accept ? 1 : 0;
-    // Forward: the finest guide whose LDS build fits; when none does, the
-    // instance keeps its per-bin tables in global memory (built by a pre-pass)
-    // and only the guide is staged in LDS.  VJP: tables and histograms in LDS
-    // at the finest guide that fits both kernels; otherwise bin ranges.
-    const int g_want = std::min(kMaxGuideLog2,
-                                std::max(kMinGuideLog2, ceil_log2(s.n_nodes) + tuning->guide_shift));
-    auto build_bytes = [&](int g) { return lds_build_layout(s.n_nodes, g, coef_size).bytes; };
-    auto vjp_bytes = [&](int g) {
-      return lds_table_bytes(s.n_nodes, g, coef_size) + pad4(s.n_nodes) * 16;
-    };
-    d.guide_log2 = g_want;
-    while (d.guide_log2 > kMinGuideLog2 &&
-           std::max(build_bytes(d.guide_log2), vjp_bytes(d.guide_log2)) > kMaxLds)
-      --d.guide_log2;
-    if (vjp_bytes(d.guide_log2) > kMaxLds) {  // the VJP takes bin ranges; fit the build only
+    // float32 curves of at most kFastMaxNodes nodes take the cell tables
+    // (fast_tables.hpp) when both kernels' LDS images fit
+    if (tuning->table_format == 0 && dtype == LOITS_HIP_F32 && s.n_nodes <= kFastMaxNodes &&
+        chunk_events <= kFastMaxChunk) {
+      int g = fast_guide_log2(s.n_nodes);
+      while (g > kMinGuideLog2 && std::max(fast_lds_forward(s.n_nodes, g).bytes,
+                                           fast_lds_vjp(s.n_nodes, g).bytes) > kMaxLds)
+        --g;
+      if (std::max(fast_lds_forward(s.n_nodes, g).bytes, fast_lds_vjp(s.n_nodes, g).bytes) <=
+          kMaxLds)
+        d.fast_log2 = g;
+    }
+    if (d.fast_log2 > 0) {
+      ++n_cell;
+      d.guide_log2 = d.fast_log2;
+      d.table_off = table_off;
+      table_off += align16(fast_blob_layout(s.n_nodes, d.fast_log2).bytes);
+      lds_f = std::max(lds_f, fast_lds_forward(s.n_nodes, d.fast_log2).bytes);
+      lds_t = std::max(lds_t, fast_lds_forward(s.n_nodes, d.fast_log2).bytes);
+      lds_b = std::max(lds_b, fast_lds_vjp(s.n_nodes, d.fast_log2).bytes);
+    } else {
+      // Forward: the finest guide whose LDS build fits; when none does, the
+      // instance keeps its per-bin tables in global memory (built by a pre-pass)
+      // and only the guide is staged in LDS.  VJP: tables and histograms in LDS
+      // at the finest guide that fits both kernels; otherwise bin ranges.
+      const int g_want = std::min(kMaxGuideLog2,
+                                  std::max(kMinGuideLog2, ceil_log2(s.n_nodes) + tuning->guide_shift));
+      auto build_bytes = [&](int g) { return lds_build_layout(s.n_nodes, g, coef_size).bytes; };
+      auto vjp_bytes = [&](int g) {
+        return lds_table_bytes(s.n_nodes, g, coef_size) + pad4(s.n_nodes) * 16;
+      };
       d.guide_log2 = g_want;
-      while (d.guide_log2 > kMinGuideLog2 && build_bytes(d.guide_log2) > kMaxLds) --d.guide_log2;
-      d.vjp_ranges = vjp_range_count(s.n_nodes);
-    }
-    d.global_tables = build_bytes(d.guide_log2) > kMaxLds ? 1 : 0;
-    if (d.global_tables) d.guide_log2 = g_want;
-    d.table_off = table_off;
-    const TableLayout tl = table_layout(s.n_nodes, d.guide_log2, coef_size);
-    table_off += align16(tl.bytes);
-    const int64_t guide_bytes = (int64_t(2) << d.guide_log2) * 4;
-    if (d.global_tables) {
-      ++n_global;
-      lds_f = std::max(lds_f, guide_bytes);
-      lds_t = std::max(lds_t, lds_global_build_bytes(d.guide_log2));
-    } else {
-      lds_f = std::max(lds_f, build_bytes(d.guide_log2));
-      lds_t = std::max(lds_t, build_bytes(d.guide_log2));
-    }
-    if (d.vjp_ranges > 0) {
-      // (a curve that turns out not to be monotone takes the global-table VJP,
-      // which stages the guide)
-      ++n_ranged;
-      lds_b = std::max(lds_b, std::max(guide_bytes,
-                                       lds_range_bytes(vjp_range_width(s.n_nodes, d.vjp_ranges))));
-    } else {
-      lds_b = std::max(lds_b, (tl.cdf - tl.thr) + pad4(s.n_nodes) * 16);
+      while (d.guide_log2 > kMinGuideLog2 &&
+             std::max(build_bytes(d.guide_log2), vjp_bytes(d.guide_log2)) > kMaxLds)
+        --d.guide_log2;
+      if (vjp_bytes(d.guide_log2) > kMaxLds) {  // the VJP takes bin ranges; fit the build only
+        d.guide_log2 = g_want;
+        while (d.guide_log2 > kMinGuideLog2 && build_bytes(d.guide_log2) > kMaxLds) --d.guide_log2;
+        d.vjp_ranges = vjp_range_count(s.n_nodes);
+      }
+      d.global_tables = build_bytes(d.guide_log2) > kMaxLds ? 1 : 0;
+      if (d.global_tables) d.guide_log2 = g_want;
+      d.table_off = table_off;
+      const TableLayout tl = table_layout(s.n_nodes, d.guide_log2, coef_size);
+      table_off += align16(tl.bytes);
+      const int64_t guide_bytes = (int64_t(2) << d.guide_log2) * 4;
+      if (d.global_tables) {
+        ++n_global;
+        lds_f = std::max(lds_f, guide_bytes);
+        lds_t = std::max(lds_t, lds_global_build_bytes(d.guide_log2));
+      } else {
+        lds_f = std::max(lds_f, build_bytes(d.guide_log2));
+        lds_t = std::max(lds_t, build_bytes(d.guide_log2));
+      }
+      if (d.vjp_ranges > 0) {
+        // (a curve that turns out not to be monotone takes the global-table VJP,
+        // which stages the guide)
+        ++n_ranged;
+        lds_b = std::max(lds_b, std::max(guide_bytes,
+                                         lds_range_bytes(vjp_range_width(s.n_nodes, d.vjp_ranges))));
+      } else {
+        lds_b = std::max(lds_b, (tl.cdf - tl.thr) + pad4(s.n_nodes) * 16);
+      }
     }
     L.max_nodes = std::max(L.max_nodes, s.n_nodes);
     L.total_nodes = std::max(L.total_nodes, s.node_off + s.n_nodes);
@@ -201,24 +231,35 @@ int make_plan(const loits_hip_instance* instances, int64_t n, int dtype,
       mask_words += per_seg * nsegs * 4;
       ch.partial_off = partial_doubles;  // VJP overflow sums, then partials
       partial_doubles += 2 * pad4(s.n_nodes);
-      if (d.vjp_ranges == 0)
-        plan->items.push_back(VjpItem{int32_t(plan->chunks.size()), 0});
-      else
-        for (int r = 0; r < d.vjp_ranges; ++r)
-          ranged.push_back(VjpItem{int32_t(plan->chunks.size()), r});
+      const int32_t ci = int32_t(plan->chunks.size());
+      if (d.fast_log2 > 0) {
+        plan->items.push_back(VjpItem{ci, 0});
+        plan->fwd_order.push_back(ci);
+      } else {
+        guide_chunks.push_back(ci);
+        if (d.vjp_ranges == 0)
+          guide_items.push_back(VjpItem{ci, 0});
+        else
+          for (int r = 0; r < d.vjp_ranges; ++r) ranged.push_back(VjpItem{ci, r});
+      }
       plan->chunks.push_back(ch);
     }
     out += s.events;
   }
   L.capacity = out;
   L.n_chunks = int64_t(plan->chunks.size());
+  L.cell_chunks = int64_t(plan->fwd_order.size());
+  L.backward_cell = int64_t(plan->items.size());
+  plan->fwd_order.insert(plan->fwd_order.end(), guide_chunks.begin(), guide_chunks.end());
+  plan->items.insert(plan->items.end(), guide_items.begin(), guide_items.end());
   plan->n_plain = int64_t(plan->items.size());
   plan->items.insert(plan->items.end(), ranged.begin(), ranged.end());
   if (plan->items.size() > size_t(INT32_MAX))
     return fail(LOITS_HIP_EUNSUPPORTED, "batched: too many VJP work items");
   L.desc_bytes = align16(n * int64_t(sizeof(InstanceDesc))) +
                  align16(L.n_chunks * int64_t(sizeof(ChunkDesc))) +
-                 int64_t(plan->items.size()) * int64_t(sizeof(VjpItem));
+                 align16(int64_t(plan->items.size()) * int64_t(sizeof(VjpItem))) +
+                 L.n_chunks * int64_t(sizeof(int32_t));
   L.table_bytes = std::max<int64_t>(table_off, 16);
   L.lds_forward = lds_f;
   L.lds_backward = lds_b;
@@ -242,6 +283,7 @@ int make_plan(const loits_hip_instance* instances, int64_t n, int dtype,
   L.backward_plain = plan->n_plain;
   L.global_instances = n_global;
   L.range_instances = n_ranged;
+  L.cell_instances = n_cell;
   return LOITS_HIP_OK;
 }
 
@@ -252,12 +294,16 @@ BatchedParams params_of(const loits_hip_batched_args* a) {
   P.inst = reinterpret_cast<const InstanceDesc*>(desc);
   const int64_t chunks_at = align16(L.n_instances * int64_t(sizeof(InstanceDesc)));
   P.chunks = reinterpret_cast<const ChunkDesc*>(desc + chunks_at);
-  P.vjp_items = reinterpret_cast<const int2*>(
-      desc + chunks_at + align16(L.n_chunks * int64_t(sizeof(ChunkDesc))));
+  const int64_t items_at = chunks_at + align16(L.n_chunks * int64_t(sizeof(ChunkDesc)));
+  P.vjp_items = reinterpret_cast<const int2*>(desc + items_at);
+  P.fwd_order = reinterpret_cast<const int32_t*>(
+      desc + items_at + align16(int64_t(L.grid_backward) * int64_t(sizeof(VjpItem))));
   P.n_instances = L.n_instances;
   P.n_chunks = L.n_chunks;
   P.n_vjp_items = L.grid_backward;
   P.n_vjp_plain = L.backward_plain;
+  P.n_cell_chunks = L.cell_chunks;
+  P.n_vjp_cell = L.backward_cell;
   P.x = a->x;
   P.density = a->density;
   P.tables = static_cast<uint8_t*>(a->tables);
@@ -384,8 +430,11 @@ int loits_hip_batched_encode(const loits_hip_instance* instances, int64_t n,
     std::memcpy(out, plan.inst.data(), plan.inst.size() * sizeof(InstanceDesc));
     const int64_t chunks_at = align16(n * int64_t(sizeof(InstanceDesc)));
     std::memcpy(out + chunks_at, plan.chunks.data(), plan.chunks.size() * sizeof(ChunkDesc));
-    std::memcpy(out + chunks_at + align16(int64_t(plan.chunks.size() * sizeof(ChunkDesc))),
-                plan.items.data(), plan.items.size() * sizeof(VjpItem));
+    const int64_t items_at =
+        chunks_at + align16(int64_t(plan.chunks.size() * sizeof(ChunkDesc)));
+    std::memcpy(out + items_at, plan.items.data(), plan.items.size() * sizeof(VjpItem));
+    std::memcpy(out + items_at + align16(int64_t(plan.items.size() * sizeof(VjpItem))),
+                plan.fwd_order.data(), plan.fwd_order.size() * sizeof(int32_t));
     return LOITS_HIP_OK;
   } catch (const std::exception& e) {
     return fail(LOITS_HIP_ERUNTIME, std::string("batched_encode: ") + e.what());

@@ -1,0 +1,157 @@
+// fast_tables.hpp -- cell tables of the float32 batched path (curves of at
+// most kFastMaxNodes nodes).
+//
+// The bin of key k (u = k 2^-24) is the reference's count-based interval
+// (cpp/loits_core.cpp:12-18), j = clamp(#{t : T_t <= k} - 1, 0, n - 2) with
+// T_t = ceil(C_t 2^24).  Since T_0 = 0 and the clamp drops the last
+// threshold, j = #{i in 1..n-2 : S_i <= k} for the sorted thresholds S
+// (S = T when the CDF is monotone): the "splits" S_1..S_{n-2}.
+//
+// The 2^24 keys are cut into G = 2^g cells of 2^s keys (s = 24 - g).  Cell c
+// covers [L, L + 2^s), L = c 2^s.  Its word holds jL = #{splits <= L} and,
+// when at most one split lies strictly inside (L, L + 2^s), that split's
+// offset p (2^s when there is none), so that
+//     j = jL + (k - L >= p)                              ("direct" word)
+// Cells with two or more inside splits point to 2^f level-2 words covering
+// 2^(s-f) keys each, in the same format (sub-cells of a single key never hold
+// an inside split), or, past the level-2 budget, to a bisection of the sorted
+// splits ("search" word).
+//
+// Word encodings (32 bits):
+//   direct:   bit 31 clear; bits 0-14 jL, bits 15-28 p
+//   indirect: bits 31 set, 30 clear; bits 0-19 index of the first level-2 word
+//             in the cell-word array, bits 24-27 f
+//   search:   bits 31, 30 set; bits 0-14 jL, bits 15-29 number of inside splits
+//             (the splits S_{jL+1} .. S_{jL+cnt})
+//
+// Per-bin interpolation data, float4 {M_j, A_j, T_j (bits), 0}: the value of
+// key k in bin j is fma(M_j, float(k - T_j), A_j) with M_j = m_j 2^-24 and
+// A_j = x_j + m_j (T_j 2^-24 - C_j) rounded from float64 (the reference's
+// x_j + m_j (u - C_j), cpp/loits_core.cpp:190-195, with the exact integer
+// k - T_j).
+#pragma once
+
+#include <cstdint>
+
+#include "loits_device.hpp"
+
+namespace qloits {
+
+constexpr int kFastMaxNodes = 4096;
+constexpr int64_t kFastMaxChunk = int64_t(1) << 18;  // keeps the VJP's integer sums < 2^62
+constexpr uint32_t kFastMulti = 0x80000000u;
+constexpr uint32_t kFastSearch = 0x40000000u;
+constexpr uint32_t kFastTables = 4u;  // TableHeader flag: tables in the cell format
+// VJP fixed point: |round(g 2^e)| < 2^kFastQBits, so that sum q k over a chunk
+// of at most 2^18 keys below 2^24 stays below 2^62.
+constexpr int kFastQBits = 20;
+
+__host__ __device__ __forceinline__ int fast_l2_cap(int64_t n) { return n <= 2048 ? 4096 : 2048; }
+
+// Blob of one instance after the TableHeader (byte offsets from table_off):
+//   srt[pad4(pow2(n))] u32   sorted thresholds (room for a bitonic sort)
+//   cells[G + l2cap]  u32    level-1 then level-2 words
+//   own[pad4(n)]      u32    each bin's own threshold T_j (= srt when monotone)
+//   cdf[pad4(n)]      f64
+struct FastBlobLayout {
+  int64_t srt, cells, own, cdf, bytes;
+};
+
+__host__ __device__ __forceinline__ FastBlobLayout fast_blob_layout(int64_t n, int g) {
+  FastBlobLayout b;
+  b.srt = sizeof(TableHeader);
+  b.cells = b.srt + pad4(pow2_ceil(n)) * 4;
+  b.own = b.cells + ((int64_t(1) << g) + fast_l2_cap(n)) * 4;
+  b.cdf = b.own + pad4(n) * 4;
+  b.bytes = b.cdf + pad4(n) * 8;
+  return b;
+}
+
+// LDS of the forward (build + sampling), byte offsets:
+//   cells[G + l2cap] u32 (the float64 CDF aliases its front during the build)
+//   list[l2cap / 4]  uint2 (cells with level-2 words, used during the build)
+//   bins[pad4(n)]    float4
+//   srt[pad4(pow2(n))] u32
+struct FastLdsForward {
+  int64_t cells, list, bins, srt, bytes;
+};
+
+__host__ __device__ __forceinline__ FastLdsForward fast_lds_forward(int64_t n, int g) {
+  FastLdsForward l;
+  l.cells = 0;
+  l.list = l.cells + ((int64_t(1) << g) + fast_l2_cap(n)) * 4;
+  l.bins = l.list + (fast_l2_cap(n) / 4) * 8;
+  l.srt = l.bins + pad4(n) * 16;
+  l.bytes = l.srt + pad4(pow2_ceil(n)) * 4;
+  return l;
+}
+
+// LDS of the VJP: cells[G + l2cap] | srt[pad4(n)] | a0[kFastMaxNodes] | a1[kFastMaxNodes]
+// (64-bit integer histograms at a fixed distance, one address for both adds).
+struct FastLdsVjp {
+  int64_t cells, srt, a0, a1, bytes;
+};
+
+__host__ __device__ __forceinline__ FastLdsVjp fast_lds_vjp(int64_t n, int g) {
+  FastLdsVjp l;
+  l.cells = 0;
+  l.srt = ((int64_t(1) << g) + fast_l2_cap(n)) * 4;
+  l.a0 = l.srt + pad4(n) * 4;
+  l.a0 = (l.a0 + 15) & ~int64_t(15);
+  l.a1 = l.a0 + int64_t(kFastMaxNodes) * 8;
+  l.bytes = l.a1 + int64_t(kFastMaxNodes) * 8;
+  return l;
+}
+
+// Cells for n nodes: 2^(ceil log2 n + 3), within [2^12, 2^14].
+__host__ __device__ __forceinline__ int fast_guide_log2(int64_t n) {
+  int l = 0;
+  while ((int64_t(1) << l) < n) ++l;
+  l += 3;
+  return l < 12 ? 12 : (l > 14 ? 14 : l);
+}
+
+__host__ __device__ __forceinline__ uint32_t fast_direct(uint32_t jl, uint32_t p) {
+  return jl | (p << 15);
+}
+
+// Bin of in-cell offset `off` from a direct word.
+__host__ __device__ __forceinline__ int fast_direct_bin(uint32_t w, uint32_t off) {
+  return int(w & 0x7FFFu) + (off >= ((w >> 15) & 0x3FFFu) ? 1 : 0);
+}
+
+// Number of entries of the ascending a[lo, lo + len) that are <= key.
+__host__ __device__ __forceinline__ int count_le(const uint32_t* a, int lo, int len,
+                                                 uint32_t key) {
+  int c = 0;
+  while (len > 0) {
+    const int half = len >> 1;
+    if (a[lo + half] <= key) {
+      lo += half + 1;
+      c += half + 1;
+      len -= half + 1;
+    } else {
+      len = half;
+    }
+  }
+  return c;
+}
+
+// Bin of key k whose level-1 word w has bit 31 set (indirect or search).
+__host__ __device__ __forceinline__ int fast_multi_bin(uint32_t w, uint32_t k, int s,
+                                                       const uint32_t* cells,
+                                                       const uint32_t* srt) {
+  uint32_t off = k & ((1u << s) - 1u);
+  if ((w & kFastSearch) == 0u) {  // indirect: one level-2 word
+    const int f = int((w >> 24) & 0xFu);
+    const int ss = s - f;
+    const uint32_t w2 = cells[(w & 0xFFFFFu) + (off >> ss)];
+    off &= (1u << ss) - 1u;
+    if ((w2 & kFastMulti) == 0u) return fast_direct_bin(w2, off);
+    w = w2;
+  }
+  const int jl = int(w & 0x7FFFu), cnt = int((w >> 15) & 0x7FFFu);
+  return jl + count_le(srt, jl + 1, cnt, k);
+}
+
+}  // namespace qloits

@@ -205,6 +205,8 @@ struct InstanceDesc {
   int32_t n_chunks;
   int32_t global_tables; // 1: per-bin tables too large for LDS stay in global memory
   int32_t vjp_ranges;    // 0: VJP tables and histograms in LDS; R > 0: R bin ranges
+  int32_t fast_log2;     // > 0: float32 cell-table path with 2^fast_log2 cells (fast_tables.hpp)
+  int32_t reserved;
 };
 
 // A chunk is the work of one work-group; each of its waves owns a contiguous

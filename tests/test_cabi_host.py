@@ -42,7 +42,8 @@ class Instance(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("threads", C.c_int32), ("guide_shift", C.c_int32), ("chunk_events", C.c_int64),
-                ("scan_mode", C.c_int32), ("persistent", C.c_int32)]
+                ("scan_mode", C.c_int32), ("persistent", C.c_int32), ("table_format", C.c_int32),
+                ("reserved", C.c_int32)]
 
 
 class Layout(C.Structure):
@@ -51,7 +52,8 @@ class Layout(C.Structure):
         "table_bytes", "scratch_bytes", "lds_forward", "lds_backward", "lds_tables", "n_segments",
         "mask_words", "partial_doubles")] + [(n, C.c_int32) for n in (
             "dtype", "threads", "scan_mode", "persistent", "grid_forward", "grid_backward",
-            "global_instances", "range_instances")] + [("backward_plain", C.c_int64)]
+            "global_instances", "range_instances", "cell_instances", "reserved")] + \
+        [("backward_plain", C.c_int64), ("cell_chunks", C.c_int64), ("backward_cell", C.c_int64)]
 
 
 def make_instances(nodes, events):
@@ -189,12 +191,26 @@ def test_large_curves_use_global_tables(lib):
 
 
 def test_vjp_bin_ranges_without_global_tables(lib):
-    """4096 nodes: the forward builds its tables in LDS, the VJP takes one bin range;
-    2500 nodes keep both in LDS."""
+    """Guide tables (table_format 1): at 4096 nodes the forward builds its tables
+    in LDS and the VJP takes one bin range; 2500 nodes keep both in LDS.  With
+    cell tables (the default) both curves keep everything in LDS."""
     inst = make_instances([4096, 2500], [300000, 10])
-    tun = Tuning(1024, 2, 1 << 17, 0, 0)
     lay = Layout()
+    cell = Tuning(1024, 2, 1 << 17, 0, 0)
+    assert lib.loits_hip_batched_layout(inst, 2, 1, C.byref(cell), C.byref(lay)) == 0
+    assert lay.cell_instances == 2 and lay.global_instances == 0 and lay.range_instances == 0
+    assert lay.n_chunks == 3 + 1 and lay.grid_backward == 4 and lay.backward_plain == 4
+    assert lay.cell_chunks == 4 and lay.backward_cell == 4
+    assert max(lay.lds_forward, lay.lds_backward) <= 160 * 1024 - 2048
+    # float64 curves and chunks above 2^18 events keep the guide tables
+    assert lib.loits_hip_batched_layout(inst, 2, 0, C.byref(cell), C.byref(lay)) == 0
+    assert lay.cell_instances == 0
+    big = Tuning(1024, 2, 1 << 19, 0, 0)
+    assert lib.loits_hip_batched_layout(inst, 2, 1, C.byref(big), C.byref(lay)) == 0
+    assert lay.cell_instances == 0
+    tun = Tuning(1024, 2, 1 << 17, 0, 0, 1)
     assert lib.loits_hip_batched_layout(inst, 2, 1, C.byref(tun), C.byref(lay)) == 0
+    assert lay.cell_instances == 0 and lay.cell_chunks == 0 and lay.backward_cell == 0
     assert lay.global_instances == 0 and lay.range_instances == 1
     assert lay.n_chunks == 3 + 1 and lay.grid_backward == 4 and lay.backward_plain == 1
     assert lay.lds_backward <= 160 * 1024 - 2048

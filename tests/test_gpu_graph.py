@@ -26,10 +26,11 @@ def _grad_err(got, ref, node_off, n_nodes):
     return worst
 
 
-@pytest.mark.parametrize("dtype,window", [(torch.float32, (0.05, 0.95)),
-                                          (torch.float32, None),
-                                          (torch.float64, (0.1, 0.9))])
-def test_graph_replays_match_eager_and_oracle(oracle_port, hip_device, dtype, window):
+@pytest.mark.parametrize("dtype,window,fmt", [(torch.float32, (0.05, 0.95), 0),
+                                              (torch.float32, (0.05, 0.95), 1),
+                                              (torch.float32, None, 0),
+                                              (torch.float64, (0.1, 0.9), 0)])
+def test_graph_replays_match_eager_and_oracle(oracle_port, hip_device, dtype, window, fmt):
     from quantom_loits_amd.batched import BatchedLOITS, Tuning
     from quantom_loits_amd.workloads import ragged_family
 
@@ -40,7 +41,8 @@ def test_graph_replays_match_eager_and_oracle(oracle_port, hip_device, dtype, wi
     accept = np.ones(n, np.uint8)
     if window is not None:  # (no window, all accepted: the dense path, no pack)
         accept[2] = 0
-    tuning = Tuning(threads=512, chunk_events=8192, scan_mode=1 if dtype == torch.float64 else 0)
+    tuning = Tuning(threads=512, chunk_events=8192, scan_mode=1 if dtype == torch.float64 else 0,
+                    table_format=fmt)
     s = BatchedLOITS(w["n_nodes"], w["events"], base, np.zeros(n, np.uint64), accept,
                      w["node_off"], dtype=dtype, window=window, tuning=tuning)
     x = torch.from_numpy(w["x"]).to(dtype).cuda()
@@ -53,7 +55,7 @@ def test_graph_replays_match_eager_and_oracle(oracle_port, hip_device, dtype, wi
 
     replays = 4
     for r in range(replays):
-        print("graph replay", dtype, window, r, flush=True)
+        print("graph replay", dtype, window, fmt, r, flush=True)
         rr = np.random.default_rng(100 + r)
         dens = w["density"] * rr.uniform(0.5, 2.0, w["density"].shape)
         streams = rr.integers(0, 2 ** 33, n).astype(np.uint64)
