@@ -285,7 +285,8 @@ typedef struct loits_hip_batched_args {
   int32_t reserved;
   double window_lo, window_hi;
   void* tables;            /* layout->table_bytes */
-  void* scratch;           /* layout->scratch_bytes */
+  void* scratch;           /* layout->scratch_bytes, zero-filled before its first call;
+                              the kernels return their counters to zero */
   void* events;            /* [capacity] values, layout->dtype */
   int64_t* kept;           /* [n_instances] kept events per instance */
   int64_t* total;          /* device scalar: total kept */

@@ -116,22 +116,22 @@ __device__ __forceinline__ Tables<Real> tables_of(uint8_t* base, const InstanceD
   return t;
 }
 
-// 16-byte copies by the whole work-group (LDS <-> global), four loads per
+// 16-byte copies by the whole work-group (LDS <-> global), eight loads per
 // thread in flight before their stores (an index past the end reloads the
 // last unit and stores nothing).
 __device__ __forceinline__ void lds_copy(void* dst, const void* src, int64_t bytes) {
   const uint4* s = static_cast<const uint4*>(src);
   uint4* d = static_cast<uint4*>(dst);
   const int64_t units = bytes / 16, nt = blockDim.x;
-  for (int64_t i = threadIdx.x; i < units; i += 4 * nt) {
-    uint4 v[4];
+  for (int64_t i = threadIdx.x; i < units; i += 8 * nt) {
+    uint4 v[8];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < 8; ++k) {
       const int64_t u = i + k * nt;
       v[k] = s[u < units ? u : units - 1];
     }
 #pragma unroll
-    for (int k = 0; k < 4; ++k)
+    for (int k = 0; k < 8; ++k)
       if (i + k * nt < units) d[i + k * nt] = v[k];
   }
 }
@@ -152,6 +152,7 @@ __device__ __forceinline__ int64_t guide_bytes(const Tables<Real>& T, const Inst
 struct BuildShared {
   double scratch[16];
   double norm, inv;
+  uint32_t wtot[64];  // per-wave totals of the cell-table scans
   uint32_t bits;  // AND over nodes: 1 monotone, 2 finite, 4 x > 0, 8 x < 0
   uint32_t flags;
   int assigned, used;
@@ -663,8 +664,13 @@ __device__ void build_fast(const BatchedParams& P, int inst, const InstanceDesc&
   if (!mono) bitonic_sort_u32(srt, pw);
 
   // 4. per-cell counts of the splits S_1 .. S_{n-2}: all (bits 0-15) and those
-  //    at the cell's first key (bits 16-31); splits >= 2^24 lie in no cell
-  for (int64_t c = tid; c < G; c += nt) cells[c] = 0u;
+  //    at the cell's first key (bits 16-31); splits >= 2^24 lie in no cell.
+  //    The level-2 words and the crowded-cell list start cleared.
+  {
+    uint4* c4 = reinterpret_cast<uint4*>(cells);
+    for (int64_t i = tid; i < (G + l2cap) / 4; i += nt) c4[i] = make_uint4(0u, 0u, 0u, 0u);
+    for (int i = tid; i < l2cap / 4; i += nt) list[i] = make_uint2(kNoCell, 0u);
+  }
   __syncthreads();
   for (int64_t t = 1 + tid; t <= n - 2; t += nt) {
     const uint32_t key = srt[t];
@@ -672,107 +678,177 @@ __device__ void build_fast(const BatchedParams& P, int inst, const InstanceDesc&
   }
   __syncthreads();
 
-  // 5. level-1 words.  Wave w owns the cells [w CW, (w + 1) CW), CW = G / nwaves,
-  //    in rows of 64 consecutive cells (lane l takes cell 64 r + l of the range),
-  //    so every LDS access is conflict-free; the prefix over cells in order is a
-  //    wave scan per row (skipped for rows without splits) plus the totals of the
-  //    waves before.  Scanned per cell: splits | level-2 words asked for << 16
-  //    (each total stays below 2^16: at most n - 2 splits and 4 (n - 2) words).
-  //    A crowded cell's list entry sits at its first level-2 word / 4 (every
-  //    crowded cell asks for at least 4 words).
+  // 5. level-1 words: a work-group exclusive scan, in cell order, of
+  //    splits | level-2 words asked for << 16 (each total stays below 2^16: at
+  //    most n - 2 splits and 4 (n - 2) words).  Pass r of thread t holds the
+  //    four cells 4 (r nt + t) .. + 3 (16-byte LDS accesses, conflict-free);
+  //    up to four passes are scanned together.  A crowded cell's list entry
+  //    sits at its first level-2 word / 4 (every crowded cell asks for at least
+  //    four words).
+  const int nwaves = nt >> 6, lane = lane_id();
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   {
-    const int nwaves = nt >> 6, wave = tid >> 6, lane = lane_id();
-    const int64_t CW = G / nwaves;  // a multiple of 64 (G >= 4096, nwaves <= 16)
-    const int rows = int(CW / kWave);
-    uint32_t* cw = cells + int64_t(wave) * CW + lane;
-    auto scan_value = [&](uint32_t v, int* f) -> uint32_t {
-      const uint32_t all = v & 0xFFFFu, inside = all - (v >> 16);
-      *f = inside >= 2u ? guide_fanout_log2(int(inside), s) : 0;
-      return all + (inside >= 2u ? (1u << *f) << 16 : 0u);
-    };
-    uint32_t tot = 0u;
-    for (int r = 0; r < rows; ++r) {
-      int f;
-      tot += scan_value(cw[r * kWave], &f);
+    const int passes = int(G / (4 * int64_t(nt)));
+    uint4* c4 = reinterpret_cast<uint4*>(cells);
+    uint32_t carry = 0u;
+    uint32_t used_hi = 0u;  // end of this thread's last level-2 words
+    for (int r0 = 0; r0 < passes; r0 += 4) {
+      const int np = passes - r0 < 4 ? passes - r0 : 4;
+      uint32_t v[4][4], a[4][4], gs[4], inc[4];
+      bool crowd = false;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const uint4 q4 = r < np ? c4[(r0 + r) * nt + tid] : make_uint4(0u, 0u, 0u, 0u);
+        v[r][0] = q4.x;
+        v[r][1] = q4.y;
+        v[r][2] = q4.z;
+        v[r][3] = q4.w;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const uint32_t all = v[r][q] & 0xFFFFu, inside = all - (v[r][q] >> 16);
+          a[r][q] = all;
+          crowd = crowd || inside >= 2u;
+        }
+      }
+      if (ballot(crowd) != 0ull) {  // some cell of this wave holds two or more inside splits
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const uint32_t inside = a[r][q] - (v[r][q] >> 16);
+            if (inside >= 2u) a[r][q] += (1u << guide_fanout_log2(int(inside), s)) << 16;
+          }
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        gs[r] = a[r][0] + a[r][1] + a[r][2] + a[r][3];
+        inc[r] = wave_inclusive_scan(gs[r]);
+      }
+      if (lane == kWave - 1)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          if (r < np) sh.wtot[r * nwaves + wave] = inc[r];
+      __syncthreads();
+      // prefix over (pass, wave) in order, redone by every wave
+      const uint32_t tv = lane < np * nwaves ? sh.wtot[lane] : 0u;
+      const uint32_t tinc = wave_inclusive_scan(tv);
+      const uint32_t tex = tinc - tv;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        if (r >= np) continue;
+        uint32_t ex = carry + wave_lane(tex, r * nwaves + wave) + inc[r] - gs[r];
+        uint32_t wv[4], exq[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          exq[q] = ex;
+          ex += a[r][q];
+          const uint32_t all = v[r][q] & 0xFFFFu, zero = v[r][q] >> 16;
+          const uint32_t lo = exq[q] & 0xFFFFu;
+          const uint32_t jl = lo + zero;  // splits <= L
+          // no or one inside split: F = lo + all is jl or jl + 1
+          const uint32_t nxt = srt[jl + 1];
+          wv[q] = all - zero == 1u ? ((nxt << (32 - s)) | (lo + all)) : jl;
+        }
+        if (crowd) {
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const uint32_t all = v[r][q] & 0xFFFFu, zero = v[r][q] >> 16, inside = all - zero;
+            if (inside < 2u) continue;
+            const uint32_t jl = (exq[q] & 0xFFFFu) + zero, woff = exq[q] >> 16;
+            const int f = guide_fanout_log2(int(inside), s);
+            if (woff + (1u << f) <= uint32_t(l2cap)) {
+              wv[q] = kCellMulti | (uint32_t(f) << 14) | (woff << 18);
+              list[woff >> 2] = make_uint2(uint32_t(f), jl);
+              used_hi = woff + (1u << f) > used_hi ? woff + (1u << f) : used_hi;
+            } else {
+              wv[q] = kCellMulti | kCellSearch | jl | (inside << 14);
+            }
+          }
+        }
+        c4[(r0 + r) * nt + tid] = make_uint4(wv[0], wv[1], wv[2], wv[3]);
+      }
+      carry += wave_lane(tinc, np * nwaves - 1);
+      if (r0 + 4 < passes) __syncthreads();  // sh.wtot is reused
     }
-    tot = wave_sum(tot);
-    uint32_t* wtot = reinterpret_cast<uint32_t*>(sh.scratch);  // one word per wave
-    for (int i = tid; i < l2cap / 4; i += nt) list[i] = make_uint2(0xFFFFFFFFu, 0u);
-    if (lane == 0) wtot[wave] = tot;
-    __syncthreads();
-    uint32_t run = 0u;
-    for (int w2 = 0; w2 < wave; ++w2) run += wtot[w2];
-    for (int r = 0; r < rows; ++r) {
-      const uint32_t v = cw[r * kWave];
-      const uint32_t lo_key = (uint32_t(wave) * uint32_t(CW) + uint32_t(r * kWave + lane)) << s;
-      if (ballot(v != 0u) == 0ull) {  // no split in these 64 cells
-        cw[r * kWave] = fast_direct(run & 0xFFFFu, S);
-        continue;
-      }
-      int f;
-      const uint32_t a = scan_value(v, &f);
-      const uint32_t inc = wave_inclusive_scan(a);
-      const uint32_t ex = run + inc - a;
-      const uint32_t all = v & 0xFFFFu, zero = v >> 16, inside = all - zero;
-      const uint32_t jl = (ex & 0xFFFFu) + zero;  // splits <= L
-      const uint32_t woff = ex >> 16;
-      uint32_t w;
-      if (inside < 2u) {
-        w = fast_direct(jl, inside == 1u ? srt[jl + 1] - lo_key : S);
-      } else if (woff + (1u << f) <= uint32_t(l2cap)) {
-        w = kFastMulti | (uint32_t(f) << 24) | uint32_t(G + woff);
-        list[woff >> 2] = make_uint2(lo_key >> s, jl | (woff << 16));
-        atomicMax(&sh.used, int(woff + (1u << f)));
-      } else {
-        w = kFastMulti | kFastSearch | jl | ((inside < 0x7FFFu ? inside : 0x7FFFu) << 15);
-      }
-      cw[r * kWave] = w;
-      run += wave_lane(inc, kWave - 1);
+    // level-2 words in use: one atomic per wave (those of the cells that fit
+    // are a prefix in cell order)
+    if (ballot(used_hi != 0u) != 0ull) {
+      used_hi = wave_inclusive_max(used_hi);
+      if (lane == kWave - 1) atomicMax(&sh.used, int(used_hi));
     }
   }
   __syncthreads();
 
-  // 6. level-2 words of the crowded cells: per-sub-cell counts of the splits
-  //    strictly inside the cell (all | at the sub-cell's first key << 16),
-  //    scattered by the splits, then one wave scan per crowded cell
+  // 6. level-2 words of the crowded cells.  Per-sub-cell counts of the splits
+  //    strictly inside the cell (all | at the sub-cell's first key << 16) are
+  //    scattered by the splits; then one flat scan over the level-2 words in
+  //    use, four per thread (one list slot): the exclusive sum P of the counts
+  //    and, by a running maximum of the slots that start a cell, each word's
+  //    cell.  A sub-cell's jL is its cell's jL + P - P(cell's first word) + the
+  //    splits at the sub-cell's first key.
   {
-    const int used = sh.used, R = (used + 3) / 4;  // list slots in use (some empty)
-    for (int i = tid; i < used; i += nt) cells[G + i] = 0u;
-    __syncthreads();
+    const int used = sh.used, R = used / 4;
+    uint32_t* l2 = cells + G;
+    uint4* l24 = reinterpret_cast<uint4*>(l2);
     for (int64_t t = 1 + tid; t <= n - 2; t += nt) {
       const uint32_t key = srt[t];
       if (key >= (1u << 24)) continue;
       const uint32_t w = cells[key >> s];
       const uint32_t off = key & (S - 1u);
-      if ((w & (kFastMulti | kFastSearch)) != kFastMulti || off == 0u) continue;
-      const int ss = s - int((w >> 24) & 0xFu);
-      atomicAdd(&cells[(w & 0xFFFFFu) + (off >> ss)],
+      if ((w & (kCellMulti | kCellSearch)) != kCellMulti || off == 0u) continue;
+      const int ss = s - int((w >> 14) & 0xFu);
+      atomicAdd(&l2[(w >> 18) + (off >> ss)],
                 1u + ((off & ((1u << ss) - 1u)) == 0u ? 0x10000u : 0u));
     }
     __syncthreads();
-    const int wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6, lane = lane_id();
-    for (int e = wave; e < R; e += nwaves) {
-      const uint2 ent = list[e];
-      if (ent.x == 0xFFFFFFFFu) continue;  // (wave-uniform)
-      const uint32_t c = ent.x, wo = ent.y >> 16, jl = ent.y & 0xFFFFu;
-      const int f = int((cells[c] >> 24) & 0xFu);
-      const int ss = s - f;
-      const uint32_t S2 = 1u << ss;
-      const int nsub = 1 << f, per = (nsub + kWave - 1) / kWave;
-      const int q0 = lane * per, q1 = q0 + per < nsub ? q0 + per : nsub;
-      uint32_t local = 0u;
-      for (int q = q0; q < q1; ++q) local += cells[G + wo + q] & 0xFFFFu;
-      uint32_t run = jl + wave_inclusive_scan(local) - local;
-      for (int q = q0; q < q1; ++q) {
-        const uint32_t v = cells[G + wo + q];
-        const uint32_t all = v & 0xFFFFu, zero = v >> 16, in2 = all - zero;
-        const uint32_t jl2 = run + zero;
-        const uint32_t klo = (c << s) + (uint32_t(q) << ss);
-        cells[G + wo + q] = in2 <= 1u
-                                ? fast_direct(jl2, in2 == 1u ? srt[jl2 + 1] - klo : S2)
-                                : kFastMulti | kFastSearch | jl2 | (in2 << 15);
-        run += all;
+    uint32_t carry_sum = 0u, carry_max = 0u;
+    for (int b0 = 0; b0 < R; b0 += nt) {
+      const int slot = b0 + tid;
+      const bool valid = slot < R;
+      const uint4 q4 = valid ? l24[slot] : make_uint4(0u, 0u, 0u, 0u);
+      const uint32_t v[4] = {q4.x, q4.y, q4.z, q4.w};
+      const uint2 ent = valid ? list[slot] : make_uint2(kNoCell, 0u);
+      const uint32_t start = ent.x != kNoCell ? uint32_t(slot + 1) : 0u;
+      const uint32_t local = (v[0] & 0xFFFFu) + (v[1] & 0xFFFFu) + (v[2] & 0xFFFFu) + (v[3] & 0xFFFFu);
+      const uint32_t isum = wave_inclusive_scan(local);
+      const uint32_t imax = wave_inclusive_max(start);
+      if (lane == kWave - 1) {
+        sh.wtot[wave] = isum;
+        sh.wtot[32 + wave] = imax;
       }
+      __syncthreads();
+      uint32_t before = carry_sum, owner1 = carry_max > imax ? carry_max : imax;
+      uint32_t tot_sum = 0u, tot_max = 0u;
+      for (int w2 = 0; w2 < nwaves; ++w2) {
+        const uint32_t ts = sh.wtot[w2], tm = sh.wtot[32 + w2];
+        if (w2 < wave) {
+          before += ts;
+          owner1 = tm > owner1 ? tm : owner1;
+        }
+        tot_sum += ts;
+        tot_max = tm > tot_max ? tm : tot_max;
+      }
+      const uint32_t p0 = before + isum - local;  // P of the slot's first word
+      if (valid && start != 0u) list[slot].y = ent.y - p0;  // jL - P(first word)
+      __syncthreads();
+      if (valid) {
+        const uint2 oe = list[int(owner1) - 1];  // {f, jL - P(first word)} of the word's cell
+        const int ss = s - int(oe.x);
+        uint32_t p = p0, wv[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const uint32_t all = v[q] & 0xFFFFu, zero = v[q] >> 16, inside = all - zero;
+          const uint32_t jl = oe.y + p + zero;
+          const uint32_t nxt = srt[jl + 1];
+          wv[q] = inside >= 2u ? kCellMulti | kCellSearch | jl | (inside << 14)
+                               : cell_direct(jl, inside, nxt, ss);
+          p += all;
+        }
+        l24[slot] = make_uint4(wv[0], wv[1], wv[2], wv[3]);
+      }
+      carry_sum += tot_sum;
+      carry_max = tot_max > carry_max ? tot_max : carry_max;
+      if (b0 + nt < R) __syncthreads();  // sh.wtot is reused
     }
   }
   if (tid == 0) {
@@ -990,6 +1066,75 @@ __device__ __forceinline__ int next_segment(int* counter) {
 }
 
 // ---------------------------------------------------------------------------
+// Finalize, by one work-group: chunk_row = exclusive scan of chunk_kept (four
+// chunks per thread per tile); kept per instance; total; dense flag (every
+// chunk kept all of its slots).
+__device__ void finalize_chunks(const BatchedParams& P) {
+  __shared__ int64_t scratch[16];
+  __shared__ int sh_sparse;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  if (tid == 0) sh_sparse = 0;
+  const int64_t nch = P.n_chunks;
+  int64_t carry = 0;
+  bool sparse = false;
+  for (int64_t base = 0; base < nch; base += 4 * int64_t(nt)) {
+    const int64_t c0 = base + 4 * int64_t(tid);
+    int64_t v[4], local = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int64_t ci = c0 + q;
+      v[q] = ci < nch ? P.chunk_kept[ci] : 0;
+      if (ci < nch) {
+        const ChunkDesc ch = P.chunks[ci];
+        sparse = sparse || v[q] != ch.slot_hi - ch.slot_lo;
+      }
+      local += v[q];
+    }
+    int64_t tot;
+    int64_t run = carry + block_exclusive_scan(local, scratch, &tot);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (c0 + q < nch) P.chunk_row[c0 + q] = run;
+      run += v[q];
+    }
+    carry += tot;
+  }
+  if (sparse) atomicOr(&sh_sparse, 1);
+  __syncthreads();  // chunk_row written above is visible to the whole work-group
+  for (int64_t i = tid; i < P.n_instances; i += nt) {
+    const InstanceDesc in = P.inst[i];
+    const int64_t c1 = int64_t(in.first_chunk) + in.n_chunks;
+    P.kept[i] = (c1 < nch ? P.chunk_row[c1] : carry) - P.chunk_row[in.first_chunk];
+  }
+  if (tid == 0) {
+    P.total[0] = carry;
+    P.flags[0] = sh_sparse ? 0 : 1;  // dense
+  }
+}
+
+// Standalone finalize (a plan without chunks).
+__global__ void __launch_bounds__(1024) finalize_kernel(BatchedParams P) { finalize_chunks(P); }
+
+// End of a sampling work-group: the last of the n_chunks groups to finish
+// (counted in flags[1], which it resets) finalizes.
+__device__ void finalize_if_last(const BatchedParams& P) {
+  __shared__ int sh_last;
+  __syncthreads();  // this group's chunk_kept and seg_row are written
+  if (threadIdx.x == 0) {
+    __threadfence();
+    uint32_t* counter = reinterpret_cast<uint32_t*>(P.flags + 1);
+    const uint32_t old = atomicAdd(counter, 1u);
+    sh_last = old == uint32_t(P.n_chunks - 1);
+    if (sh_last) {
+      __threadfence();
+      *counter = 0u;
+    }
+  }
+  __syncthreads();
+  if (sh_last) finalize_chunks(P);
+}
+
+// ---------------------------------------------------------------------------
 // Sampling: one work-group per chunk; waves take segments from a counter.  While every slot
 // of a segment so far was kept, rows are positional and no keep mask is
 // written; the first iteration with a dropped slot is recorded, and masks are
@@ -1001,6 +1146,7 @@ __device__ __forceinline__ int next_segment(int* counter) {
 // (find_bins, slot_values); returns the keep bits.
 template <typename Real, bool WINDOW>
 struct GuideLookup {
+  static constexpr int kBinBias = 0;  // returns j
   const uint32_t* guide;
   const uint32_t* srt;
   const typename CoefOf<Real>::type* coef;
@@ -1018,44 +1164,57 @@ struct GuideLookup {
   }
 };
 
-// Bins of a trip's keys from the cell tables of fast_tables.hpp: one cell word
-// per key; the level-2 step or bisection only at key positions where some lane
-// of the wave needs it (each position's branch is wave-uniform).
+// j + 1 of a trip's keys from the cell tables of fast_tables.hpp: one cell
+// word per key; the level-2 step or bisection only at key positions where some
+// lane of the wave needs it (each position's branch is wave-uniform).
 template <int NK>
 __device__ __forceinline__ void cell_bins(const uint32_t (&k)[NK], const uint32_t* cells,
-                                          const uint32_t* srt, int s, int (&j)[NK]) {
-  const uint32_t mask = (1u << s) - 1u;
-  uint32_t w[NK];
+                                          const uint32_t* l2, const uint32_t* srt, int s,
+                                          int (&jj)[NK]) {
+  // the ones below the key's offset, held in a vector register so that X is a
+  // single shift-or (one scalar operand per vector instruction)
+  uint32_t low = (1u << (32 - s)) - 1u;
+  asm("v_mov_b32 %0, %1" : "=v"(low) : "s"(low));
+  uint32_t w[NK], x[NK], t[NK];
 #pragma unroll
   for (int e = 0; e < NK; ++e) w[e] = cells[k[e] >> s];
 #pragma unroll
-  for (int e = 0; e < NK; ++e) j[e] = fast_direct_bin(w[e], k[e] & mask);
+  for (int e = 0; e < NK; ++e) {
+    x[e] = (k[e] << (32 - s)) | low;
+    t[e] = w[e] & (kCellMulti | kCellField);
+  }
 #pragma unroll
-  for (int e = 0; e < NK; ++e)
-    if (ballot((w[e] & kFastMulti) != 0u) != 0ull)
-      if ((w[e] & kFastMulti) != 0u) j[e] = fast_multi_bin(w[e], k[e], s, cells, srt);
+  for (int e = 0; e < NK; ++e) {
+    int r = int(t[e]) + (x[e] >= w[e] ? 1 : 0);
+    const bool m = t[e] > kCellField;  // indirect or search word
+    if (ballot(m) != 0ull)
+      if (m) r = cell_multi_jj(w[e], x[e], k[e], l2, srt);
+    jj[e] = r;
+  }
 }
 
 // The same over the cell tables of fast_tables.hpp (float32): one cell word per
 // key, the level-2 step only at key positions where some lane needs it, then
-// one float4 {M, A, T} read per key.
+// one float4 {M, A, T} read per key.  Returns j + 1 (kBinBias).
 template <bool WINDOW>
 struct CellLookup {
+  static constexpr int kBinBias = 1;
   const uint32_t* cells;
+  const uint32_t* l2;
   const uint32_t* srt;
-  const float4* bins;
-  const double* cdf;  // global
-  const float* x;     // global
+  const float4* bins1;  // bins - 1
+  const double* cdf;    // global
+  const float* x;       // global
   int s;
   bool safe;
 
   template <int NK>
   __device__ __forceinline__ int operator()(const uint32_t (&k)[NK], int (&j)[NK], float (&v)[NK],
                                             const BatchedParams& P) const {
-    cell_bins<NK>(k, cells, srt, s, j);
+    cell_bins<NK>(k, cells, l2, srt, s, j);
     float4 b[NK];
 #pragma unroll
-    for (int e = 0; e < NK; ++e) b[e] = bins[j[e]];
+    for (int e = 0; e < NK; ++e) b[e] = bins1[j[e]];
     if (!WINDOW && safe) {  // every value of this curve is kept (kSafe)
 #pragma unroll
       for (int e = 0; e < NK; ++e)
@@ -1086,7 +1245,7 @@ struct CellLookup {
 #pragma unroll 1
       for (int e = 0; e < NK; ++e) {
         if ((unsafe >> e) & 1) {
-          const double v64 = interp_reference(cdf, x, j[e], k[e]);
+          const double v64 = interp_reference(cdf, x, j[e] - 1, k[e]);
           v[e] = static_cast<float>(v64);
           keep |= keep_value<WINDOW>(v64, P) ? (1 << e) : 0;
         }
@@ -1100,6 +1259,7 @@ template <typename Real, bool WINDOW, typename Lookup>
 __device__ __forceinline__ void sample_segment(const BatchedParams& P, int c, const ChunkDesc& ch,
                                                const InstanceDesc& in, int w, const Lookup& lookup,
                                                int64_t* sh_kept) {
+  constexpr int kBias = Lookup::kBinBias;  // the lookup returns j + kBias
   const int nw = P.segs, lane = lane_id();  // segment w of nw in this chunk
   const Segment sg = segment_of(in.counter_base, ch.slot_lo, ch.slot_hi, w, nw);
   const int64_t seg = int64_t(c) * nw + w;
@@ -1154,13 +1314,15 @@ __device__ __forceinline__ void sample_segment(const BatchedParams& P, int c, co
         reinterpret_cast<double2*>(out + row)[0] = make_double2(v[0], v[1]);
         reinterpret_cast<double2*>(out + row)[1] = make_double2(v[2], v[3]);
       }
-      if (bins_out) *reinterpret_cast<int4*>(bins_out + row) = make_int4(j[0], j[1], j[2], j[3]);
+      if (bins_out)
+        *reinterpret_cast<int4*>(bins_out + row) =
+            make_int4(j[0] - kBias, j[1] - kBias, j[2] - kBias, j[3] - kBias);
     } else {
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         if ((keepbits >> e) & 1) {
           out[row] = v[e];
-          if (bins_out) bins_out[row] = j[e];
+          if (bins_out) bins_out[row] = j[e] - kBias;
           ++row;
         }
       }
@@ -1170,9 +1332,12 @@ __device__ __forceinline__ void sample_segment(const BatchedParams& P, int c, co
 
   // A curve whose every value is kept (kSafe, no window): slot 4 i + e of the
   // segment is row 4 i + e - off0, so rows need no keep bits, ballots or masks;
-  // whole blocks go out as one 16-byte store when the rows are aligned.
+  // whole blocks go out as one 16-byte store when the rows are aligned, and
+  // trips of whole blocks only (every trip but the segment's last) skip the
+  // per-lane bounds.
   if (!WINDOW && lookup.safe) {
     const bool vec = off0 == 0u && (base_row & 3) == 0;
+    const uint32_t nfull = vec ? span / 4 : 0u;  // blocks whose four rows all exist
     for (uint32_t i0 = 0; i0 < nblk; i0 += 64 * kTrip) {
       uint32_t k[4 * kTrip];
 #pragma unroll
@@ -1187,6 +1352,24 @@ __device__ __forceinline__ void sample_segment(const BatchedParams& P, int c, co
       int j[4 * kTrip];
       Real v[4 * kTrip];
       lookup(k, j, v, P);
+      if (i0 + 64 * kTrip <= nfull) {  // wave-uniform
+#pragma unroll
+        for (int h = 0; h < kTrip; ++h) {
+          const uint32_t i = i0 + 64 * h + uint32_t(lane);
+          if constexpr (sizeof(Real) == 4) {
+            *reinterpret_cast<float4*>(out + 4 * i) =
+                make_float4(v[4 * h], v[4 * h + 1], v[4 * h + 2], v[4 * h + 3]);
+          } else {
+            reinterpret_cast<double2*>(out + 4 * i)[0] = make_double2(v[4 * h], v[4 * h + 1]);
+            reinterpret_cast<double2*>(out + 4 * i)[1] = make_double2(v[4 * h + 2], v[4 * h + 3]);
+          }
+          if (bins_out)
+            *reinterpret_cast<int4*>(bins_out + 4 * i) =
+                make_int4(j[4 * h] - kBias, j[4 * h + 1] - kBias, j[4 * h + 2] - kBias,
+                          j[4 * h + 3] - kBias);
+        }
+        continue;
+      }
 #pragma unroll
       for (int h = 0; h < kTrip; ++h) {
         const uint32_t i = i0 + 64 * h + uint32_t(lane);
@@ -1200,14 +1383,15 @@ __device__ __forceinline__ void sample_segment(const BatchedParams& P, int c, co
           }
           if (bins_out)
             *reinterpret_cast<int4*>(bins_out + 4 * i) =
-                make_int4(j[4 * h], j[4 * h + 1], j[4 * h + 2], j[4 * h + 3]);
+                make_int4(j[4 * h] - kBias, j[4 * h + 1] - kBias, j[4 * h + 2] - kBias,
+                          j[4 * h + 3] - kBias);
         } else {
 #pragma unroll
           for (int e = 0; e < 4; ++e) {
             const uint32_t r = 4 * i + e;
             if (i < nblk && r >= off0 && r < span) {
               out[r - off0] = v[4 * h + e];
-              if (bins_out) bins_out[r - off0] = j[4 * h + e];
+              if (bins_out) bins_out[r - off0] = j[4 * h + e] - kBias;
             }
           }
         }
@@ -1309,6 +1493,7 @@ __global__ void __launch_bounds__(1024) sample_kernel(BatchedParams P) {
       sample_segment<Real, WINDOW>(P, c, ch, in, sgi, lk, sh_kept);
   }
   chunk_rows(P, c, sh_kept);
+  finalize_if_last(P);
 }
 
 // Sampling of the float32 cell-table instances (fast_tables.hpp): one
@@ -1329,61 +1514,16 @@ __global__ void __launch_bounds__(1024) sample_cell_kernel(BatchedParams P) {
   double* cdf_copy = P.partials + ch.partial_off;
   build_fast<SEQUENTIAL>(P, ch.instance, in, lds, ch.index == 0, sh, cdf_copy);
   const FastLdsForward L = fast_lds_forward(in.n_nodes, in.fast_log2);
-  const CellLookup<WINDOW> lk{reinterpret_cast<const uint32_t*>(lds + L.cells),
+  const uint32_t* cells = reinterpret_cast<const uint32_t*>(lds + L.cells);
+  const CellLookup<WINDOW> lk{cells, cells + (int64_t(1) << in.fast_log2),
                               reinterpret_cast<const uint32_t*>(lds + L.srt),
-                              reinterpret_cast<const float4*>(lds + L.bins), cdf_copy,
+                              reinterpret_cast<const float4*>(lds + L.bins) - 1, cdf_copy,
                               static_cast<const float*>(P.x) + in.node_off, 24 - in.fast_log2,
                               (sh.flags & kSafe) != 0};
   for (int sgi = next_segment(&sh_next); sgi < P.segs; sgi = next_segment(&sh_next))
     sample_segment<float, WINDOW>(P, c, ch, in, sgi, lk, sh_kept);
   chunk_rows(P, c, sh_kept);
-}
-
-// ---------------------------------------------------------------------------
-// Finalize: one work-group.  chunk_row = exclusive scan of chunk_kept (four
-// chunks per thread per tile); kept per instance; total; dense flag (every
-// chunk kept all of its slots).
-__global__ void __launch_bounds__(1024) finalize_kernel(BatchedParams P) {
-  __shared__ int64_t scratch[16];
-  __shared__ int sh_sparse;
-  const int tid = threadIdx.x, nt = blockDim.x;
-  if (tid == 0) sh_sparse = 0;
-  const int64_t nch = P.n_chunks;
-  int64_t carry = 0;
-  bool sparse = false;
-  for (int64_t base = 0; base < nch; base += 4 * int64_t(nt)) {
-    const int64_t c0 = base + 4 * int64_t(tid);
-    int64_t v[4], local = 0;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int64_t ci = c0 + q;
-      v[q] = ci < nch ? P.chunk_kept[ci] : 0;
-      if (ci < nch) {
-        const ChunkDesc ch = P.chunks[ci];
-        sparse = sparse || v[q] != ch.slot_hi - ch.slot_lo;
-      }
-      local += v[q];
-    }
-    int64_t tot;
-    int64_t run = carry + block_exclusive_scan(local, scratch, &tot);
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      if (c0 + q < nch) P.chunk_row[c0 + q] = run;
-      run += v[q];
-    }
-    carry += tot;
-  }
-  if (sparse) atomicOr(&sh_sparse, 1);
-  __syncthreads();  // chunk_row written above is visible to the whole work-group
-  for (int64_t i = tid; i < P.n_instances; i += nt) {
-    const InstanceDesc in = P.inst[i];
-    const int64_t c1 = int64_t(in.first_chunk) + in.n_chunks;
-    P.kept[i] = (c1 < nch ? P.chunk_row[c1] : carry) - P.chunk_row[in.first_chunk];
-  }
-  if (tid == 0) {
-    P.total[0] = carry;
-    P.flags[0] = sh_sparse ? 0 : 1;  // dense
-  }
+  finalize_if_last(P);
 }
 
 // Pack: one work-group per chunk, segments round-robin over its waves; a no-op when dense.
@@ -1508,9 +1648,9 @@ struct ScaleProbe {
   int64_t kept[4], srow0[4];
   uint32_t bits[4];
 
-  template <typename Real>
-  __device__ __forceinline__ void load(const BatchedParams& P, const Real* grad, int c, int s0,
-                                       int nwv, int lane) {
+  // Rows of four segments, then (load_bits) the first cotangent of each at this
+  // lane; split so that other loads can be issued in between.
+  __device__ __forceinline__ void load_rows(const BatchedParams& P, int c, int s0, int nwv) {
     const int nw = P.segs;
     const int64_t crow = P.chunk_row[c];
 #pragma unroll
@@ -1520,8 +1660,19 @@ struct ScaleProbe {
       kept[q] = sw < nw ? P.seg_kept[sseg] : 0;
       srow0[q] = sw < nw ? crow + P.seg_row[sseg] : 0;
     }
+  }
+
+  template <typename Real>
+  __device__ __forceinline__ void load_bits(const Real* grad, int lane) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) bits[q] = grad_bits(grad, kept[q], srow0[q], lane);
+  }
+
+  template <typename Real>
+  __device__ __forceinline__ void load(const BatchedParams& P, const Real* grad, int c, int s0,
+                                       int nwv, int lane) {
+    load_rows(P, c, s0, nwv);
+    load_bits(grad, lane);
   }
 
   // Largest |g| bits over the four segments, each from its first 64-row block
@@ -1932,16 +2083,21 @@ __device__ void fast_vjp_body(const BatchedParams& P, const float* grad, float* 
   const int tid = threadIdx.x, nt = blockDim.x;
   const int probe_wave = __builtin_amdgcn_readfirstlane(tid >> 6), probe_nwv = nt >> 6;
   ScaleProbe probe;
-  probe.load(P, grad, c, probe_wave, probe_nwv, lane_id());
+  probe.load_rows(P, c, probe_wave, probe_nwv);
 
   uint32_t* cells = reinterpret_cast<uint32_t*>(lds + L.cells);
   uint32_t* srt = reinterpret_cast<uint32_t*>(lds + L.srt);
   uint64_t* a0 = reinterpret_cast<uint64_t*>(lds + L.a0);
   uint64_t* a1 = a0 + kFastMaxNodes;  // one address, an immediate offset for the second add
+  uint64_t* a0m = a0 - 1;
+  uint64_t* a1m = a1 - 1;
+  const uint32_t* l2 = cells + G;
   const int64_t nb = pad4(n);
   double* outl = P.partials + ch.partial_off;  // [2 nb]: float64 sums, then partials
-  lds_copy(cells, blob + B.cells, pad4(G + int64_t(hdr->level2_words)) * 4);
-  lds_copy(srt, blob + B.srt, nb * 4);
+  // the sorted splits and every cell word (level 2 at its capacity, so that the
+  // copy does not wait for the header) in one contiguous copy
+  lds_copy(srt, blob + B.srt, B.own - B.srt);
+  probe.load_bits(grad, lane_id());
   for (int64_t i = tid; i < nb; i += nt) {
     a0[i] = 0ull;
     a1[i] = 0ull;
@@ -2020,19 +2176,20 @@ __device__ void fast_vjp_body(const BatchedParams& P, const float* grad, float* 
       }
     };
 
-    auto add = [&](int j, uint32_t k, float gv) {
+    // jj = j + 1 (cell_bins): the arrays are addressed from one entry below
+    auto add = [&](int jj, uint32_t k, float gv) {
       const float gs = gv * scale;
       if (__builtin_expect(__builtin_fabsf(gs) < kQLimit, 1)) {
         const int q = static_cast<int>(__builtin_rintf(gs));
-        atomicAdd(reinterpret_cast<unsigned long long*>(a0 + j),
+        atomicAdd(reinterpret_cast<unsigned long long*>(a0m + jj),
                   static_cast<unsigned long long>(static_cast<int64_t>(q)));
-        atomicAdd(reinterpret_cast<unsigned long long*>(a1 + j),
+        atomicAdd(reinterpret_cast<unsigned long long*>(a1m + jj),
                   static_cast<unsigned long long>(static_cast<int64_t>(q) * int64_t(k)));
       } else {
         sh_outl = 1;
-        unsafeAtomicAdd(outl + j, double(gv));
-        unsafeAtomicAdd(outl + nb + j,
-                        double(gv) * double(static_cast<int32_t>(k - own[j])));
+        unsafeAtomicAdd(outl + (jj - 1), double(gv));
+        unsafeAtomicAdd(outl + nb + (jj - 1),
+                        double(gv) * double(static_cast<int32_t>(k - own[jj - 1])));
       }
     };
 
@@ -2041,6 +2198,7 @@ __device__ void fast_vjp_body(const BatchedParams& P, const float* grad, float* 
       // base + 4 i + e; whole aligned blocks load as one 16-byte read
       const int64_t base = row0 - int64_t(off0);
       const bool vec = off0 == 0u && (base & 3) == 0;
+      const uint32_t nfull = vec ? span / 4 : 0u;  // blocks whose four rows all exist
       for (uint32_t i0 = 0; i0 < nblk; i0 += 64 * kTrip) {
         uint32_t k[4 * kTrip];
 #pragma unroll
@@ -2054,10 +2212,11 @@ __device__ void fast_vjp_body(const BatchedParams& P, const float* grad, float* 
         }
         float gv[4 * kTrip];
         int live[kTrip];
+        const bool whole = i0 + 64 * kTrip <= nfull;  // wave-uniform: no per-lane bounds
 #pragma unroll
         for (int h = 0; h < kTrip; ++h) {
           const uint32_t i = i0 + 64 * h + uint32_t(lane);
-          if (vec && 4 * i + 4 <= span) {
+          if (whole || (vec && 4 * i + 4 <= span)) {
             const float4 g4 = *reinterpret_cast<const float4*>(grad + base + 4 * int64_t(i));
             gv[4 * h] = g4.x;
             gv[4 * h + 1] = g4.y;
@@ -2076,7 +2235,12 @@ __device__ void fast_vjp_body(const BatchedParams& P, const float* grad, float* 
           }
         }
         int j[4 * kTrip];
-        cell_bins<4 * kTrip>(k, cells, srt, s, j);
+        cell_bins<4 * kTrip>(k, cells, l2, srt, s, j);
+        if (whole) {
+#pragma unroll
+          for (int e = 0; e < 4 * kTrip; ++e) add(j[e], k[e], gv[e]);
+          continue;
+        }
 #pragma unroll
         for (int h = 0; h < kTrip; ++h) {
           if (live[h] == 0xF) {
@@ -2113,7 +2277,7 @@ __device__ void fast_vjp_body(const BatchedParams& P, const float* grad, float* 
         if (h == 0 || i0 + 64 * h < nblk) rows_of(it + h, i0 + 64 * h, kb[h], rw[h]);
       }
       int j[4 * kTrip];
-      cell_bins<4 * kTrip>(k, cells, srt, s, j);
+      cell_bins<4 * kTrip>(k, cells, l2, srt, s, j);
       float gv[4 * kTrip];
 #pragma unroll
       for (int h = 0; h < kTrip; ++h) {

@@ -462,7 +462,8 @@ int loits_hip_batched_forward(const loits_hip_batched_args* args, void* stream) 
                            : launch_sample<double>(P, L.threads, seq, L.lds_forward, s),
                        "batched sample launch");
     if (rc != LOITS_HIP_OK) return rc;
-    return check_hip(launch_finalize(P, s), "batched finalize launch");
+    // the last sampling work-group finalizes (row offsets, totals, dense flag)
+    return L.n_chunks > 0 ? LOITS_HIP_OK : check_hip(launch_finalize(P, s), "batched finalize launch");
   } catch (const std::exception& e) {
     return fail(LOITS_HIP_ERUNTIME, std::string("batched_forward: ") + e.what());
   }

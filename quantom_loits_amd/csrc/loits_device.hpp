@@ -168,6 +168,24 @@ __device__ __forceinline__ T wave_inclusive_scan(T v) {
   return v;
 }
 
+// Inclusive running maximum over the lanes of a wave (0 is the identity).
+__device__ __forceinline__ uint32_t wave_inclusive_max(uint32_t v) {
+  uint32_t t;
+  t = dpp_shift_u32<0x111, 0xF>(v);
+  v = t > v ? t : v;
+  t = dpp_shift_u32<0x112, 0xF>(v);
+  v = t > v ? t : v;
+  t = dpp_shift_u32<0x114, 0xF>(v);
+  v = t > v ? t : v;
+  t = dpp_shift_u32<0x118, 0xF>(v);
+  v = t > v ? t : v;
+  t = dpp_shift_u32<0x142, 0xA>(v);
+  v = t > v ? t : v;
+  t = dpp_shift_u32<0x143, 0xC>(v);
+  v = t > v ? t : v;
+  return v;
+}
+
 template <typename T>
 __device__ __forceinline__ T wave_lane(T v, int lane) {
   if constexpr (sizeof(T) == 4) {
