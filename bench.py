@@ -65,6 +65,8 @@ def parse():
     p.add_argument("--dropin-only", action="store_true",
                    help="only time the reference-API (two-axis) path at the GAN shapes and "
                         "print those timings")
+    p.add_argument("--dropin-shape", default=None, metavar="GRID,EVENTS",
+                   help="with --dropin-only: time only this grid and event count")
     p.add_argument("--ragged-only", action="store_true",
                    help="only time the 4096-curve ragged batch (fused graph) and print it")
     p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -294,7 +296,7 @@ def gan_workload(dev, warmup=2, iterations=5):
     return out
 
 
-def dropin_workloads(dev, steps):
+def dropin_workloads(dev, steps, only=None):
     """The reference-API path (LOITS(backend="hip") bindings, both axes) at the
     reference's own GAN shapes: grid g x g cells of K = 10 nodes per axis, one
     theory instance; forward + backward, the forward's two host reads included.
@@ -308,6 +310,8 @@ def dropin_workloads(dev, steps):
     out = {}
     rng = np.random.default_rng(3)
     for grid, n_events in ((10, 10000), (99, 10000), (99, 1000000), (99, 10000000)):
+        if only is not None and (grid, n_events) != only:
+            continue
         k = 10
         xb = np.sort(rng.uniform(0.001, 0.999, (1, grid, k)), axis=2)
         qb = np.sort(rng.uniform(1.0, 100.0, (1, grid, k)), axis=2)
@@ -448,7 +452,8 @@ def main():
 
     dev = torch.device("cuda", 0)
     if args.dropin_only:
-        print(json.dumps({"dropin_two_axis": dropin_workloads(dev, args.steps)}))
+        only = tuple(int(v) for v in args.dropin_shape.split(",")) if args.dropin_shape else None
+        print(json.dumps({"dropin_two_axis": dropin_workloads(dev, args.steps, only)}))
         return
     if args.ragged_only:
         print(json.dumps({"ragged_4096_window": ragged_workload(dev, args.steps, loop=False)}))

@@ -57,6 +57,12 @@ const char* loits_hip_version(void);
  * reference registry (loits.py:13-46). */
 int loits_hip_device_info(int device, char* arch, int arch_len, int* cu_count);
 
+/* Blocks until the work queued on `stream` (NULL: the default stream) is done.
+ * The profiling regions of the bindings drain the stream at both ends with it,
+ * as the reference's GPU binding waits on its queue when a region opens
+ * (sycl/bindings.cpp:13-27). */
+int loits_hip_synchronize(void* stream);
+
 /* ---------------------------------------------------------------------- */
 /* RNG (rng/philox.hpp:17-61)                                              */
 

@@ -20,16 +20,29 @@ namespace py = pybind11;
 
 namespace {
 
-class RegionGuard {
+// A profiler range (at::RecordFunction) over one stage of a binding.  With
+// profile_regions set, the caller's stream is drained when the range opens and
+// again before it closes, so the range covers the stage's GPU work and not
+// only its launches; otherwise nothing is recorded and nothing waits.
+class StageRange {
  public:
-  RegionGuard(bool enabled, const char* name) {
-    if (!enabled) return;
-    guard_ = std::make_unique<at::RecordFunction>(at::RecordScope::USER_SCOPE);
-    if (guard_->isActive()) guard_->before(name);
+  StageRange(bool on, const char* name, void* stream) {
+    if (!on) return;
+    stream_ = stream;
+    drain();
+    range_ = std::make_unique<at::RecordFunction>(at::RecordScope::USER_SCOPE);
+    if (range_->isActive()) range_->before(name);
   }
+  ~StageRange() {
+    if (range_) drain();  // the range itself ends when range_ is destroyed
+  }
+  StageRange(const StageRange&) = delete;
+  StageRange& operator=(const StageRange&) = delete;
 
  private:
-  std::unique_ptr<at::RecordFunction> guard_;
+  void drain() { (void)loits_hip_synchronize(stream_); }
+  void* stream_ = nullptr;
+  std::unique_ptr<at::RecordFunction> range_;
 };
 
 void check(int status, const char* what) {
@@ -111,7 +124,6 @@ std::vector<at::Tensor> forward(at::Tensor x_bins, at::Tensor xsec_x, at::Tensor
                                 at::Tensor xsec_q, at::Tensor weights, at::Tensor acceptance,
                                 int64_t n_events, uint64_t seed, uint64_t sequence,
                                 bool profile_regions, py::object stream_obj) {
-  RegionGuard total(profile_regions, "loits::forward");
   for (auto* p : {&x_bins, &xsec_x, &q_bins, &xsec_q, &weights, &acceptance})
     require_device(*p, "theory output");
   const int dt = dtype_code(xsec_x, "xsec_x");
@@ -129,13 +141,14 @@ std::vector<at::Tensor> forward(at::Tensor x_bins, at::Tensor xsec_x, at::Tensor
       throw std::invalid_argument("all theory outputs must be on one GPU");
   StreamScope scope(stream_obj, xsec_x);
   void* stream = stream_of(stream_obj, xsec_x);
+  StageRange total(profile_regions, "loits::forward", stream);
   auto i64 = xsec_x.options().dtype(at::kLong);
   auto f64 = xsec_x.options().dtype(at::kDouble);
 
   at::Tensor counts = at::empty({s.batch, s.cells}, i64);
   int64_t nmax = 0, active = 0;
   {
-    RegionGuard region(profile_regions, "loits::forward::allocation");
+    StageRange region(profile_regions, "loits::forward::allocation", stream);
     at::Tensor na = at::empty({2}, i64);
     check(loits_hip_allocate(weights.data_ptr(), dt, s.batch * s.cells, n_events,
                              counts.data_ptr<int64_t>(), na.data_ptr<int64_t>(), stream),
@@ -164,20 +177,21 @@ std::vector<at::Tensor> forward(at::Tensor x_bins, at::Tensor xsec_x, at::Tensor
   {
     // The HIP forward fuses the reference's stages into three kernels:
     // curves (rho + CDF, both axes), sampling (both Philox streams,
-    // interpolation, compaction) and row offsets.  The reference's stage
-    // names are kept; stages fused into one kernel are nested markers.
-    RegionGuard r1(profile_regions, "loits::forward::rho_x");
-    RegionGuard r2(profile_regions, "loits::forward::rho_q2");
-    RegionGuard r3(profile_regions, "loits::forward::cdf_x");
-    RegionGuard r4(profile_regions, "loits::forward::cdf_q2");
+    // interpolation, compaction) and row offsets, all launched by the one
+    // call below.  The reference's stage names are kept: the fused stages are
+    // empty ranges here and their GPU time falls in stream_compaction.
+    StageRange r1(profile_regions, "loits::forward::rho_x", stream);
+    StageRange r2(profile_regions, "loits::forward::rho_q2", stream);
+    StageRange r3(profile_regions, "loits::forward::cdf_x", stream);
+    StageRange r4(profile_regions, "loits::forward::cdf_q2", stream);
   }
   {
-    RegionGuard region(profile_regions, "loits::forward::stream_compaction");
+    StageRange region(profile_regions, "loits::forward::stream_compaction", stream);
     {
-      RegionGuard a(profile_regions, "loits::forward::random_x");
-      RegionGuard b(profile_regions, "loits::forward::random_q2");
-      RegionGuard c(profile_regions, "loits::forward::interpolation_x");
-      RegionGuard d(profile_regions, "loits::forward::interpolation_q2");
+      StageRange a(profile_regions, "loits::forward::random_x", stream);
+      StageRange b(profile_regions, "loits::forward::random_q2", stream);
+      StageRange c(profile_regions, "loits::forward::interpolation_x", stream);
+      StageRange d(profile_regions, "loits::forward::interpolation_q2", stream);
     }
     loits_hip_forward_args a{};
     a.shape = s;
@@ -214,7 +228,7 @@ std::vector<at::Tensor> forward(at::Tensor x_bins, at::Tensor xsec_x, at::Tensor
   if (flags != 0)
     throw std::runtime_error("loits_hip_forward: compaction look-back did not complete");
   const int64_t nvalid = host_valid[0].item<int64_t>();
-  RegionGuard region(profile_regions, "loits::forward::state_pack");
+  StageRange region(profile_regions, "loits::forward::state_pack", stream);
   return {events_storage.narrow(0, 0, nvalid), norm_x, norm_q, cdf_x, cdf_q, u_x, u_q,
           interval_x, interval_q, packed_storage.narrow(0, 0, nvalid), row_offsets};
 }
@@ -236,7 +250,6 @@ std::vector<at::Tensor> backward(at::Tensor grad_events, at::Tensor x_bins, at::
   }
   const bool profile_regions = !tail.empty() && tail[0].cast<bool>();
   py::object stream_obj = tail.size() > 1 ? tail[1] : py::none();
-  RegionGuard total(profile_regions, "loits::backward");
   grad_events = grad_events.contiguous();
   require_device(grad_events, "grad_events");
   const int dt = dtype_code(xsec_x, "xsec_x");
@@ -270,6 +283,7 @@ std::vector<at::Tensor> backward(at::Tensor grad_events, at::Tensor x_bins, at::
       throw std::invalid_argument("backward: every tensor must be on xsec_x's GPU");
   StreamScope scope(stream_obj, xsec_x);
   void* stream = stream_of(stream_obj, xsec_x);
+  StageRange total(profile_regions, "loits::backward", stream);
   at::Tensor grad_x = at::empty(xsec_x.sizes(), xsec_x.options());
   at::Tensor grad_q = at::empty(xsec_q.sizes(), xsec_q.options());
   const int64_t ws_bytes = loits_hip_backward_workspace_size(&s, u_x.size(2));
@@ -277,14 +291,14 @@ std::vector<at::Tensor> backward(at::Tensor grad_events, at::Tensor x_bins, at::
   at::Tensor workspace = at::empty({ws_bytes}, xsec_x.options().dtype(at::kByte));
   at::Tensor acc_u8 = acceptance.contiguous().view(at::kByte);
   {
-    RegionGuard a(profile_regions, "loits::backward::interpolation_x");
-    RegionGuard b(profile_regions, "loits::backward::interpolation_q2");
+    StageRange a(profile_regions, "loits::backward::interpolation_x", stream);
+    StageRange b(profile_regions, "loits::backward::interpolation_q2", stream);
   }
   {
-    RegionGuard r1(profile_regions, "loits::backward::cdf_x");
-    RegionGuard r2(profile_regions, "loits::backward::cdf_q2");
-    RegionGuard r3(profile_regions, "loits::backward::rho_x");
-    RegionGuard r4(profile_regions, "loits::backward::rho_q2");
+    StageRange r1(profile_regions, "loits::backward::cdf_x", stream);
+    StageRange r2(profile_regions, "loits::backward::cdf_q2", stream);
+    StageRange r3(profile_regions, "loits::backward::rho_x", stream);
+    StageRange r4(profile_regions, "loits::backward::rho_q2", stream);
     loits_hip_backward_args a{};
     a.shape = s;
     a.dtype = dt;
@@ -315,7 +329,7 @@ std::vector<at::Tensor> backward(at::Tensor grad_events, at::Tensor x_bins, at::
       a.row_offsets = row_offsets.data_ptr<int64_t>();
     check(loits_hip_backward(&a, stream), "loits_hip_backward");
   }
-  RegionGuard region(profile_regions, "loits::backward::state_pack");
+  StageRange region(profile_regions, "loits::backward::state_pack", stream);
   return {grad_x, grad_q};
 }
 

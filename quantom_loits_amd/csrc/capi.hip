@@ -373,6 +373,10 @@ int loits_hip_device_info(int device, char* arch, int arch_len, int* cu_count) {
   return LOITS_HIP_OK;
 }
 
+int loits_hip_synchronize(void* stream) {
+  return check_hip(hipStreamSynchronize(static_cast<hipStream_t>(stream)), "hipStreamSynchronize");
+}
+
 int loits_hip_philox_fill(float* values, int64_t count, uint64_t seed, uint64_t stream_id,
                           void* stream) {
   if (count < 0 || (count > 0 && !values))

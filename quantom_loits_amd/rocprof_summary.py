@@ -46,11 +46,25 @@ def sources_sha256(root=ROOT):
 
 def short_name(kernel: str) -> str:
     """'void qloits::vjp_cell_kernel<2>(...)' -> 'vjp_cell_kernel'."""
-    k = kernel.strip()
+    k = kernel.strip().replace("(anonymous namespace)::", "")
     k = re.sub(r"^void\s+", "", k)
     k = k.split("(")[0]
     k = re.sub(r"<.*>$", "", k)
     return k.split("::")[-1]
+
+
+def display_name(kernel: str) -> str:
+    """'void qloits::sample_kernel<double, true>(...)' -> 'sample_kernel<double, true>'."""
+    k = re.sub(r"^void\s+", "", kernel.strip().replace("(anonymous namespace)::", ""))
+    depth, cut = 0, len(k)
+    for i, ch in enumerate(k):  # the parameter list opens at the first '(' outside <>
+        depth += ch == "<"
+        depth -= ch == ">"
+        if ch == "(" and depth == 0:
+            cut = i
+            break
+    k = k[:cut]
+    return k[k.rfind("::", 0, k.find("<") if "<" in k else len(k)) + 2:] if "::" in k else k
 
 
 def _csvs(d, pattern):
@@ -185,7 +199,7 @@ def cmd_sq(a):
     print("| " + " | ".join(head) + " |")
     print("| " + " | ".join(["---"] * len(head)) + " |")
     for k, v in sorted(keep.items()):
-        name = labels.get(short_name(k), short_name(k))
+        name = labels.get(short_name(k), display_name(k))
         row = [name] + [f"{v[c] / (a.events / 64):.2f}" if c in v else "" for c in cols] + \
               [f"{v[c] / CUS:.0f}" if c in v else "" for c in cols_cu]
         print("| " + " | ".join(row) + " |")
