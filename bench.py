@@ -24,7 +24,6 @@ from its sources into oracle/_ref, in a fresh pinned process).
 from __future__ import annotations
 
 import argparse
-import hashlib
 import json
 import os
 import statistics
@@ -85,20 +84,13 @@ def algorithmic_bytes(kept, nodes, s=4):
     return fwd, bwd
 
 
-def sources_sha256():
-    """Fingerprint of the kernel sources (the counters in profiles/ belong to one)."""
-    h = hashlib.sha256()
-    for p in sorted((ROOT / "quantom_loits_amd" / "csrc").glob("*")):
-        if p.suffix in (".hip", ".hpp", ".cpp", ".h"):
-            h.update(p.name.encode())
-            h.update(p.read_bytes())
-    return h.hexdigest()
-
-
 def profiled_traffic(kernel):
     """HBM bytes per launch of `kernel` from profiles/traffic.json (rocprofv3
     FETCH_SIZE and WRITE_SIZE runs scaled by a plain copy), or None when the
     kernel sources changed since those counters were taken."""
+    sys.path.insert(0, str(ROOT))
+    from quantom_loits_amd.rocprof_summary import sources_sha256
+
     tfile = ROOT / "profiles" / "traffic.json"
     try:
         t = json.loads(tfile.read_text())

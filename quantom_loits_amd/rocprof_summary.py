@@ -34,13 +34,20 @@ CUS = 256
 ROOT = Path(__file__).resolve().parent.parent
 
 
+# The sources behind the batched kernels' traffic (kernels, device helpers and
+# the planner that sizes their chunks and tables).
+BATCHED_SOURCES = ("batched.hip", "batched.hpp", "fast_tables.hpp", "loits_device.hpp",
+                   "capi.hip", "capi_internal.hpp")
+
+
 def sources_sha256(root=ROOT):
-    """SHA-256 over the kernel sources (same rule as bench.py)."""
+    """SHA-256 over the batched path's sources (bench.py compares it with the
+    one stored beside the counters in profiles/traffic.json)."""
     h = hashlib.sha256()
-    for p in sorted((Path(root) / "quantom_loits_amd" / "csrc").glob("*")):
-        if p.suffix in (".hip", ".hpp", ".cpp", ".h"):
-            h.update(p.name.encode())
-            h.update(p.read_bytes())
+    for name in BATCHED_SOURCES:
+        p = Path(root) / "quantom_loits_amd" / "csrc" / name
+        h.update(name.encode())
+        h.update(p.read_bytes())
     return h.hexdigest()
 
 
