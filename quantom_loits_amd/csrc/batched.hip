@@ -1551,70 +1551,82 @@ __global__ void __launch_bounds__(1024) pack_kernel(BatchedParams P, Real* packe
 // ---------------------------------------------------------------------------
 // Tail of the VJP for one instance, after its histogram (h0 = sum of
 // cotangents, h1 = sum of cotangent * (u - C_j), per bin j) is complete in
-// LDS.  gC (cpp/loits_core.cpp:294-301 summed per bin), the CDF adjoint
-// (:355-365) and the density adjoint (:416-431) with blocked parallel scans.
-template <typename Real, int MAXPER>
+// LDS or in the chunk's partial sums.  gC (cpp/loits_core.cpp:294-301 summed
+// per bin), the CDF adjoint (:355-365) and the density adjoint (:416-431).
+// Nodes go right to left in tiles of kTailPer * blockDim.x, kTailPer
+// consecutive nodes per thread, with the suffix sum carried from tile to tile;
+// each node's rho adjoint replaces its h0 (read before by the same thread
+// only), and a second pass writes the density gradient.  Registers do not grow
+// with the node count.
+constexpr int kTailPer = 4;
+
+template <typename Real>
 __device__ void vjp_tail(const BatchedParams& P, const InstanceDesc& in, const double* cdf,
-                         double inv_n, const double* h0, const double* h1, Real* grad_density,
+                         double inv_n, double* h0, const double* h1, Real* grad_density,
                          double* scratch) {
   const int64_t n = in.n_nodes;
   const Real* x = static_cast<const Real*>(P.x) + in.node_off;
   const Real* s = static_cast<const Real*>(P.density) + in.node_off;
   const double acc = in.accept ? 1.0 : 0.0;
-  const int nt = blockDim.x;
-  const int64_t per = (n + nt - 1) / nt;
-  const int64_t t0 = threadIdx.x * per;
-  double gc[MAXPER];
-#pragma unroll
-  for (int k = 0; k < MAXPER; ++k) {
-    const int64_t t = t0 + k;
-    double g = 0.0;
-    if (k < per && t < n) {
-      if (t + 1 < n) {
-        const double inv_d = 1.0 / (cdf[t + 1] - cdf[t] + kEpsilon);
-        const double db = double(x[t + 1]) - double(x[t]);
-        g += db * (inv_d * inv_d * h1[t] - inv_d * h0[t]);
-      }
-      if (t >= 1) {
-        const double inv_d = 1.0 / (cdf[t] - cdf[t - 1] + kEpsilon);
-        const double db = double(x[t]) - double(x[t - 1]);
-        g -= db * inv_d * inv_d * h1[t - 1];
-      }
-    }
-    gc[k] = g;
-  }
-  double seg = 0.0;
-#pragma unroll
-  for (int k = MAXPER - 1; k >= 0; --k) {
-    seg += gc[k];
-    gc[k] = seg;  // sum over k' >= k within the segment
-  }
-  const double after = block_exclusive_suffix(seg, scratch);
-  double grho[MAXPER];
+  const int64_t tile = int64_t(kTailPer) * blockDim.x;
+  const int64_t mine = int64_t(threadIdx.x) * kTailPer;
+  double carry = 0.0;  // sum of gC over the tiles to the right
   double local_norm = 0.0;
+  for (int64_t base = ((n - 1) / tile) * tile; base >= 0; base -= tile) {
+    const int64_t t0 = base + mine;
+    double gc[kTailPer];
 #pragma unroll
-  for (int k = 0; k < MAXPER; ++k) {
-    const int64_t t = t0 + k;
-    double g = 0.0;
-    if (k < per && t < n) {
+    for (int k = 0; k < kTailPer; ++k) {
+      const int64_t t = t0 + k;
+      double g = 0.0;
+      if (t < n) {
+        if (t + 1 < n) {
+          const double inv_d = 1.0 / (cdf[t + 1] - cdf[t] + kEpsilon);
+          const double db = double(x[t + 1]) - double(x[t]);
+          g += db * (inv_d * inv_d * h1[t] - inv_d * h0[t]);
+        }
+        if (t >= 1) {
+          const double inv_d = 1.0 / (cdf[t] - cdf[t - 1] + kEpsilon);
+          const double db = double(x[t]) - double(x[t - 1]);
+          g -= db * inv_d * inv_d * h1[t - 1];
+        }
+      }
+      gc[k] = g;
+    }
+    double seg = 0.0;
+#pragma unroll
+    for (int k = kTailPer - 1; k >= 0; --k) {
+      seg += gc[k];
+      gc[k] = seg;  // sum over k' >= k within the thread's nodes
+    }
+    double total;
+    const double excl = block_exclusive_scan(seg, scratch, &total);
+    const double after = carry + (total - excl - seg);  // nodes right of this thread's
+#pragma unroll
+    for (int k = 0; k < kTailPer; ++k) {
+      const int64_t t = t0 + k;
+      if (t >= n) continue;
       const double S_t = gc[k] + after;
-      const double S_next = (k + 1 < MAXPER && k + 1 < per) ? gc[k + 1] + after : after;
+      const double S_next = k + 1 < kTailPer ? gc[k + 1] + after : after;
+      double g = 0.0;
       if (t >= 1) g += 0.5 * (double(x[t]) - double(x[t - 1])) * acc * S_t;
       if (t + 1 < n) g += 0.5 * (double(x[t + 1]) - double(x[t])) * acc * S_next;
       local_norm += g * double(s[t]);
+      h0[t] = g;
     }
-    grho[k] = g;
+    carry += total;
   }
   const double grad_norm = -block_sum(local_norm, scratch) * inv_n * inv_n;
   Real* out = grad_density + in.node_off;
+  for (int64_t base = 0; base < n; base += tile) {
 #pragma unroll
-  for (int k = 0; k < MAXPER; ++k) {
-    const int64_t t = t0 + k;
-    if (k < per && t < n) {
+    for (int k = 0; k < kTailPer; ++k) {
+      const int64_t t = base + mine + k;
+      if (t >= n) continue;
       double wgt = 0.0;
       if (t >= 1) wgt += 0.5 * (double(x[t]) - double(x[t - 1]));
       if (t + 1 < n) wgt += 0.5 * (double(x[t + 1]) - double(x[t]));
-      out[t] = static_cast<Real>(grho[k] * inv_n + grad_norm * wgt);
+      out[t] = static_cast<Real>(h0[t] * inv_n + grad_norm * wgt);
     }
   }
 }
@@ -1733,7 +1745,7 @@ __device__ __forceinline__ void add_f64(double* p, double v) {
 // GLOBAL (global-table instances): only the guide is staged in LDS; srt and
 // coef are read from the blob and the fp64 histograms are the chunk's partial
 // sums in global memory.
-template <typename Real, int MAXPER, bool GLOBAL>
+template <typename Real, bool GLOBAL>
 __device__ __forceinline__ void vjp_body(const BatchedParams& P, const Real* grad,
                                          Real* grad_density, uint8_t* lds, int c) {
   using Coef = typename CoefOf<Real>::type;
@@ -1777,7 +1789,7 @@ __device__ __forceinline__ void vjp_body(const BatchedParams& P, const Real* gra
   double* h1 = h0 + nb;
   uint64_t* a0 = reinterpret_cast<uint64_t*>(h0);
   uint64_t* a1 = reinterpret_cast<uint64_t*>(h1);
-  constexpr int kDepth = MAXPER < 4 ? MAXPER : 4;  // bins per thread with loads in flight
+  constexpr int kDepth = 4;  // bins per thread with loads in flight
   if (fixed) {
     float2* vb = reinterpret_cast<float2*>(s_bin);
     for (int64_t j0 = tid; j0 < n - 1; j0 += kDepth * nt) {
@@ -2015,7 +2027,7 @@ __device__ __forceinline__ void vjp_body(const BatchedParams& P, const Real* gra
   }
 
   if (in.n_chunks == 1) {
-    vjp_tail<Real, MAXPER>(P, in, T.cdf, T.hdr->inv_n, h0, h1, grad_density, scratch);
+    vjp_tail<Real>(P, in, T.cdf, T.hdr->inv_n, h0, h1, grad_density, scratch);
     return;
   }
   // multi-chunk instance: publish partial sums; the last chunk reduces them in order
@@ -2048,7 +2060,7 @@ __device__ __forceinline__ void vjp_body(const BatchedParams& P, const Real* gra
   }
   if (tid == 0) P.done[ch.instance] = 0u;
   __syncthreads();
-  vjp_tail<Real, MAXPER>(P, in, T.cdf, T.hdr->inv_n, h0, h1, grad_density, scratch);
+  vjp_tail<Real>(P, in, T.cdf, T.hdr->inv_n, h0, h1, grad_density, scratch);
 }
 
 // VJP of a float32 curve with cell tables (fast_tables.hpp): one work-group per
@@ -2062,7 +2074,6 @@ __device__ __forceinline__ void vjp_body(const BatchedParams& P, const Real* gra
 // not finite) goes to float64 sums of g and g (k - T_j) in the chunk's
 // partials.  Then the interpolation adjoint per bin, the CDF and density
 // adjoints (vjp_tail), in the last chunk of the instance to finish.
-template <int MAXPER>
 __device__ void fast_vjp_body(const BatchedParams& P, const float* grad, float* grad_density,
                               uint8_t* lds, int c) {
   __shared__ double scratch[16];
@@ -2331,7 +2342,7 @@ __device__ void fast_vjp_body(const BatchedParams& P, const float* grad, float* 
   }
   __syncthreads();
   if (in.n_chunks == 1) {
-    vjp_tail<float, MAXPER>(P, in, cdf, hdr->inv_n, h0, h1, grad_density, scratch);
+    vjp_tail<float>(P, in, cdf, hdr->inv_n, h0, h1, grad_density, scratch);
     return;
   }
   // multi-chunk instance: publish partial sums; the last chunk reduces them in order
@@ -2363,7 +2374,7 @@ __device__ void fast_vjp_body(const BatchedParams& P, const float* grad, float* 
   }
   if (tid == 0) P.done[ch.instance] = 0u;
   __syncthreads();
-  vjp_tail<float, MAXPER>(P, in, cdf, hdr->inv_n, h0, h1, grad_density, scratch);
+  vjp_tail<float>(P, in, cdf, hdr->inv_n, h0, h1, grad_density, scratch);
 }
 
 // Exclusive prefix maximum over the work-group (wmax: 16 words of LDS).
@@ -2394,7 +2405,7 @@ __device__ uint32_t block_exclusive_max(uint32_t v, uint32_t* wmax) {
 // start of the key's cell and of the next cell, and a bisection between them
 // (usually zero or one step) finishes it.  Partial sums go to the chunk's
 // partial slots of these bins; the last item of the instance reduces them.
-template <typename Real, int MAXPER>
+template <typename Real>
 __device__ void vjp_range_body(const BatchedParams& P, const Real* grad, Real* grad_density,
                                uint8_t* lds, int c, int q) {
   __shared__ double scratch[16];
@@ -2703,7 +2714,7 @@ __device__ void vjp_range_body(const BatchedParams& P, const Real* grad, Real* g
   }
   if (arrivals == 1u) {
     __syncthreads();
-    vjp_tail<Real, MAXPER>(P, in, T.cdf, T.hdr->inv_n, d0, d1, grad_density, scratch);
+    vjp_tail<Real>(P, in, T.cdf, T.hdr->inv_n, d0, d1, grad_density, scratch);
     return;
   }
   // one device-scope fence per work-group; the last of the instance's
@@ -2731,34 +2742,33 @@ __device__ void vjp_range_body(const BatchedParams& P, const Real* grad, Real* g
   }
   if (tid == 0) P.done[ch.instance] = 0u;
   __syncthreads();
-  vjp_tail<Real, MAXPER>(P, in, T.cdf, T.hdr->inv_n, h0, h1, grad_density, scratch);
+  vjp_tail<Real>(P, in, T.cdf, T.hdr->inv_n, h0, h1, grad_density, scratch);
 }
 
 // VJP of the guide-table instances without bin ranges: one work-group per chunk.
-template <typename Real, int MAXPER>
+template <typename Real>
 __global__ void __launch_bounds__(1024) vjp_kernel(BatchedParams P, const Real* grad,
                                                    Real* grad_density) {
   extern __shared__ __align__(16) uint8_t lds[];
   const int c = P.vjp_items[P.n_vjp_cell + blockIdx.x].x;
   if (P.inst[P.chunks[c].instance].global_tables)
-    vjp_body<Real, MAXPER, true>(P, grad, grad_density, lds, c);
+    vjp_body<Real, true>(P, grad, grad_density, lds, c);
   else
-    vjp_body<Real, MAXPER, false>(P, grad, grad_density, lds, c);
+    vjp_body<Real, false>(P, grad, grad_density, lds, c);
 }
 
 // VJP of the float32 cell-table instances: one work-group per chunk (the first
 // n_vjp_cell items).
-template <int MAXPER>
 __global__ void __launch_bounds__(1024) vjp_cell_kernel(BatchedParams P, const float* grad,
                                                         float* grad_density) {
   extern __shared__ __align__(16) uint8_t lds[];
-  fast_vjp_body<MAXPER>(P, grad, grad_density, lds, P.vjp_items[blockIdx.x].x);
+  fast_vjp_body(P, grad, grad_density, lds, P.vjp_items[blockIdx.x].x);
 }
 
 // VJP of the instances with bin ranges: one work-group per (chunk, range).
 // A curve whose CDF turns out not to be monotone has no key interval per
 // range; its range-0 items take the whole chunk with the global-table VJP.
-template <typename Real, int MAXPER>
+template <typename Real>
 __global__ void __launch_bounds__(1024) vjp_range_kernel(BatchedParams P, const Real* grad,
                                                          Real* grad_density) {
   extern __shared__ __align__(16) uint8_t lds[];
@@ -2766,9 +2776,9 @@ __global__ void __launch_bounds__(1024) vjp_range_kernel(BatchedParams P, const 
   const InstanceDesc& in = P.inst[P.chunks[item.x].instance];
   const TableHeader* hdr = reinterpret_cast<const TableHeader*>(P.tables + in.table_off);
   if (hdr->flags & kMonotone)
-    vjp_range_body<Real, MAXPER>(P, grad, grad_density, lds, item.x, item.y);
+    vjp_range_body<Real>(P, grad, grad_density, lds, item.x, item.y);
   else if (item.y == 0)
-    vjp_body<Real, MAXPER, true>(P, grad, grad_density, lds, item.x);
+    vjp_body<Real, true>(P, grad, grad_density, lds, item.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -2830,49 +2840,35 @@ hipError_t launch_pack(const BatchedParams& P, int threads, void* packed, int32_
 }
 
 template <typename Real>
-hipError_t launch_vjp(const BatchedParams& P, int threads, int64_t lds, int maxper,
-                      const void* grad, void* grad_density, hipStream_t stream) {
+hipError_t launch_vjp(const BatchedParams& P, int threads, int64_t lds, const void* grad,
+                      void* grad_density, hipStream_t stream) {
   // cell-table items, other plain items, then bin-range items (independent
   // instances: any order)
-#define QL_VJP_CASE(MP)                                                                      \
-  if (maxper <= MP) {                                                                        \
-    if constexpr (sizeof(Real) == 4) {                                                       \
-      if (P.n_vjp_cell > 0) {                                                                \
-        auto kern = vjp_cell_kernel<MP>;                                                     \
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),             \
-                                           hipFuncAttributeMaxDynamicSharedMemorySize,      \
-                                           int(lds));                                        \
-        if (e != hipSuccess) return e;                                                       \
-        hipLaunchKernelGGL(kern, dim3(unsigned(P.n_vjp_cell)), dim3(threads), size_t(lds),   \
-                           stream, P, static_cast<const float*>(grad),                       \
-                           static_cast<float*>(grad_density));                               \
-        e = hipGetLastError();                                                               \
-        if (e != hipSuccess) return e;                                                       \
-      }                                                                                      \
-    }                                                                                        \
-    const int64_t grids[2] = {P.n_vjp_plain - P.n_vjp_cell, P.n_vjp_items - P.n_vjp_plain};  \
-    for (int kind = 0; kind < 2; ++kind) {                                                   \
-      if (grids[kind] == 0) continue;                                                        \
-      auto kern = kind == 0 ? vjp_kernel<Real, MP> : vjp_range_kernel<Real, MP>;             \
-      hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),               \
-                                         hipFuncAttributeMaxDynamicSharedMemorySize,        \
-                                         int(lds));                                          \
-      if (e != hipSuccess) return e;                                                         \
-      hipLaunchKernelGGL(kern, dim3(unsigned(grids[kind])), dim3(threads), size_t(lds),      \
-                         stream, P, static_cast<const Real*>(grad),                          \
-                         static_cast<Real*>(grad_density));                                  \
-      e = hipGetLastError();                                                                 \
-      if (e != hipSuccess) return e;                                                         \
-    }                                                                                        \
-    return hipSuccess;                                                                       \
+  if constexpr (sizeof(Real) == 4) {
+    if (P.n_vjp_cell > 0) {
+      hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(vjp_cell_kernel),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+      if (e != hipSuccess) return e;
+      hipLaunchKernelGGL(vjp_cell_kernel, dim3(unsigned(P.n_vjp_cell)), dim3(threads),
+                         size_t(lds), stream, P, static_cast<const float*>(grad),
+                         static_cast<float*>(grad_density));
+      e = hipGetLastError();
+      if (e != hipSuccess) return e;
+    }
   }
-  QL_VJP_CASE(2)
-  QL_VJP_CASE(4)
-  QL_VJP_CASE(8)
-  QL_VJP_CASE(16)
-  QL_VJP_CASE(32)
-#undef QL_VJP_CASE
-  return hipErrorInvalidValue;
+  const int64_t grids[2] = {P.n_vjp_plain - P.n_vjp_cell, P.n_vjp_items - P.n_vjp_plain};
+  for (int kind = 0; kind < 2; ++kind) {
+    if (grids[kind] == 0) continue;
+    auto kern = kind == 0 ? vjp_kernel<Real> : vjp_range_kernel<Real>;
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(kern, dim3(unsigned(grids[kind])), dim3(threads), size_t(lds), stream, P,
+                       static_cast<const Real*>(grad), static_cast<Real*>(grad_density));
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 template hipError_t launch_tables<float>(const BatchedParams&, int, bool, int64_t, bool,
@@ -2883,9 +2879,9 @@ template hipError_t launch_sample<float>(const BatchedParams&, int, bool, int64_
 template hipError_t launch_sample<double>(const BatchedParams&, int, bool, int64_t, hipStream_t);
 template hipError_t launch_pack<float>(const BatchedParams&, int, void*, int32_t*, hipStream_t);
 template hipError_t launch_pack<double>(const BatchedParams&, int, void*, int32_t*, hipStream_t);
-template hipError_t launch_vjp<float>(const BatchedParams&, int, int64_t, int, const void*, void*,
+template hipError_t launch_vjp<float>(const BatchedParams&, int, int64_t, const void*, void*,
                                       hipStream_t);
-template hipError_t launch_vjp<double>(const BatchedParams&, int, int64_t, int, const void*, void*,
+template hipError_t launch_vjp<double>(const BatchedParams&, int, int64_t, const void*, void*,
                                        hipStream_t);
 
 }  // namespace qloits

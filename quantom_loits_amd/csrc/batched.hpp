@@ -56,7 +56,7 @@ template <typename Real>
 hipError_t launch_pack(const BatchedParams& P, int threads, void* packed, int32_t* packed_bins,
                        hipStream_t stream);
 template <typename Real>
-hipError_t launch_vjp(const BatchedParams& P, int threads, int64_t lds, int maxper,
-                      const void* grad, void* grad_density, hipStream_t stream);
+hipError_t launch_vjp(const BatchedParams& P, int threads, int64_t lds, const void* grad,
+                      void* grad_density, hipStream_t stream);
 
 }  // namespace qloits

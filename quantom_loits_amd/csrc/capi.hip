@@ -517,10 +517,9 @@ int loits_hip_batched_backward(const loits_hip_batched_args* args, const void* g
                                "batched tables launch");
       if (rc != LOITS_HIP_OK) return rc;
     }
-    const int maxper = int((L.max_nodes + L.threads - 1) / L.threads);
-    return check_hip(f32 ? launch_vjp<float>(P, L.threads, L.lds_backward, maxper, grad_events,
+    return check_hip(f32 ? launch_vjp<float>(P, L.threads, L.lds_backward, grad_events,
                                              grad_density, s)
-                         : launch_vjp<double>(P, L.threads, L.lds_backward, maxper, grad_events,
+                         : launch_vjp<double>(P, L.threads, L.lds_backward, grad_events,
                                               grad_density, s),
                      "batched vjp launch");
   } catch (const std::exception& e) {
