@@ -1541,9 +1541,33 @@ __global__ void __launch_bounds__(1024) pack_kernel(BatchedParams P, Real* packe
     const int64_t src = in.out_off + int64_t(sg.p_lo - in.counter_base);
     const int64_t dst = P.chunk_row[blockIdx.x] + P.seg_row[seg];
     const Real* ev = static_cast<const Real*>(P.events);
-    for (int64_t r = lane_id(); r < cnt; r += 64) {
-      packed[dst + r] = ev[src + r];
-      if (packed_bins && P.bins) packed_bins[dst + r] = P.bins[src + r];
+    const int lane = lane_id();
+    // eight rows per lane in flight: all loads before the stores
+    for (int64_t r0 = 0; r0 < cnt; r0 += 8 * kWave) {
+      Real v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int64_t r = r0 + u * kWave + lane;
+        v[u] = r < cnt ? ev[src + r] : Real(0);
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int64_t r = r0 + u * kWave + lane;
+        if (r < cnt) packed[dst + r] = v[u];
+      }
+      if (packed_bins && P.bins) {
+        int32_t b[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int64_t r = r0 + u * kWave + lane;
+          b[u] = r < cnt ? P.bins[src + r] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int64_t r = r0 + u * kWave + lane;
+          if (r < cnt) packed_bins[dst + r] = b[u];
+        }
+      }
     }
   }
 }
