@@ -1309,7 +1309,7 @@ __device__ __forceinline__ void sample_segment(const BatchedParams& P, int c, co
     }
     if (keepbits == 0xF && ((base_row + row) & 3) == 0) {
       if constexpr (sizeof(Real) == 4) {
-        *reinterpret_cast<float4*>(out + row) = make_float4(v[0], v[1], v[2], v[3]);
+        store_stream4(out + row, v[0], v[1], v[2], v[3]);
       } else {
         reinterpret_cast<double2*>(out + row)[0] = make_double2(v[0], v[1]);
         reinterpret_cast<double2*>(out + row)[1] = make_double2(v[2], v[3]);
@@ -1357,8 +1357,7 @@ __device__ __forceinline__ void sample_segment(const BatchedParams& P, int c, co
         for (int h = 0; h < kTrip; ++h) {
           const uint32_t i = i0 + 64 * h + uint32_t(lane);
           if constexpr (sizeof(Real) == 4) {
-            *reinterpret_cast<float4*>(out + 4 * i) =
-                make_float4(v[4 * h], v[4 * h + 1], v[4 * h + 2], v[4 * h + 3]);
+            store_stream4(out + 4 * i, v[4 * h], v[4 * h + 1], v[4 * h + 2], v[4 * h + 3]);
           } else {
             reinterpret_cast<double2*>(out + 4 * i)[0] = make_double2(v[4 * h], v[4 * h + 1]);
             reinterpret_cast<double2*>(out + 4 * i)[1] = make_double2(v[4 * h + 2], v[4 * h + 3]);
@@ -1375,8 +1374,7 @@ __device__ __forceinline__ void sample_segment(const BatchedParams& P, int c, co
         const uint32_t i = i0 + 64 * h + uint32_t(lane);
         if (vec && 4 * i + 4 <= span) {
           if constexpr (sizeof(Real) == 4) {
-            *reinterpret_cast<float4*>(out + 4 * i) =
-                make_float4(v[4 * h], v[4 * h + 1], v[4 * h + 2], v[4 * h + 3]);
+            store_stream4(out + 4 * i, v[4 * h], v[4 * h + 1], v[4 * h + 2], v[4 * h + 3]);
           } else {
             reinterpret_cast<double2*>(out + 4 * i)[0] = make_double2(v[4 * h], v[4 * h + 1]);
             reinterpret_cast<double2*>(out + 4 * i)[1] = make_double2(v[4 * h + 2], v[4 * h + 3]);
@@ -2252,7 +2250,7 @@ __device__ void fast_vjp_body(const BatchedParams& P, const float* grad, float* 
         for (int h = 0; h < kTrip; ++h) {
           const uint32_t i = i0 + 64 * h + uint32_t(lane);
           if (whole || (vec && 4 * i + 4 <= span)) {
-            const float4 g4 = *reinterpret_cast<const float4*>(grad + base + 4 * int64_t(i));
+            const float4 g4 = load_stream4(grad + base + 4 * int64_t(i));
             gv[4 * h] = g4.x;
             gv[4 * h + 1] = g4.y;
             gv[4 * h + 2] = g4.z;
@@ -2317,7 +2315,7 @@ __device__ void fast_vjp_body(const BatchedParams& P, const float* grad, float* 
 #pragma unroll
       for (int h = 0; h < kTrip; ++h) {
         if (kb[h] == 0xF && (rw[h] & 3) == 0) {
-          const float4 g4 = *reinterpret_cast<const float4*>(grad + rw[h]);
+          const float4 g4 = load_stream4(grad + rw[h]);
           gv[4 * h] = g4.x;
           gv[4 * h + 1] = g4.y;
           gv[4 * h + 2] = g4.z;
@@ -2662,7 +2660,7 @@ __device__ void vjp_range_body(const BatchedParams& P, const Real* grad, Real* g
         Real g[4];
         if (R == 1 && kb[h] == 0xF && (rw[h] & 3) == 0) {
           if constexpr (fixed) {
-            const float4 g4 = *reinterpret_cast<const float4*>(grad + rw[h]);
+            const float4 g4 = load_stream4(reinterpret_cast<const float*>(grad + rw[h]));
             g[0] = g4.x; g[1] = g4.y; g[2] = g4.z; g[3] = g4.w;
           } else {
             const double2 ga = reinterpret_cast<const double2*>(grad + rw[h])[0];

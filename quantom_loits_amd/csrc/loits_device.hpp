@@ -168,6 +168,21 @@ __device__ __forceinline__ T wave_inclusive_scan(T v) {
   return v;
 }
 
+// Streaming (non-temporal) 16-byte store and load: events and cotangents are
+// written or read once per launch, and keeping them out of the caches leaves
+// room for the curve tables and the other operand.
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void store_stream4(float* p, float a, float b, float c, float d) {
+  const f32x4_t v = {a, b, c, d};
+  __builtin_nontemporal_store(v, reinterpret_cast<f32x4_t*>(p));
+}
+
+__device__ __forceinline__ float4 load_stream4(const float* p) {
+  const f32x4_t v = __builtin_nontemporal_load(reinterpret_cast<const f32x4_t*>(p));
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+
 // Inclusive running maximum over the lanes of a wave (0 is the identity).
 __device__ __forceinline__ uint32_t wave_inclusive_max(uint32_t v) {
   uint32_t t;
