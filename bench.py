@@ -370,6 +370,7 @@ def cpu_child():
     from oracle.oracle import Oracle, available
     from quantom_loits_amd.workloads import reference_1d_inputs, uniform_family
 
+    model, phys, logical = cpu_description()  # before OpenMP pins this thread to one place
     if available("openmp", portable=True):
         lib, kind = Oracle("openmp", portable=True), "reference"
     else:
@@ -391,7 +392,6 @@ def cpu_child():
             fwd.append(t1 - t0)
             both.append(t2 - t0)
     n_ev = b * E
-    model, phys, logical = cpu_description()
     print(json.dumps(dict(
         value=n_ev / statistics.median(both), unit="events/s", cores=threads if kind ==
         "reference" else 1, kind=kind,

@@ -109,15 +109,24 @@ def _mean(v):
     return sum(v) / len(v) if v else float("nan")
 
 
+def _median(v):
+    if not v:
+        return float("nan")
+    w = sorted(v)
+    m = len(w) // 2
+    return w[m] if len(w) % 2 else 0.5 * (w[m - 1] + w[m])
+
+
 def cmd_trace(a):
     by = trace(a.dir)
     print(f"# Kernel trace: {a.title}\n")
     print("Command (from the repository root on the GPU host, `cd /tmp && export "
           "TMPDIR=/tmp` first):\n\n```\n" + a.command + "\n```\n")
-    print("| kernel | calls | average (us) | min (us) | max (us) |")
-    print("| --- | --- | --- | --- | --- |")
+    print("| kernel | calls | average (us) | median (us) | min (us) | max (us) |")
+    print("| --- | --- | --- | --- | --- | --- |")
     for k, v in sorted(by.items(), key=lambda kv: -sum(kv[1])):
-        print(f"| `{k[:90]}` | {len(v)} | {_mean(v):.1f} | {min(v):.1f} | {max(v):.1f} |")
+        print(f"| `{k[:90]}` | {len(v)} | {_mean(v):.1f} | {_median(v):.1f} | {min(v):.1f} | "
+              f"{max(v):.1f} |")
 
 
 def _kv(pairs, conv=str):
@@ -147,10 +156,10 @@ def cmd_traffic(a):
           f"(`__amd_rocclr_copyBuffer`) reads FETCH_SIZE = {c_fetch / 1024:.1f} KiB and "
           f"WRITE_SIZE = {c_write / 1024:.1f} KiB per dispatch, so bytes = FETCH_SIZE x "
           f"{f_fetch:.4f} and WRITE_SIZE x {f_write:.4f}.\n")
-    print("Achieved = scaled traffic / the kernel's average duration in the kernel trace; "
+    print("Achieved = scaled traffic / the kernel's median duration in the kernel trace; "
           "peak 8 TB/s.\n")
     print("| kernel | FETCH_SIZE (KiB) | WRITE_SIZE (KiB) | traffic (MB) | algorithmic (MB) | "
-          "average (us) | traffic GB/s | algorithmic GB/s | algorithmic of peak |")
+          "median (us) | traffic GB/s | algorithmic GB/s | algorithmic of peak |")
     print("| --- | --- | --- | --- | --- | --- | --- | --- | --- |")
     kernels = {}
     for k in sorted(set(fetch) | set(write)):
@@ -160,10 +169,10 @@ def cmd_traffic(a):
         wk = _mean(write.get(k, {}).get("WRITE_SIZE", []))
         t = fk * 1024 * f_fetch + wk * 1024 * f_write
         name = short_name(k)
-        dur = _mean(durations.get(k, []))
+        dur = _median(durations.get(k, []))
         ab = alg.get(name)
         kernels[name] = dict(traffic_bytes=t, fetch_kib=fk, write_kib=wk,
-                             average_us=dur if dur == dur else None)
+                             median_us=dur if dur == dur else None)
         row = [f"`{k[:80]}`", f"{fk:.1f}", f"{wk:.1f}", f"{t / 1e6:.1f}",
                f"{ab / 1e6:.1f}" if ab else "", f"{dur:.1f}" if dur == dur else "",
                f"{t / (dur * 1e-6) / 1e9:.0f}" if dur == dur else "",
