@@ -578,6 +578,26 @@ __device__ void bitonic_sort_u32(uint32_t* a, int64_t pw) {
 // thresholds and the cell words (fast_blob_layout) for the VJP.
 // cdf_copy (global, n doubles, private to the work-group) receives the CDF for
 // the float64 re-evaluation of values near a keep boundary.
+// Keep class of a bin of a monotone curve under a window [lo, hi], from its
+// end nodes a, b: every fp64 value of the bin lies in [min(a, b), max(a, b)]
+// (x_j + m (u - C_j) with 0 <= (u - C_j) m / (x_{j+1} - x_j) < 1), so
+//   1: all kept (one sign, inside the window by a relative 4e-6),
+//   2: none kept (outside it by as much),
+//   0: decided per value.
+__device__ __forceinline__ uint32_t window_class(double a, double b, double lo, double hi) {
+  const double vlo = a < b ? a : b, vhi = a < b ? b : a;
+  if (!(__builtin_isfinite(vlo) && __builtin_isfinite(vhi))) return 0u;
+  constexpr double e = 4e-6;
+  const double alo = __builtin_fabs(lo), ahi = __builtin_fabs(hi);
+  const double avlo = __builtin_fabs(vlo), avhi = __builtin_fabs(vhi);
+  if (vhi < lo - e * (alo > avhi ? alo : avhi) || vlo > hi + e * (ahi > avlo ? ahi : avlo))
+    return 2u;
+  if ((vlo > 0.0 || vhi < 0.0) && vlo > lo + e * (alo > avlo ? alo : avlo) &&
+      vhi < hi - e * (ahi > avhi ? ahi : avhi))
+    return 1u;
+  return 0u;
+}
+
 template <bool SEQUENTIAL>
 __device__ void build_fast(const BatchedParams& P, int inst, const InstanceDesc& in, uint8_t* lds,
                            bool writer, BuildShared& sh, double* cdf_copy) {
@@ -631,6 +651,9 @@ __device__ void build_fast(const BatchedParams& P, int inst, const InstanceDesc&
       const double r = double(thr) * 0x1.0p-24 - c;
       b.x = static_cast<float>(m * 0x1.0p-24);
       b.y = static_cast<float>(__builtin_fma(m, r, xt));
+      // keep class for a window (not the last bin: it also takes keys past C_{n-1})
+      if (P.use_window && t + 2 < n)
+        b.w = __uint_as_float(window_class(xt, double(x[t + 1]), P.win_lo, P.win_hi));
     }
     bins[t] = b;
   }
@@ -1207,6 +1230,7 @@ struct CellLookup {
   const float* x;       // global
   int s;
   bool safe;
+  bool classed;  // window keep classes in the bins' fourth word (monotone curve)
 
   template <int NK>
   __device__ __forceinline__ int operator()(const uint32_t (&k)[NK], int (&j)[NK], float (&v)[NK],
@@ -1225,11 +1249,8 @@ struct CellLookup {
     }
     const float lo = static_cast<float>(P.win_lo), hi = static_cast<float>(P.win_hi);
     int keep = 0, unsafe = 0;
-#pragma unroll
-    for (int e = 0; e < NK; ++e) {
-      v[e] = __builtin_fmaf(
-          b[e].x, static_cast<float>(static_cast<int32_t>(k[e] - __float_as_uint(b[e].z))),
-          b[e].y);
+    // the float32 decision, or "unsafe" where it could differ from fp64's
+    auto decide = [&](int e) {
       const float a = __builtin_fabsf(v[e]);
       bool ok = a >= 1e-30f && a <= 1e30f;
       bool in = true;
@@ -1240,6 +1261,28 @@ struct CellLookup {
       }
       keep |= (ok && in) ? (1 << e) : 0;
       unsafe |= ok ? 0 : (1 << e);
+    };
+#pragma unroll
+    for (int e = 0; e < NK; ++e)
+      v[e] = __builtin_fmaf(
+          b[e].x, static_cast<float>(static_cast<int32_t>(k[e] - __float_as_uint(b[e].z))),
+          b[e].y);
+    if (WINDOW && classed) {  // the bin's class decides, except in straddling bins
+      int check = 0;
+#pragma unroll
+      for (int e = 0; e < NK; ++e) {
+        const uint32_t cls = __float_as_uint(b[e].w);
+        keep |= int(cls & 1u) << e;
+        check |= (cls == 0u ? 1 : 0) << e;
+      }
+      if (ballot(check != 0) != 0ull) {
+#pragma unroll
+        for (int e = 0; e < NK; ++e)
+          if ((check >> e) & 1) decide(e);
+      }
+    } else {
+#pragma unroll
+      for (int e = 0; e < NK; ++e) decide(e);
     }
     if (__builtin_expect(unsafe != 0, 0)) {
 #pragma unroll 1
@@ -1517,7 +1560,8 @@ __global__ void __launch_bounds__(1024) sample_cell_kernel(BatchedParams P) {
                               reinterpret_cast<const uint32_t*>(lds + L.srt),
                               reinterpret_cast<const float4*>(lds + L.bins) - 1, cdf_copy,
                               static_cast<const float*>(P.x) + in.node_off, 24 - in.fast_log2,
-                              (sh.flags & kSafe) != 0};
+                              (sh.flags & kSafe) != 0,
+                              WINDOW && (sh.flags & kMonotone) != 0};
   for (int sgi = next_segment(&sh_next); sgi < P.segs; sgi = next_segment(&sh_next))
     sample_segment<float, WINDOW>(P, c, ch, in, sgi, lk, sh_kept);
   chunk_rows(P, c, sh_kept);
