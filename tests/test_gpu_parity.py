@@ -312,6 +312,46 @@ def test_batched_ragged_window_matches_oracle(oracle_port, hip_device):
             assert err <= (1e-10 if dtype == torch.float64 else 1e-4), (i, err)
 
 
+@pytest.mark.parametrize("fmt", [0, 1])
+def test_window_keep_classes_match_oracle(oracle_port, hip_device, fmt):
+    """Window cuts whose per-bin keep classes (all kept / none kept / per value)
+    meet their edge cases: window edges exactly on nodes, decreasing x, curves
+    wholly inside or outside the window, bins straddling an edge by 1e-7, and
+    a non-monotone CDF (negative densities; classes off).  float32, bins and
+    kept counts exact, values within 1e-5 of the oracle on the same data."""
+    from quantom_loits_amd.batched import BatchedLOITS, Tuning
+
+    window = (0.25, 0.75)
+    grid = np.linspace(0.0, 1.0, 257)  # holds 0.25 and 0.75 exactly
+    rng = np.random.default_rng(31)
+    xs = [grid, grid[::-1].copy(), np.linspace(0.3, 0.7, 200), np.linspace(2.0, 3.0, 150),
+          0.25 + 1e-7 * np.arange(-60, 61), grid]
+    ds = [rng.uniform(0.2, 2.0, len(v)) for v in xs]
+    ds[5] = np.sin(np.linspace(0.0, 9.0, len(grid))) + 0.3  # dips below zero
+    n_nodes = np.array([len(v) for v in xs], np.int64)
+    events = np.full(len(xs), 20000, np.int64)
+    node_off = np.concatenate([[0], np.cumsum(n_nodes)[:-1]]).astype(np.int64)
+    base = np.concatenate([[0], np.cumsum(events)[:-1]]).astype(np.uint64)
+    streams = np.arange(len(xs), dtype=np.uint64) * 2
+    x32 = np.concatenate(xs).astype(np.float32)
+    d32 = np.concatenate(ds).astype(np.float32)
+    ref_v, ref_b, ref_k, _, _ = oracle_port.batched_forward(
+        x32.astype(np.float64), d32.astype(np.float64), node_off, n_nodes, events, base,
+        streams, 77, window=window)
+    assert ref_k[3] == 0 and ref_k[2] > 0
+    s = BatchedLOITS(n_nodes, events, base, streams, None, node_off, dtype=torch.float32,
+                     window=window, tuning=Tuning(threads=256, chunk_events=4096, scan_mode=1,
+                                                  table_format=fmt))
+    if fmt == 0:
+        assert s.layout["cell_instances"] == len(xs)
+    values, kept, _, ex = s.run_forward(torch.from_numpy(x32).cuda(), torch.from_numpy(d32).cuda(),
+                                        77, want_bins=True)
+    assert kept.cpu().numpy().tolist() == ref_k.tolist()
+    assert np.array_equal(ex["bins"].cpu().numpy(), ref_b)
+    v = values.cpu().numpy().astype(np.float64)
+    assert np.max(np.abs(v - ref_v) / np.maximum(np.abs(ref_v), 1e-30)) <= 1e-5
+
+
 @pytest.mark.parametrize("dtype,fmt", [(torch.float64, 0), (torch.float32, 0), (torch.float32, 1)])
 def test_batched_tiny_curves_and_event_counts_match_oracle(oracle_port, hip_device, dtype, fmt):
     """Two- and three-node curves (one and two bins) and event counts at the
