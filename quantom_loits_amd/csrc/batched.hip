@@ -2506,24 +2506,16 @@ __device__ void vjp_range_body(const BatchedParams& P, const Real* grad, Real* g
 
   uint32_t* cell_bin = reinterpret_cast<uint32_t*>(lds);  // [kRangeCells + 4]
   uint32_t* thr = cell_bin + (kRangeCells + 4);           // [W + 4]: T_lo .. T_hi
-  uint8_t* bd = reinterpret_cast<uint8_t*>(thr + (W + 4));  // [W] per-bin data
-  uint64_t* a0 = reinterpret_cast<uint64_t*>(bd + int64_t(8) * W);
+  // float64: C_j per bin; float32 keeps none (it sums q and q k exactly)
+  uint8_t* bd = reinterpret_cast<uint8_t*>(thr + (W + 4));
+  uint64_t* a0 = reinterpret_cast<uint64_t*>(bd + (fixed ? 0 : int64_t(8) * W));
   uint64_t* a1 = a0 + W;
   double* d0 = reinterpret_cast<double*>(a0);
   double* d1 = reinterpret_cast<double*>(a1);
 
   for (int t = tid; t <= nbins; t += nt) thr[t] = T.srt[lo + t];
-  for (int i = tid; i < nbins; i += nt) {
-    const int j = lo + i;
-    if constexpr (fixed) {  // t / tau_d = float(k - T_j) alpha + beta
-      const double tau_d = vjp_tau_d(T.cdf, j, n);
-      const double r = double(T.srt[j]) * 0x1.0p-24 - T.cdf[j];
-      reinterpret_cast<float2*>(bd)[i] =
-          make_float2(static_cast<float>(0x1.0p-24 / tau_d), static_cast<float>(r / tau_d));
-    } else {  // t = u - C_j
-      reinterpret_cast<double*>(bd)[i] = T.cdf[j];
-    }
-  }
+  if constexpr (!fixed)
+    for (int i = tid; i < nbins; i += nt) reinterpret_cast<double*>(bd)[i] = T.cdf[lo + i];
   for (int i = tid; i < 2 * W; i += nt) a0[i] = 0u;
   for (int i = lo + tid; i < hi; i += nt) {
     outl[i] = 0.0;
@@ -2587,15 +2579,15 @@ __device__ void vjp_range_body(const BatchedParams& P, const Real* grad, Real* g
     const uint32_t gm = sh_gmax;
     const int ex = int(gm >> 23);
     if (ex == 255) {
-      scale = __builtin_nanf("");
-    } else {
-      scale_log2 = 24 - (ex - 126);
+      scale = __builtin_nanf("");  // inf / NaN cotangents: every value takes the float64 sums
+    } else {  // |round(g 2^e)| < 2^kFastQBits, as in the cell-table VJP
+      scale_log2 = (kFastQBits - 2) - (ex - 126);
       scale_log2 = scale_log2 > 126 ? 126 : scale_log2;
       scale = __builtin_ldexpf(1.0f, scale_log2);
     }
   }
+  constexpr float kQLimit = float(1 << kFastQBits);
   const int jmax = n - 2 - lo;  // relative clamp (only the last range reaches it)
-  const float2* s_vb = reinterpret_cast<const float2*>(bd);
   const double* s_c = reinterpret_cast<const double*>(bd);
 
   // segments of the chunk, taken from the work-group's counter
@@ -2684,18 +2676,11 @@ __device__ void vjp_range_body(const BatchedParams& P, const Real* grad, Real* g
           if (a[e] >= b[e]) pending &= ~(1u << e);
         }
       }
-      float th[NK];
       double tc[NK];
   #pragma unroll
       for (int e = 0; e < NK; ++e) {
         a[e] = a[e] > jmax ? jmax : a[e];
-        if constexpr (fixed) {
-          const float2 ab = s_vb[a[e]];
-          th[e] = __builtin_fmaf(static_cast<float>(static_cast<int32_t>(k[e] - thr[a[e]])), ab.x,
-                                 ab.y);
-        } else {
-          tc[e] = double(k[e]) * 0x1.0p-24 - s_c[a[e]];
-        }
+        if constexpr (!fixed) tc[e] = double(k[e]) * 0x1.0p-24 - s_c[a[e]];
       }
   #pragma unroll
       for (int h = 0; h < kTrip; ++h) {
@@ -2723,20 +2708,21 @@ __device__ void vjp_range_body(const BatchedParams& P, const Real* grad, Real* g
         for (int e = 0; e < 4; ++e) {
           if (!((uh >> e) & 1u)) continue;
           const int jr = a[4 * h + e];
-          if constexpr (fixed) {
+          if constexpr (fixed) {  // a0 = sum q, a1 = sum q k (exact, < 2^62)
             const float gf = float(g[e]);
             const float gs = gf * scale;
-            if (__builtin_expect(__builtin_fabsf(gs) < 0x1.0p30f, 1)) {
-              const int ia = static_cast<int>(__builtin_rintf(gs));
-              const int ib = static_cast<int>(__builtin_rintf(gs * th[4 * h + e]));
+            const uint32_t ke = k[4 * h + e];
+            if (__builtin_expect(__builtin_fabsf(gs) < kQLimit, 1)) {
+              const int qv = static_cast<int>(__builtin_rintf(gs));
               atomicAdd(reinterpret_cast<unsigned long long*>(a0 + jr),
-                        static_cast<unsigned long long>(static_cast<int64_t>(ia)));
+                        static_cast<unsigned long long>(static_cast<int64_t>(qv)));
               atomicAdd(reinterpret_cast<unsigned long long*>(a1 + jr),
-                        static_cast<unsigned long long>(static_cast<int64_t>(ib)));
+                        static_cast<unsigned long long>(static_cast<int64_t>(qv) * int64_t(ke)));
             } else {
               sh_outl = 1;
               unsafeAtomicAdd(outl + lo + jr, double(gf));
-              unsafeAtomicAdd(outl + nb + lo + jr, double(gf) * double(th[4 * h + e]));
+              unsafeAtomicAdd(outl + nb + lo + jr,
+                              double(gf) * double(static_cast<int32_t>(ke - thr[jr])));
             }
           } else {
             const double ge = double(g[e]);
@@ -2759,12 +2745,14 @@ __device__ void vjp_range_body(const BatchedParams& P, const Real* grad, Real* g
     for (int i = tid; i < nbins; i += nt) {
       const int j = lo + i;
       double v0, v1;
-      if constexpr (fixed) {
-        v0 = double(static_cast<int64_t>(a0[i])) * inv_scale +
-             (any_outl ? load_l2(outl + j) : 0.0);
-        v1 = (double(static_cast<int64_t>(a1[i])) * inv_scale +
+      if constexpr (fixed) {  // sum g (u - C_j) = 2^-24 sum g (k - T_j) + (T_j 2^-24 - C_j) sum g
+        const uint64_t A0 = a0[i], A1 = a1[i];
+        const uint32_t Tj = thr[i];
+        v0 = double(static_cast<int64_t>(A0)) * inv_scale + (any_outl ? load_l2(outl + j) : 0.0);
+        v1 = (double(static_cast<int64_t>(A1 - uint64_t(Tj) * A0)) * inv_scale +
               (any_outl ? load_l2(outl + nb + j) : 0.0)) *
-             vjp_tau_d(T.cdf, j, n);
+                 0x1.0p-24 +
+             (double(Tj) * 0x1.0p-24 - T.cdf[j]) * v0;
       } else {
         v0 = d0[i];
         v1 = d1[i];

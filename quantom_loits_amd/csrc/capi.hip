@@ -184,7 +184,7 @@ int make_plan(const loits_hip_instance* instances, int64_t n, int dtype,
       if (vjp_bytes(d.guide_log2) > kMaxLds) {  // the VJP takes bin ranges; fit the build only
         d.guide_log2 = g_want;
         while (d.guide_log2 > kMinGuideLog2 && build_bytes(d.guide_log2) > kMaxLds) --d.guide_log2;
-        d.vjp_ranges = vjp_range_count(s.n_nodes);
+        d.vjp_ranges = vjp_range_count(s.n_nodes, dtype == LOITS_HIP_F32);
       }
       d.global_tables = build_bytes(d.guide_log2) > kMaxLds ? 1 : 0;
       if (d.global_tables) d.guide_log2 = g_want;
@@ -205,22 +205,28 @@ int make_plan(const loits_hip_instance* instances, int64_t n, int dtype,
         // which stages the guide)
         ++n_ranged;
         lds_b = std::max(lds_b, std::max(guide_bytes,
-                                         lds_range_bytes(vjp_range_width(s.n_nodes, d.vjp_ranges))));
+                                         lds_range_bytes(vjp_range_width(s.n_nodes, d.vjp_ranges),
+                                                         dtype == LOITS_HIP_F32)));
       } else {
         lds_b = std::max(lds_b, (tl.cdf - tl.thr) + pad4(s.n_nodes) * 16);
       }
     }
     L.max_nodes = std::max(L.max_nodes, s.n_nodes);
     L.total_nodes = std::max(L.total_nodes, s.node_off + s.n_nodes);
-    const int64_t nch = std::max<int64_t>(1, (s.events + chunk_events - 1) / chunk_events);
+    // float32 bin-range VJPs sum q k as 64-bit integers: at most kFastMaxChunk
+    // keys per chunk keep the sums below 2^62
+    const int64_t ce = (dtype == LOITS_HIP_F32 && d.vjp_ranges > 0)
+                           ? std::min<int64_t>(chunk_events, kFastMaxChunk)
+                           : chunk_events;
+    const int64_t nch = std::max<int64_t>(1, (s.events + ce - 1) / ce);
     if (plan->chunks.size() + size_t(nch) > size_t(INT32_MAX))
       return fail(LOITS_HIP_EUNSUPPORTED, "batched: too many tiles");
     d.first_chunk = int32_t(plan->chunks.size());
     d.n_chunks = int32_t(nch);
     for (int64_t c = 0; c < nch; ++c) {
       ChunkDesc ch{};
-      ch.slot_lo = c * chunk_events;
-      ch.slot_hi = std::min(s.events, (c + 1) * chunk_events);
+      ch.slot_lo = c * ce;
+      ch.slot_hi = std::min(s.events, (c + 1) * ce);
       ch.instance = int32_t(i);
       ch.index = int32_t(c);
       const uint64_t p_lo = s.counter_base + uint64_t(ch.slot_lo);

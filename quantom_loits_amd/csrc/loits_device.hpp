@@ -449,16 +449,19 @@ __host__ __device__ __forceinline__ int64_t lds_global_build_bytes(int guide_log
 }
 
 // Bin-range VJP (curves whose VJP tables and histograms do not fit LDS): the
-// bins of a curve are split into R ranges of at most kRangeBins bins; one work
-// item (chunk, range) sums the keys of its range over a cell table of at most
-// kRangeCells cells.  LDS: cell_bin[kRangeCells + 4] | thr[W + 4] | per-bin
-// data[W] (8 B) | two 64-bit histograms[W].
+// bins of a curve are split into R ranges; one work item (chunk, range) sums
+// the keys of its range over a cell table of at most kRangeCells cells.  LDS:
+// cell_bin[kRangeCells + 4] | thr[W + 4] | [float64: C_j per bin, 8 B] | two
+// 64-bit histograms[W].  float32 sums exact integers (no per-bin data), so its
+// ranges hold up to kRangeBinsF32 bins, float64's kRangeBins.
 constexpr int kRangeBins = 4096;
+constexpr int kRangeBinsF32 = 6144;
 constexpr int kRangeCellsLog2 = 13;
 constexpr int kRangeCells = 1 << kRangeCellsLog2;
 
-__host__ __device__ __forceinline__ int vjp_range_count(int64_t n) {
-  return int((n - 1 + kRangeBins - 1) / kRangeBins);
+__host__ __device__ __forceinline__ int vjp_range_count(int64_t n, bool f32) {
+  const int w = f32 ? kRangeBinsF32 : kRangeBins;
+  return int((n - 1 + w - 1) / w);
 }
 
 // Bins per range (a multiple of 4); every range is non-empty.
@@ -466,8 +469,8 @@ __host__ __device__ __forceinline__ int vjp_range_width(int64_t n, int ranges) {
   return int(pad4((n - 1 + ranges - 1) / ranges));
 }
 
-__host__ __device__ __forceinline__ int64_t lds_range_bytes(int width) {
-  return int64_t(kRangeCells + 4) * 4 + int64_t(width + 4) * 4 + int64_t(width) * 24;
+__host__ __device__ __forceinline__ int64_t lds_range_bytes(int width, bool f32) {
+  return int64_t(kRangeCells + 4) * 4 + int64_t(width + 4) * 4 + int64_t(width) * (f32 ? 16 : 24);
 }
 
 // Bin of key k from a direct guide word; ko = k - L (the offset in the cell).
