@@ -63,6 +63,11 @@ __global__ void philox_words_kernel(uint32_t* words, int64_t first, int64_t bloc
 // Batched layout.
 namespace {
 
+// Work of one node's share of a table build or staging, in keys (a chunk's
+// sampling loop costs one unit per key): about 15 us per 2048-node build
+// against about 0.75 ns per key per CU.
+constexpr int64_t kBuildKeysPerNode = 10;
+
 int ceil_log2(int64_t v) {
   int l = 0;
   while ((int64_t(1) << l) < v) ++l;
@@ -251,6 +256,27 @@ int make_plan(const loits_hip_instance* instances, int64_t n, int dtype,
       plan->chunks.push_back(ch);
     }
     out += s.events;
+  }
+  // Work-groups are dispatched in index order as CUs free up, so each launch's
+  // chunks (and VJP items) go largest first: the long ones start early and the
+  // short ones fill the end instead of one long chunk running alone.  Cost in
+  // key units: the chunk's keys plus a table build or staging of kBuildKeysPerNode
+  // keys per node.  Stable, so equal costs keep instance order.
+  {
+    auto cost = [&](int32_t ci) {
+      const ChunkDesc& ch = plan->chunks[size_t(ci)];
+      return (ch.slot_hi - ch.slot_lo) +
+             kBuildKeysPerNode * plan->inst[size_t(ch.instance)].n_nodes;
+    };
+    auto by_cost = [&](int32_t a, int32_t b) { return cost(a) > cost(b); };
+    auto item_by_cost = [&](const VjpItem& a, const VjpItem& b) {
+      return cost(a.chunk) > cost(b.chunk);
+    };
+    std::stable_sort(plan->fwd_order.begin(), plan->fwd_order.end(), by_cost);
+    std::stable_sort(guide_chunks.begin(), guide_chunks.end(), by_cost);
+    std::stable_sort(plan->items.begin(), plan->items.end(), item_by_cost);
+    std::stable_sort(guide_items.begin(), guide_items.end(), item_by_cost);
+    std::stable_sort(ranged.begin(), ranged.end(), item_by_cost);
   }
   L.capacity = out;
   L.n_chunks = int64_t(plan->chunks.size());
