@@ -55,6 +55,8 @@ def parse():
     p.add_argument("--guide-shift", type=int, default=2)
     p.add_argument("--scan-mode", type=int, default=0)
     p.add_argument("--table-format", type=int, default=0)
+    p.add_argument("--persistent", type=int, default=0, choices=(0, 1),
+                   help="1: the batched forward and VJP as one persistent launch each")
     p.add_argument("--autotune", action="store_true",
                    help="pick threads / chunk / guide shift with the launch-shape autotuner")
     p.add_argument("--graph", action="store_true",
@@ -217,7 +219,7 @@ def extra_workloads(dev, steps, fp64_row):
     return out
 
 
-def ragged_workload(dev, steps, loop=True):
+def ragged_workload(dev, steps, loop=True, persistent=0):
     """4096 ragged curves (N log-uniform in [128, 16384], E in [1024, 2^20],
     window cut [0.05, 0.95]) as one captured graph (forward, pack, VJP; the
     planner picks each curve's table format and chunks), against a per-curve
@@ -231,7 +233,7 @@ def ragged_workload(dev, steps, loop=True):
     xr = torch.from_numpy(w["x"]).to(dev)
     dr = torch.from_numpy(w["density"]).to(dev)
     base = np.concatenate([[0], np.cumsum(w["events"])[:-1]]).astype(np.uint64)
-    tun = Tuning()
+    tun = Tuning(persistent=persistent)
     sr = BatchedLOITS(w["n_nodes"], w["events"], base, None, None, w["node_off"],
                       dtype=torch.float32, device=dev, window=(0.05, 0.95), tuning=tun)
     g = sr.capture(xr, dr, SEED)
@@ -246,7 +248,7 @@ def ragged_workload(dev, steps, loop=True):
                fused_events_per_s=ev / (fused * 1e-3),
                algorithmic_gbs=(fb + bb) / (fused * 1e-3) / 1e9,
                tuning=dict(threads=tun.threads, chunk_events=tun.chunk_events,
-                           guide_shift=tun.guide_shift),
+                           guide_shift=tun.guide_shift, persistent=tun.persistent),
                cell_table_curves=sr.layout["cell_instances"],
                global_table_curves=sr.layout["global_instances"],
                range_vjp_curves=sr.layout["range_instances"])
@@ -458,13 +460,15 @@ def main():
         print(json.dumps({"dropin_two_axis": dropin_workloads(dev, args.steps, only)}))
         return
     if args.ragged_only:
-        print(json.dumps({"ragged_4096_window": ragged_workload(dev, args.steps, loop=False)}))
+        print(json.dumps({"ragged_4096_window": ragged_workload(dev, args.steps, loop=False,
+                                                                persistent=args.persistent)}))
         return
     x_np, d_np = uniform_family(B, N, seed=0, dtype=np.float32)
     x = torch.from_numpy(x_np.reshape(-1)).to(dev)
     d = torch.from_numpy(d_np.reshape(-1)).to(dev)
     tun = Tuning(threads=args.threads, guide_shift=args.guide_shift, chunk_events=args.chunk,
-                 scan_mode=args.scan_mode, table_format=args.table_format)
+                 scan_mode=args.scan_mode, table_format=args.table_format,
+                 persistent=args.persistent)
     if args.autotune:
         from quantom_loits_amd.autotune import Autotuner
         tun = Autotuner().tuning_for(np.full(B, N), np.full(B, E), torch.float32,

@@ -236,11 +236,21 @@ typedef struct loits_hip_tuning {
                                to [2^12, 2^14] and lowered until the tables fit in LDS */
   int64_t chunk_events;     /* events per work-group (rounded up to 4*threads) */
   int32_t scan_mode;        /* 0 = parallel CDF scan, 1 = the reference's sequential chain */
-  int32_t persistent;       /* reserved, must be 0 */
+  int32_t persistent;       /* 0 = one launch per kind of work: cell-table chunks, guide-table
+                               chunks (forward); cell, guide and bin-range items (VJP).
+                               1 = one persistent launch for the forward and one for the VJP:
+                               resident work-groups take every chunk (VJP item) from a device
+                               queue, largest first, until none is left.  Either way curves
+                               whose per-bin tables exceed LDS ("global tables") are built by
+                               a launch of their own before the forward's sampling. */
   int32_t table_format;     /* 0 = cell tables for float32 curves of <= 4096 nodes when
-                               chunk_events <= 2^18 (guide tables otherwise), 1 = guide
-                               tables for every curve */
+                               their chunks are <= 2^18 events (guide tables otherwise),
+                               1 = guide tables for every curve */
   int32_t reserved;         /* must be 0 */
+  /* Per grid-size bucket b = ceil(log2(n_nodes)) (curves of 2^(b-1) + 1 .. 2^b nodes,
+   * b in 1..15); 0 keeps the values above for that bucket. */
+  int32_t bucket_chunk_log2[16];   /* events per chunk = 2^v, v in 10..30 */
+  int32_t bucket_guide_shift[16];  /* guide_shift + 1, in 1..4 */
 } loits_hip_tuning;
 
 typedef struct loits_hip_layout {
@@ -269,6 +279,8 @@ typedef struct loits_hip_layout {
   int64_t cell_chunks;      /* forward work-groups of the cell-table instances (their own
                                launch; the other chunks take a second one) */
   int64_t backward_cell;    /* VJP work items of the cell-table instances */
+  int64_t forward_items;    /* persistent forward queue length (its chunks) */
+  int64_t lds_persistent_forward;  /* dynamic LDS of the persistent forward (every kind of chunk) */
 } loits_hip_layout;
 
 /* Host only: validate `instances` and compute sizes for `dtype`. */

@@ -71,9 +71,10 @@ def plannable(n_nodes, events, dtype, tuning: Tuning) -> bool:
     table[:, 4] = np.arange(n, dtype=np.int64) * np.asarray(events, np.int64)
     table[:, 6] = 1
     try:
+        bchunk, bshift = tuning.bucket_arrays()
         ext.BatchedPlan(torch.from_numpy(table), DTYPE_CODE[dtype], tuning.threads,
                         tuning.guide_shift, tuning.chunk_events, tuning.scan_mode,
-                        tuning.persistent, tuning.table_format)
+                        tuning.persistent, tuning.table_format, bchunk, bshift)
     except (ValueError, RuntimeError):
         return False
     return True

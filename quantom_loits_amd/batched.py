@@ -15,7 +15,7 @@ reference call on the same B curves with weights 1 and ``n_events = E``.
 from __future__ import annotations
 
 from contextlib import nullcontext
-from dataclasses import dataclass
+from dataclasses import dataclass, field
 
 import numpy as np
 import torch
@@ -27,12 +27,39 @@ DTYPE_CODE = {torch.float64: 0, torch.float32: 1}
 
 @dataclass
 class Tuning:
+    """Launch shape of a batched plan.
+
+    ``persistent=1`` runs the forward as one launch and the VJP as one launch
+    of resident work-groups taking the plan's items from a device queue
+    (largest first); 0 launches each kind of work separately.  ``buckets``
+    maps a grid-size bucket b = ceil(log2(n_nodes)) to ``(chunk_events,
+    guide_shift)`` for the curves of that bucket (either may be None to keep
+    the plan-wide value)."""
+
     threads: int = 1024
     guide_shift: int = 2
     chunk_events: int = 1 << 18
     scan_mode: int = 0        # 0 parallel CDF scan, 1 the reference's sequential chain
     persistent: int = 0
     table_format: int = 0     # 0 cell tables where they apply, 1 guide tables only
+    buckets: dict = field(default_factory=dict)
+
+    def bucket_arrays(self):
+        """(chunk_log2[16], guide_shift + 1 [16]) for the native plan; 0 = plan-wide."""
+        chunk = [0] * 16
+        shift = [0] * 16
+        for b, (ce, gs) in self.buckets.items():
+            b = int(b)
+            if not 1 <= b <= 15:
+                raise ValueError(f"bucket {b} outside 1..15")
+            if ce is not None:
+                ce = int(ce)
+                if ce <= 0 or ce & (ce - 1):
+                    raise ValueError("bucket chunk_events must be a power of two")
+                chunk[b] = ce.bit_length() - 1
+            if gs is not None:
+                shift[b] = int(gs) + 1
+        return chunk, shift
 
 
 class BatchedLOITS:
@@ -75,9 +102,10 @@ class BatchedLOITS:
         self.window = window
         ext = load_extension()
         t = self.tuning
+        bchunk, bshift = t.bucket_arrays()
         self.plan = ext.BatchedPlan(torch.from_numpy(table), DTYPE_CODE[dtype], t.threads,
                                     t.guide_shift, t.chunk_events, t.scan_mode, t.persistent,
-                                    t.table_format)
+                                    t.table_format, bchunk, bshift)
         self.layout = self.plan.layout()
         self.desc = self.plan.descriptors().to(self.device)
         self.n_instances = n
@@ -136,11 +164,12 @@ class BatchedLOITS:
         return bool(st["scratch"][:4].view(torch.int32).item())
 
     # -- convenience -------------------------------------------------------
-    def run_forward(self, x, density, seed, want_bins=False, want_tables=False):
-        """Forward with one host sync; returns (values, kept, state, extras)."""
+    def run_forward(self, x, density, seed, want_bins=False, want_tables=False, state=None):
+        """Forward with one host sync; returns (values, kept, state, extras).
+        ``state`` reuses the buffers of an earlier call (new ones otherwise)."""
         self._check_inputs(x, density)
         with self.region("loits::forward::allocation"):
-            st = self.new_state()
+            st = self.new_state() if state is None else state
         extras = {}
         bins = torch.empty(max(self.capacity, 1), dtype=torch.int32, device=self.device) \
             if want_bins else None

@@ -23,6 +23,8 @@
 // multiply-adds are explicit.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cstddef>
 #include <cmath>
 #include <cstdint>
 
@@ -1494,21 +1496,17 @@ __device__ __forceinline__ void chunk_rows(const BatchedParams& P, int c, const 
   }
 }
 
-// Sampling of the guide-table instances: one work-group per chunk (the chunks
-// after the cell-table ones in fwd_order); waves take segments from a counter.
+// Sampling of one chunk of a guide-table instance by the work-group; waves take
+// segments from a counter.  Ends with the chunk's segment rows (chunk_rows).
 template <typename Real, bool WINDOW, bool SEQUENTIAL>
-__global__ void __launch_bounds__(1024) sample_kernel(BatchedParams P) {
-  extern __shared__ __align__(16) uint8_t lds[];
-  __shared__ BuildShared sh;
-  __shared__ int sh_next;  // next unclaimed segment (read after the barriers below)
-  __shared__ int64_t sh_kept[kWave];  // kept events per segment (at most 64 per chunk)
-  const int c = P.fwd_order[P.n_cell_chunks + blockIdx.x];
+__device__ __forceinline__ void guide_chunk(const BatchedParams& P, int c, uint8_t* lds,
+                                            BuildShared& sh, int* sh_next, int64_t* sh_kept) {
   const ChunkDesc ch = P.chunks[c];
   const InstanceDesc in = P.inst[ch.instance];
   const int64_t nb = pad4(in.n_nodes);
-  if (threadIdx.x == 0) sh_next = 0;
+  if (threadIdx.x == 0) *sh_next = 0;
   const Real* x_g = static_cast<const Real*>(P.x) + in.node_off;
-  if (in.global_tables) {  // built beforehand by tables_kernel; stage the guide
+  if (in.global_tables) {  // built beforehand (tables_kernel or a queue item); stage the guide
     const Tables<Real> T = tables_of<Real>(P.tables, in);
     lds_copy(lds, T.guide, guide_bytes(T, in));
     if (threadIdx.x == 0) sh.flags = T.hdr->flags;
@@ -1519,7 +1517,7 @@ __global__ void __launch_bounds__(1024) sample_kernel(BatchedParams P) {
     const GuideLookup<Real, WINDOW> lk{reinterpret_cast<const uint32_t*>(lds), T.srt, T.coef, thr,
                                        T.cdf, x_g, 24 - in.guide_log2, int(in.n_nodes),
                                        (sh.flags & kSafe) != 0};
-    for (int sgi = next_segment(&sh_next); sgi < P.segs; sgi = next_segment(&sh_next))
+    for (int sgi = next_segment(sh_next); sgi < P.segs; sgi = next_segment(sh_next))
       sample_segment<Real, WINDOW>(P, c, ch, in, sgi, lk, sh_kept);
   } else {
     build_tables<Real, SEQUENTIAL>(P, ch.instance, in, lds, ch.index == 0, sh);
@@ -1530,26 +1528,19 @@ __global__ void __launch_bounds__(1024) sample_kernel(BatchedParams P) {
     const GuideLookup<Real, WINDOW> lk{L.guide, L.srt, L.coef, thr, L.cdf, x_g,
                                        24 - in.guide_log2, int(in.n_nodes),
                                        (sh.flags & kSafe) != 0};
-    for (int sgi = next_segment(&sh_next); sgi < P.segs; sgi = next_segment(&sh_next))
+    for (int sgi = next_segment(sh_next); sgi < P.segs; sgi = next_segment(sh_next))
       sample_segment<Real, WINDOW>(P, c, ch, in, sgi, lk, sh_kept);
   }
   chunk_rows(P, c, sh_kept);
-  finalize_if_last(P);
 }
 
-// Sampling of the float32 cell-table instances (fast_tables.hpp): one
-// work-group per chunk (the first n_cell_chunks of fwd_order).  A kernel of
-// its own, so that neither path's registers limit the other's occupancy.
+// Sampling of one chunk of a float32 cell-table instance (fast_tables.hpp).
 template <bool WINDOW, bool SEQUENTIAL>
-__global__ void __launch_bounds__(1024) sample_cell_kernel(BatchedParams P) {
-  extern __shared__ __align__(16) uint8_t lds[];
-  __shared__ BuildShared sh;
-  __shared__ int sh_next;
-  __shared__ int64_t sh_kept[kWave];
-  const int c = P.fwd_order[blockIdx.x];
+__device__ __forceinline__ void cell_chunk(const BatchedParams& P, int c, uint8_t* lds,
+                                           BuildShared& sh, int* sh_next, int64_t* sh_kept) {
   const ChunkDesc ch = P.chunks[c];
   const InstanceDesc in = P.inst[ch.instance];
-  if (threadIdx.x == 0) sh_next = 0;
+  if (threadIdx.x == 0) *sh_next = 0;
   // the chunk's VJP partial sums are free during the forward: they hold this
   // group's copy of the CDF
   double* cdf_copy = P.partials + ch.partial_off;
@@ -1562,10 +1553,129 @@ __global__ void __launch_bounds__(1024) sample_cell_kernel(BatchedParams P) {
                               static_cast<const float*>(P.x) + in.node_off, 24 - in.fast_log2,
                               (sh.flags & kSafe) != 0,
                               WINDOW && (sh.flags & kMonotone) != 0};
-  for (int sgi = next_segment(&sh_next); sgi < P.segs; sgi = next_segment(&sh_next))
+  for (int sgi = next_segment(sh_next); sgi < P.segs; sgi = next_segment(sh_next))
     sample_segment<float, WINDOW>(P, c, ch, in, sgi, lk, sh_kept);
   chunk_rows(P, c, sh_kept);
+}
+
+// Sampling of the guide-table instances: one work-group per chunk (the chunks
+// after the cell-table ones in fwd_order).
+template <typename Real, bool WINDOW, bool SEQUENTIAL>
+__global__ void __launch_bounds__(1024) sample_kernel(BatchedParams P) {
+  extern __shared__ __align__(16) uint8_t lds[];
+  __shared__ BuildShared sh;
+  __shared__ int sh_next;  // next unclaimed segment (read after the barriers below)
+  __shared__ int64_t sh_kept[kWave];  // kept events per segment (at most 64 per chunk)
+  guide_chunk<Real, WINDOW, SEQUENTIAL>(P, P.fwd_order[P.n_cell_chunks + blockIdx.x], lds, sh,
+                                        &sh_next, sh_kept);
   finalize_if_last(P);
+}
+
+// Sampling of the float32 cell-table instances: one work-group per chunk (the
+// first n_cell_chunks of fwd_order).  A kernel of its own, so that neither
+// path's registers limit the other's occupancy.
+template <bool WINDOW, bool SEQUENTIAL>
+__global__ void __launch_bounds__(1024) sample_cell_kernel(BatchedParams P) {
+  extern __shared__ __align__(16) uint8_t lds[];
+  __shared__ BuildShared sh;
+  __shared__ int sh_next;
+  __shared__ int64_t sh_kept[kWave];
+  cell_chunk<WINDOW, SEQUENTIAL>(P, P.fwd_order[blockIdx.x], lds, sh, &sh_next, sh_kept);
+  finalize_if_last(P);
+}
+
+// Claim the next item of a persistent queue (counter flags[next]) for the work-group.
+__device__ __forceinline__ int claim_item(const BatchedParams& P, int next, int* sh_item) {
+  if (threadIdx.x == 0) *sh_item = atomicAdd(P.flags + next, 1);
+  __syncthreads();
+  return *sh_item;
+}
+
+// End of a persistent launch: the last work-group to leave returns the queue's
+// counters to zero (every other group has made its last claim by then).
+__device__ __forceinline__ void leave_queue(const BatchedParams& P, int next, int exited) {
+  if (threadIdx.x == 0) {
+    const int old = atomicAdd(P.flags + exited, 1);
+    if (old == int(gridDim.x) - 1) {
+      atomicExch(P.flags + next, 0);
+      atomicExch(P.flags + exited, 0);
+    }
+  }
+}
+
+// Items of the persistent launches, each compiled as a function of its own
+// (noinline) so that every kind keeps the register allocation it has in its
+// own kernel; the work-group's dynamic LDS and shared words are declared here.
+// A callee's arguments arrive in vector registers, so the kernel passes the
+// address of its kernel-argument segment (BatchedParams first), which each
+// callee makes wave-uniform and reads through the constant address space, and
+// the item index, made wave-uniform too: both stay scalar.  (The segment
+// pointer itself is not available inside a callee.)
+using KernargPtr = const __attribute__((address_space(4))) uint8_t*;
+
+__device__ __forceinline__ uint64_t kernarg_address() {
+  return reinterpret_cast<uint64_t>(__builtin_amdgcn_kernarg_segment_ptr());
+}
+
+__device__ __forceinline__ KernargPtr uniform_kernarg(uint64_t a) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane(uint32_t(a));
+  const uint32_t hi = __builtin_amdgcn_readfirstlane(uint32_t(a >> 32));
+  return (KernargPtr)((uint64_t(hi) << 32) | lo);
+}
+
+__device__ __forceinline__ const BatchedParams& launch_params(uint64_t kargs) {
+  return *(const BatchedParams*)(const __attribute__((address_space(4))) BatchedParams*)
+      uniform_kernarg(kargs);
+}
+
+template <bool WINDOW, bool SEQUENTIAL>
+__device__ __noinline__ void persistent_cell_chunk(uint64_t kargs, int c_arg) {
+  const BatchedParams& P = launch_params(kargs);
+  const int c = __builtin_amdgcn_readfirstlane(c_arg);
+  extern __shared__ __align__(16) uint8_t lds[];
+  __shared__ BuildShared sh;
+  __shared__ int sh_next;
+  __shared__ int64_t sh_kept[kWave];
+  cell_chunk<WINDOW, SEQUENTIAL>(P, c, lds, sh, &sh_next, sh_kept);
+}
+
+template <typename Real, bool WINDOW, bool SEQUENTIAL>
+__device__ __noinline__ void persistent_guide_chunk(uint64_t kargs, int c_arg) {
+  const BatchedParams& P = launch_params(kargs);
+  const int c = __builtin_amdgcn_readfirstlane(c_arg);
+  extern __shared__ __align__(16) uint8_t lds[];
+  __shared__ BuildShared sh;
+  __shared__ int sh_next;
+  __shared__ int64_t sh_kept[kWave];
+  guide_chunk<Real, WINDOW, SEQUENTIAL>(P, c, lds, sh, &sh_next, sh_kept);
+}
+
+// Persistent forward: resident work-groups take the chunks of fwd_queue,
+// largest first, cell or guide tables alike.  The global-table instances are
+// built by tables_kernel beforehand: a build inside the launch would hand its
+// tables to work-groups on other XCDs, whose L2s are not coherent with the
+// builder's within a launch, and the agent-scope release and acquire fences
+// that takes (an L2 write-back per build, an invalidate per waiting chunk) cost
+// more than the separate launch.  The last chunk to finish finalizes.
+template <typename Real, bool WINDOW, bool SEQUENTIAL>
+__global__ void __launch_bounds__(1024) forward_persistent_kernel(BatchedParams P) {
+  __shared__ int sh_item;
+  for (;;) {
+    const int item = claim_item(P, kFlagFwdNext, &sh_item);
+    if (item >= P.n_fwd_items) break;
+    const int c = P.fwd_queue[item];
+    bool cell = false;
+    if constexpr (sizeof(Real) == 4) cell = P.inst[P.chunks[c].instance].fast_log2 > 0;
+    if (cell) {
+      if constexpr (sizeof(Real) == 4)
+        persistent_cell_chunk<WINDOW, SEQUENTIAL>(kernarg_address(), c);
+    } else {
+      persistent_guide_chunk<Real, WINDOW, SEQUENTIAL>(kernarg_address(), c);
+    }
+    finalize_if_last(P);
+    __syncthreads();  // the item's LDS and shared words are free
+  }
+  leave_queue(P, kFlagFwdNext, kFlagFwdExit);
 }
 
 // Pack: one work-group per chunk, segments round-robin over its waves; a no-op when dense.
@@ -2132,7 +2242,7 @@ __device__ __forceinline__ void vjp_body(const BatchedParams& P, const Real* gra
 // VJP of a float32 curve with cell tables (fast_tables.hpp): one work-group per
 // chunk.  The work-group stages the cell words and sorted splits, regenerates
 // its keys and finds their bins as the forward did, and sums per bin
-//   a0 = sum q,  a1 = sum q k,   q = round(g 2^e), |q| < 2^kFastQBits
+//   a0 = sum q,  a1 = sum q k,   q = round(g 2^e), |q| < 2^fixed_point_bits(keys)
 // as 64-bit integers (exact, order-independent): with the bin's own threshold
 // T_j, sum g (u - C_j) = 2^-24 sum g (k - T_j) + (T_j 2^-24 - C_j) sum g, and
 // sum q (k - T_j) = a1 - T_j a0 exactly.  2^e is set from the largest |g| the
@@ -2151,6 +2261,7 @@ __device__ void fast_vjp_body(const BatchedParams& P, const float* grad, float* 
   const InstanceDesc in = P.inst[ch.instance];
   const int n = static_cast<int>(in.n_nodes);
   const int g = in.fast_log2, s = 24 - g;
+  const int qbits = fixed_point_bits(ch.slot_hi - ch.slot_lo);
   const int64_t G = int64_t(1) << g;
   const uint8_t* blob = P.tables + in.table_off;
   const TableHeader* hdr = reinterpret_cast<const TableHeader*>(blob);
@@ -2207,12 +2318,12 @@ __device__ void fast_vjp_body(const BatchedParams& P, const float* grad, float* 
     if (ex == 255) {
       scale = __builtin_nanf("");  // inf / NaN cotangents: every value takes the float64 sums
     } else {
-      scale_log2 = (kFastQBits - 2) - (ex - 126);
+      scale_log2 = (qbits - 2) - (ex - 126);
       scale_log2 = scale_log2 > 126 ? 126 : scale_log2;
       scale = __builtin_ldexpf(1.0f, scale_log2);
     }
   }
-  constexpr float kQLimit = float(1 << kFastQBits);
+  const float kQLimit = float(1 << qbits);
 
   for (int w = next_segment(&sh_next); w < nw; w = next_segment(&sh_next)) {
     const Segment sg = segment_of(in.counter_base, ch.slot_lo, ch.slot_hi, w, nw);
@@ -2486,6 +2597,7 @@ __device__ void vjp_range_body(const BatchedParams& P, const Real* grad, Real* g
   Tables<Real> T = tables_of<Real>(P.tables, in);
   const int n = static_cast<int>(in.n_nodes);
   const int R = in.vjp_ranges;
+  const int qbits = fixed_point_bits(ch.slot_hi - ch.slot_lo);
   const int W = vjp_range_width(n, R);
   const int lo = q * W;
   const int hi = lo + W < n - 1 ? lo + W : n - 1;
@@ -2580,13 +2692,13 @@ __device__ void vjp_range_body(const BatchedParams& P, const Real* grad, Real* g
     const int ex = int(gm >> 23);
     if (ex == 255) {
       scale = __builtin_nanf("");  // inf / NaN cotangents: every value takes the float64 sums
-    } else {  // |round(g 2^e)| < 2^kFastQBits, as in the cell-table VJP
-      scale_log2 = (kFastQBits - 2) - (ex - 126);
+    } else {  // |round(g 2^e)| < 2^qbits, as in the cell-table VJP
+      scale_log2 = (qbits - 2) - (ex - 126);
       scale_log2 = scale_log2 > 126 ? 126 : scale_log2;
       scale = __builtin_ldexpf(1.0f, scale_log2);
     }
   }
-  constexpr float kQLimit = float(1 << kFastQBits);
+  const float kQLimit = float(1 << qbits);
   const int jmax = n - 2 - lo;  // relative clamp (only the last range reaches it)
   const double* s_c = reinterpret_cast<const double*>(bd);
 
@@ -2835,8 +2947,129 @@ __global__ void __launch_bounds__(1024) vjp_range_kernel(BatchedParams P, const 
     vjp_body<Real, true>(P, grad, grad_density, lds, item.x);
 }
 
+// Persistent VJP: resident work-groups take items from vjp_queue, largest
+// first, whatever their kind (cell tables, guide tables, bin range).
+// The persistent VJP's arguments after BatchedParams.
+template <typename Real>
+struct VjpArgs {
+  BatchedParams P;
+  const Real* grad;
+  Real* grad_density;
+};
+
+template <typename Real>
+__device__ __forceinline__ const VjpArgs<Real>& vjp_launch_args(uint64_t kargs) {
+  return *(const VjpArgs<Real>*)(const __attribute__((address_space(4))) VjpArgs<Real>*)
+      uniform_kernarg(kargs);
+}
+
+template <typename Real>
+__device__ __noinline__ void persistent_vjp_cell(uint64_t kargs, int c_arg) {
+  extern __shared__ __align__(16) uint8_t lds[];
+  const VjpArgs<Real>& A = vjp_launch_args<Real>(kargs);
+  const int c = __builtin_amdgcn_readfirstlane(c_arg);
+  if constexpr (sizeof(Real) == 4) fast_vjp_body(A.P, A.grad, A.grad_density, lds, c);
+}
+
+template <typename Real, bool GLOBAL>
+__device__ __noinline__ void persistent_vjp_guide(uint64_t kargs, int c_arg) {
+  extern __shared__ __align__(16) uint8_t lds[];
+  const VjpArgs<Real>& A = vjp_launch_args<Real>(kargs);
+  const int c = __builtin_amdgcn_readfirstlane(c_arg);
+  vjp_body<Real, GLOBAL>(A.P, A.grad, A.grad_density, lds, c);
+}
+
+template <typename Real>
+__device__ __noinline__ void persistent_vjp_range(uint64_t kargs, int c_arg, int q_arg) {
+  extern __shared__ __align__(16) uint8_t lds[];
+  const VjpArgs<Real>& A = vjp_launch_args<Real>(kargs);
+  const int c = __builtin_amdgcn_readfirstlane(c_arg);
+  const int q = __builtin_amdgcn_readfirstlane(q_arg);
+  vjp_range_body<Real>(A.P, A.grad, A.grad_density, lds, c, q);
+}
+
+template <typename Real>
+__global__ void __launch_bounds__(1024) vjp_persistent_kernel(BatchedParams P, const Real* grad,
+                                                              Real* grad_density) {
+  static_assert(offsetof(VjpArgs<Real>, grad) == sizeof(BatchedParams) &&
+                    offsetof(VjpArgs<Real>, grad_density) == sizeof(BatchedParams) + sizeof(void*),
+                "the kernel-argument segment holds P, grad, grad_density in this order");
+  __shared__ int sh_item;
+  for (;;) {
+    const int item = claim_item(P, kFlagVjpNext, &sh_item);
+    if (item >= P.n_vjp_items) break;
+    const int idx = P.vjp_queue[item];
+    const int2 it = P.vjp_items[idx];
+    const InstanceDesc& in = P.inst[P.chunks[it.x].instance];
+    if (idx < P.n_vjp_cell) {
+      persistent_vjp_cell<Real>(kernarg_address(), it.x);
+    } else if (idx < P.n_vjp_plain) {
+      if (in.global_tables)
+        persistent_vjp_guide<Real, true>(kernarg_address(), it.x);
+      else
+        persistent_vjp_guide<Real, false>(kernarg_address(), it.x);
+    } else {
+      const TableHeader* hdr = reinterpret_cast<const TableHeader*>(P.tables + in.table_off);
+      if (hdr->flags & kMonotone)
+        persistent_vjp_range<Real>(kernarg_address(), it.x, it.y);
+      else if (it.y == 0)
+        persistent_vjp_guide<Real, true>(kernarg_address(), it.x);
+    }
+    __syncthreads();  // the item's LDS and shared words are free
+  }
+  leave_queue(P, kFlagVjpNext, kFlagVjpExit);
+}
+
 // ---------------------------------------------------------------------------
 // Host-side launchers (called from capi.hip).
+
+// Work-groups of `kern` that fit the current device at once with this shape.
+template <typename Kernel>
+int resident_groups(Kernel kern, int threads, int64_t lds) {
+  int dev = 0, cus = 0, per = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    return 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, threads, size_t(lds)) !=
+      hipSuccess)
+    return 0;
+  return per * cus;
+}
+
+template <typename Real>
+hipError_t launch_forward_persistent(const BatchedParams& P, int threads, bool sequential,
+                                     int64_t lds, hipStream_t stream) {
+  if (P.n_fwd_items == 0) return hipSuccess;
+  auto kern = P.use_window
+                  ? (sequential ? forward_persistent_kernel<Real, true, true>
+                                : forward_persistent_kernel<Real, true, false>)
+                  : (sequential ? forward_persistent_kernel<Real, false, true>
+                                : forward_persistent_kernel<Real, false, false>);
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+  if (e != hipSuccess) return e;
+  const int fit = resident_groups(kern, threads, lds);
+  if (fit <= 0) return hipErrorInvalidConfiguration;
+  const int64_t grid = std::min<int64_t>(fit, P.n_fwd_items);
+  hipLaunchKernelGGL(kern, dim3(unsigned(grid)), dim3(threads), size_t(lds), stream, P);
+  return hipGetLastError();
+}
+
+template <typename Real>
+hipError_t launch_vjp_persistent(const BatchedParams& P, int threads, int64_t lds,
+                                 const void* grad, void* grad_density, hipStream_t stream) {
+  if (P.n_vjp_items == 0) return hipSuccess;
+  auto kern = vjp_persistent_kernel<Real>;
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+  if (e != hipSuccess) return e;
+  const int fit = resident_groups(kern, threads, lds);
+  if (fit <= 0) return hipErrorInvalidConfiguration;
+  const int64_t grid = std::min<int64_t>(fit, P.n_vjp_items);
+  hipLaunchKernelGGL(kern, dim3(unsigned(grid)), dim3(threads), size_t(lds), stream, P,
+                     static_cast<const Real*>(grad), static_cast<Real*>(grad_density));
+  return hipGetLastError();
+}
 template <typename Real>
 hipError_t launch_tables(const BatchedParams& P, int threads, bool sequential, int64_t lds,
                          bool only_global, hipStream_t stream) {
@@ -2935,6 +3168,14 @@ template hipError_t launch_pack<float>(const BatchedParams&, int, void*, int32_t
 template hipError_t launch_pack<double>(const BatchedParams&, int, void*, int32_t*, hipStream_t);
 template hipError_t launch_vjp<float>(const BatchedParams&, int, int64_t, const void*, void*,
                                       hipStream_t);
+template hipError_t launch_forward_persistent<float>(const BatchedParams&, int, bool, int64_t,
+                                                     hipStream_t);
+template hipError_t launch_forward_persistent<double>(const BatchedParams&, int, bool, int64_t,
+                                                      hipStream_t);
+template hipError_t launch_vjp_persistent<float>(const BatchedParams&, int, int64_t, const void*,
+                                                 void*, hipStream_t);
+template hipError_t launch_vjp_persistent<double>(const BatchedParams&, int, int64_t, const void*,
+                                                  void*, hipStream_t);
 template hipError_t launch_vjp<double>(const BatchedParams&, int, int64_t, const void*, void*,
                                        hipStream_t);
 

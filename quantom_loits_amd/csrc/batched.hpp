@@ -15,6 +15,9 @@ struct BatchedParams {
   const int2* vjp_items;  // VJP work items {chunk, bin range}: cell-table items, other plain
                           // items, then ranged
   const int32_t* fwd_order;  // chunks of the cell-table instances first, then the others
+  const int32_t* fwd_queue;  // persistent forward: chunks, largest first
+  const int32_t* vjp_queue;  // persistent VJP items: indices into vjp_items
+  int64_t n_fwd_items;
   int64_t n_instances, n_chunks, n_vjp_items, n_vjp_plain;
   int64_t n_cell_chunks, n_vjp_cell;
   const void* x;
@@ -45,6 +48,11 @@ struct BatchedParams {
   double* partials;      // per-chunk partial VJP sums
 };
 
+// Scratch flag words (P.flags, 16 int32): [0] dense, [1] finished chunks of the
+// forward, [4]/[5] next item / exited work-groups of the persistent forward,
+// [6]/[7] the same for the persistent VJP.  Every counter returns to zero.
+constexpr int kFlagFwdNext = 4, kFlagFwdExit = 5, kFlagVjpNext = 6, kFlagVjpExit = 7;
+
 template <typename Real>
 hipError_t launch_tables(const BatchedParams& P, int threads, bool sequential, int64_t lds,
                          bool only_global, hipStream_t stream);
@@ -58,5 +66,13 @@ hipError_t launch_pack(const BatchedParams& P, int threads, void* packed, int32_
 template <typename Real>
 hipError_t launch_vjp(const BatchedParams& P, int threads, int64_t lds, const void* grad,
                       void* grad_density, hipStream_t stream);
+// One persistent launch each: as many work-groups as fit the device at once
+// (at most one per item), taking items from the queues in P.
+template <typename Real>
+hipError_t launch_forward_persistent(const BatchedParams& P, int threads, bool sequential,
+                                     int64_t lds, hipStream_t stream);
+template <typename Real>
+hipError_t launch_vjp_persistent(const BatchedParams& P, int threads, int64_t lds,
+                                 const void* grad, void* grad_density, hipStream_t stream);
 
 }  // namespace qloits

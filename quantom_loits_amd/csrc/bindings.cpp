@@ -571,8 +571,11 @@ class BatchedPlan {
  public:
   // instances: CPU int64 [n, 8] = node_off, n_nodes, events, out_off,
   // counter_base, stream, accept, 0 (counter/stream as two's-complement int64)
+  // bucket_chunk_log2 / bucket_guide_shift: empty, or 16 per-bucket overrides
+  // (loits_hip_tuning; 0 keeps the plan-wide value)
   BatchedPlan(at::Tensor instances, int64_t dtype, int64_t threads, int64_t guide_shift,
-              int64_t chunk_events, int64_t scan_mode, int64_t persistent, int64_t table_format) {
+              int64_t chunk_events, int64_t scan_mode, int64_t persistent, int64_t table_format,
+              std::vector<int64_t> bucket_chunk_log2, std::vector<int64_t> bucket_guide_shift) {
     instances = instances.contiguous();
     if (instances.device().is_cuda() || instances.scalar_type() != at::kLong ||
         instances.dim() != 2 || instances.size(1) != 8)
@@ -597,6 +600,16 @@ class BatchedPlan {
     tuning_.scan_mode = int32_t(scan_mode);
     tuning_.persistent = int32_t(persistent);
     tuning_.table_format = int32_t(table_format);
+    tuning_.reserved = 0;
+    for (auto* v : {&bucket_chunk_log2, &bucket_guide_shift})
+      if (!v->empty() && v->size() != 16)
+        throw std::invalid_argument("bucket overrides must have 16 entries (or none)");
+    for (int b = 0; b < 16; ++b) {
+      tuning_.bucket_chunk_log2[b] =
+          bucket_chunk_log2.empty() ? 0 : int32_t(bucket_chunk_log2[size_t(b)]);
+      tuning_.bucket_guide_shift[b] =
+          bucket_guide_shift.empty() ? 0 : int32_t(bucket_guide_shift[size_t(b)]);
+    }
     check(loits_hip_batched_layout(inst_.data(), n, int(dtype), &tuning_, &layout_),
           "loits_hip_batched_layout");
     desc_host_ = at::empty({layout_.desc_bytes}, at::TensorOptions().dtype(at::kByte));
@@ -628,6 +641,9 @@ class BatchedPlan {
     d["dtype"] = layout_.dtype;
     d["threads"] = layout_.threads;
     d["scan_mode"] = layout_.scan_mode;
+    d["persistent"] = layout_.persistent;
+    d["forward_items"] = layout_.forward_items;
+    d["lds_persistent_forward"] = layout_.lds_persistent_forward;
     return d;
   }
 
@@ -764,7 +780,12 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
         py::arg("row_start"), py::arg("lo"), py::arg("hi"), py::arg("bins"), py::arg("width"),
         py::arg("grad_hist"), py::arg("stream") = py::none());
   py::class_<BatchedPlan>(m, "BatchedPlan")
-      .def(py::init<at::Tensor, int64_t, int64_t, int64_t, int64_t, int64_t, int64_t, int64_t>())
+      .def(py::init<at::Tensor, int64_t, int64_t, int64_t, int64_t, int64_t, int64_t, int64_t,
+                    std::vector<int64_t>, std::vector<int64_t>>(),
+           py::arg("instances"), py::arg("dtype"), py::arg("threads"), py::arg("guide_shift"),
+           py::arg("chunk_events"), py::arg("scan_mode"), py::arg("persistent"),
+           py::arg("table_format"), py::arg("bucket_chunk_log2") = std::vector<int64_t>(),
+           py::arg("bucket_guide_shift") = std::vector<int64_t>())
       .def("layout", &BatchedPlan::layout)
       .def("descriptors", &BatchedPlan::descriptors)
       .def("descriptors_for_streams", &BatchedPlan::descriptors_for_streams)

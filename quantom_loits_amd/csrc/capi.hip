@@ -86,6 +86,8 @@ struct Plan {
   std::vector<VjpItem> items;  // VJP work items: one per chunk (cell-table instances first),
                                // then one per (chunk, range)
   std::vector<int32_t> fwd_order;  // chunks of the cell-table instances first
+  std::vector<int32_t> fwd_queue;  // persistent forward: chunks, largest first
+  std::vector<int32_t> vjp_queue;  // persistent VJP: indices into items
   int64_t n_plain = 0;
   loits_hip_layout layout;
 };
@@ -101,8 +103,16 @@ int make_plan(const loits_hip_instance* instances, int64_t n, int dtype,
     return fail(LOITS_HIP_EINVAL, "batched: threads must be 256, 512 or 1024");
   if (tuning->guide_shift < 0 || tuning->guide_shift > 3)
     return fail(LOITS_HIP_EINVAL, "batched: guide_shift must be in [0, 3]");
-  if (tuning->persistent != 0)
-    return fail(LOITS_HIP_EINVAL, "batched: persistent is reserved and must be 0");
+  if (tuning->persistent != 0 && tuning->persistent != 1)
+    return fail(LOITS_HIP_EINVAL, "batched: persistent must be 0 or 1");
+  if (tuning->reserved != 0) return fail(LOITS_HIP_EINVAL, "batched: tuning.reserved must be 0");
+  for (int b = 0; b < 16; ++b) {
+    const int cl = tuning->bucket_chunk_log2[b], gs = tuning->bucket_guide_shift[b];
+    if (cl != 0 && (cl < 10 || cl > 30))
+      return fail(LOITS_HIP_EINVAL, "batched: bucket_chunk_log2 entries must be 0 or in [10, 30]");
+    if (gs < 0 || gs > 4)
+      return fail(LOITS_HIP_EINVAL, "batched: bucket_guide_shift entries must be in [0, 4]");
+  }
   if (tuning->table_format != 0 && tuning->table_format != 1)
     return fail(LOITS_HIP_EINVAL, "batched: table_format must be 0 or 1");
   const int64_t per_iter = int64_t(4) * threads;
@@ -130,6 +140,8 @@ int make_plan(const loits_hip_instance* instances, int64_t n, int dtype,
   plan->chunks.clear();
   plan->items.clear();
   plan->fwd_order.clear();
+  plan->fwd_queue.clear();
+  plan->vjp_queue.clear();
   for (int64_t i = 0; i < n; ++i) {
     const loits_hip_instance& s = instances[i];
     if (s.n_nodes < 2 || s.n_nodes > 32767)
@@ -143,6 +155,15 @@ int make_plan(const loits_hip_instance* instances, int64_t n, int dtype,
                                         " out_off must equal the events of the instances before it");
     if (s.counter_base > ~uint64_t(0) - uint64_t(s.events))
       return fail(LOITS_HIP_EINVAL, "batched: Philox counter range overflows 64 bits");
+    // this curve's grid-size bucket may override the chunk size and guide shift
+    const int bucket = std::min(15, ceil_log2(s.n_nodes));
+    int64_t chunk_i = chunk_events;
+    if (tuning->bucket_chunk_log2[bucket] != 0)
+      chunk_i = (((int64_t(1) << tuning->bucket_chunk_log2[bucket]) + per_iter - 1) / per_iter) *
+                per_iter;
+    const int shift_i = tuning->bucket_guide_shift[bucket] != 0
+                            ? tuning->bucket_guide_shift[bucket] - 1
+                            : tuning->guide_shift;
     InstanceDesc& d = plan->inst[size_t(i)];
     d.node_off = s.node_off;
     d.n_nodes = s.n_nodes;
@@ -154,7 +175,7 @@ int make_plan(const loits_hip_instance* instances, int64_t n, int dtype,
     // float32 curves of at most kFastMaxNodes nodes take the cell tables
     // (fast_tables.hpp) when both kernels' LDS images fit
     if (tuning->table_format == 0 && dtype == LOITS_HIP_F32 && s.n_nodes <= kFastMaxNodes &&
-        chunk_events <= kFastMaxChunk) {
+        chunk_i <= kFastMaxChunk) {
       int g = fast_guide_log2(s.n_nodes);
       while (g > kMinGuideLog2 && std::max(fast_lds_forward(s.n_nodes, g).bytes,
                                            fast_lds_vjp(s.n_nodes, g).bytes) > kMaxLds)
@@ -177,7 +198,7 @@ int make_plan(const loits_hip_instance* instances, int64_t n, int dtype,
       // and only the guide is staged in LDS.  VJP: tables and histograms in LDS
       // at the finest guide that fits both kernels; otherwise bin ranges.
       const int g_want = std::min(kMaxGuideLog2,
-                                  std::max(kMinGuideLog2, ceil_log2(s.n_nodes) + tuning->guide_shift));
+                                  std::max(kMinGuideLog2, ceil_log2(s.n_nodes) + shift_i));
       auto build_bytes = [&](int g) { return lds_build_layout(s.n_nodes, g, coef_size).bytes; };
       auto vjp_bytes = [&](int g) {
         return lds_table_bytes(s.n_nodes, g, coef_size) + pad4(s.n_nodes) * 16;
@@ -221,8 +242,8 @@ int make_plan(const loits_hip_instance* instances, int64_t n, int dtype,
     // float32 bin-range VJPs sum q k as 64-bit integers: at most kFastMaxChunk
     // keys per chunk keep the sums below 2^62
     const int64_t ce = (dtype == LOITS_HIP_F32 && d.vjp_ranges > 0)
-                           ? std::min<int64_t>(chunk_events, kFastMaxChunk)
-                           : chunk_events;
+                           ? std::min<int64_t>(chunk_i, kFastMaxChunk)
+                           : chunk_i;
     const int64_t nch = std::max<int64_t>(1, (s.events + ce - 1) / ce);
     if (plan->chunks.size() + size_t(nch) > size_t(INT32_MAX))
       return fail(LOITS_HIP_EUNSUPPORTED, "batched: too many tiles");
@@ -262,12 +283,11 @@ int make_plan(const loits_hip_instance* instances, int64_t n, int dtype,
   // short ones fill the end instead of one long chunk running alone.  Cost in
   // key units: the chunk's keys plus a table build or staging of kBuildKeysPerNode
   // keys per node.  Stable, so equal costs keep instance order.
+  auto cost = [&](int32_t ci) {
+    const ChunkDesc& ch = plan->chunks[size_t(ci)];
+    return (ch.slot_hi - ch.slot_lo) + kBuildKeysPerNode * plan->inst[size_t(ch.instance)].n_nodes;
+  };
   {
-    auto cost = [&](int32_t ci) {
-      const ChunkDesc& ch = plan->chunks[size_t(ci)];
-      return (ch.slot_hi - ch.slot_lo) +
-             kBuildKeysPerNode * plan->inst[size_t(ch.instance)].n_nodes;
-    };
     auto by_cost = [&](int32_t a, int32_t b) { return cost(a) > cost(b); };
     auto item_by_cost = [&](const VjpItem& a, const VjpItem& b) {
       return cost(a.chunk) > cost(b.chunk);
@@ -288,14 +308,31 @@ int make_plan(const loits_hip_instance* instances, int64_t n, int dtype,
   plan->items.insert(plan->items.end(), ranged.begin(), ranged.end());
   if (plan->items.size() > size_t(INT32_MAX))
     return fail(LOITS_HIP_EUNSUPPORTED, "batched: too many VJP work items");
+  // Persistent queues: every forward chunk and every VJP item, largest first.
+  if (L.persistent) {
+    plan->fwd_queue = plan->fwd_order;
+    std::stable_sort(plan->fwd_queue.begin(), plan->fwd_queue.end(),
+                     [&](int32_t a, int32_t b) { return cost(a) > cost(b); });
+    plan->vjp_queue.resize(plan->items.size());
+    for (size_t k = 0; k < plan->items.size(); ++k) plan->vjp_queue[k] = int32_t(k);
+    std::stable_sort(plan->vjp_queue.begin(), plan->vjp_queue.end(), [&](int32_t a, int32_t b) {
+      return cost(plan->items[size_t(a)].chunk) > cost(plan->items[size_t(b)].chunk);
+    });
+  }
+  L.forward_items = int64_t(plan->fwd_queue.size());
   L.desc_bytes = align16(n * int64_t(sizeof(InstanceDesc))) +
                  align16(L.n_chunks * int64_t(sizeof(ChunkDesc))) +
                  align16(int64_t(plan->items.size()) * int64_t(sizeof(VjpItem))) +
-                 L.n_chunks * int64_t(sizeof(int32_t));
+                 align16(L.n_chunks * int64_t(sizeof(int32_t))) +
+                 align16(L.forward_items * int64_t(sizeof(int32_t))) +
+                 int64_t(plan->vjp_queue.size()) * int64_t(sizeof(int32_t));
   L.table_bytes = std::max<int64_t>(table_off, 16);
   L.lds_forward = lds_f;
   L.lds_backward = lds_b;
   L.lds_tables = lds_t;
+  // the persistent forward samples every kind of chunk (the global-table builds
+  // stay a launch of their own)
+  L.lds_persistent_forward = lds_f;
   L.n_segments = L.n_chunks * nsegs;
   if (lds_f > kMaxLds || lds_t > kMaxLds)
     return fail(LOITS_HIP_EUNSUPPORTED,
@@ -328,8 +365,13 @@ BatchedParams params_of(const loits_hip_batched_args* a) {
   P.chunks = reinterpret_cast<const ChunkDesc*>(desc + chunks_at);
   const int64_t items_at = chunks_at + align16(L.n_chunks * int64_t(sizeof(ChunkDesc)));
   P.vjp_items = reinterpret_cast<const int2*>(desc + items_at);
-  P.fwd_order = reinterpret_cast<const int32_t*>(
-      desc + items_at + align16(int64_t(L.grid_backward) * int64_t(sizeof(VjpItem))));
+  const int64_t order_at = items_at + align16(int64_t(L.grid_backward) * int64_t(sizeof(VjpItem)));
+  P.fwd_order = reinterpret_cast<const int32_t*>(desc + order_at);
+  const int64_t fq_at = order_at + align16(L.n_chunks * int64_t(sizeof(int32_t)));
+  P.fwd_queue = reinterpret_cast<const int32_t*>(desc + fq_at);
+  P.vjp_queue = reinterpret_cast<const int32_t*>(
+      desc + fq_at + align16(L.forward_items * int64_t(sizeof(int32_t))));
+  P.n_fwd_items = L.forward_items;
   P.n_instances = L.n_instances;
   P.n_chunks = L.n_chunks;
   P.n_vjp_items = L.grid_backward;
@@ -469,8 +511,14 @@ int loits_hip_batched_encode(const loits_hip_instance* instances, int64_t n,
     const int64_t items_at =
         chunks_at + align16(int64_t(plan.chunks.size() * sizeof(ChunkDesc)));
     std::memcpy(out + items_at, plan.items.data(), plan.items.size() * sizeof(VjpItem));
-    std::memcpy(out + items_at + align16(int64_t(plan.items.size() * sizeof(VjpItem))),
-                plan.fwd_order.data(), plan.fwd_order.size() * sizeof(int32_t));
+    const int64_t order_at = items_at + align16(int64_t(plan.items.size() * sizeof(VjpItem)));
+    std::memcpy(out + order_at, plan.fwd_order.data(), plan.fwd_order.size() * sizeof(int32_t));
+    const int64_t fq_at = order_at + align16(int64_t(plan.fwd_order.size() * sizeof(int32_t)));
+    if (!plan.fwd_queue.empty())
+      std::memcpy(out + fq_at, plan.fwd_queue.data(), plan.fwd_queue.size() * sizeof(int32_t));
+    if (!plan.vjp_queue.empty())
+      std::memcpy(out + fq_at + align16(int64_t(plan.fwd_queue.size() * sizeof(int32_t))),
+                  plan.vjp_queue.data(), plan.vjp_queue.size() * sizeof(int32_t));
     return LOITS_HIP_OK;
   } catch (const std::exception& e) {
     return fail(LOITS_HIP_ERUNTIME, std::string("batched_encode: ") + e.what());
@@ -493,6 +541,13 @@ int loits_hip_batched_forward(const loits_hip_batched_args* args, void* stream) 
               : launch_tables<double>(P, tthreads, seq, L.lds_tables, true, s),
           "batched tables launch");
       if (rt != LOITS_HIP_OK) return rt;
+    }
+    if (L.persistent) {  // one launch for every chunk, from the queue
+      if (L.n_chunks == 0) return check_hip(launch_finalize(P, s), "batched finalize launch");
+      return check_hip(
+          f32 ? launch_forward_persistent<float>(P, L.threads, seq, L.lds_persistent_forward, s)
+              : launch_forward_persistent<double>(P, L.threads, seq, L.lds_persistent_forward, s),
+          "batched persistent forward launch");
     }
     int rc = check_hip(f32 ? launch_sample<float>(P, L.threads, seq, L.lds_forward, s)
                            : launch_sample<double>(P, L.threads, seq, L.lds_forward, s),
@@ -549,6 +604,12 @@ int loits_hip_batched_backward(const loits_hip_batched_args* args, const void* g
                                "batched tables launch");
       if (rc != LOITS_HIP_OK) return rc;
     }
+    if (L.persistent)
+      return check_hip(f32 ? launch_vjp_persistent<float>(P, L.threads, L.lds_backward,
+                                                          grad_events, grad_density, s)
+                           : launch_vjp_persistent<double>(P, L.threads, L.lds_backward,
+                                                           grad_events, grad_density, s),
+                       "batched persistent vjp launch");
     return check_hip(f32 ? launch_vjp<float>(P, L.threads, L.lds_backward, grad_events,
                                              grad_density, s)
                          : launch_vjp<double>(P, L.threads, L.lds_backward, grad_events,

@@ -6,11 +6,9 @@
 #include <cstdint>
 #include <string>
 
-namespace qloits {
+#include "loits_device.hpp"  // kMaxLds
 
-// LDS a single work-group may use on gfx950 (160 KiB per CU), less the
-// kernels' static LDS.
-constexpr int64_t kMaxLds = 160 * 1024 - 2048;
+namespace qloits {
 
 void set_error(const std::string& msg);
 int fail(int status, const std::string& msg);

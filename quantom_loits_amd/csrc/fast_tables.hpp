@@ -49,9 +49,16 @@ constexpr uint32_t kCellSearch = 1u << 13;
 constexpr uint32_t kCellField = 0xFFFu;
 constexpr uint32_t kNoCell = 0xFFFFFFFFu;  // empty slot of the crowded-cell list
 constexpr uint32_t kFastTables = 4u;       // TableHeader flag: tables in the cell format
-// VJP fixed point: |round(g 2^e)| < 2^kFastQBits, so that sum q k over a chunk
-// of at most 2^18 keys below 2^24 stays below 2^62.
-constexpr int kFastQBits = 20;
+// VJP fixed point: |round(g 2^e)| < 2^bits with keys * 2^bits * 2^24 <= 2^62, so
+// that sum q and sum q k over a chunk of `keys` keys below 2^24 cannot wrap: 20
+// bits at 2^18 keys, up to 28 for chunks of at most 2^10 keys (the largest
+// float32 cotangents of the chunk are then exact and the rest keep 26 bits
+// below the largest).
+__host__ __device__ __forceinline__ int fixed_point_bits(int64_t keys) {
+  int l = 0;
+  while ((int64_t(1) << l) < keys) ++l;
+  return l >= 10 ? 38 - l : 28;
+}
 
 __host__ __device__ __forceinline__ int fast_l2_cap(int64_t n) { return n <= 2048 ? 4096 : 2048; }
 

@@ -333,6 +333,10 @@ constexpr uint32_t kSafe = 2u;
 //     2^f level-2 words (direct or search) covering S / 2^f keys each, f in
 //     bits 24-27.  A crowded cell with c thresholds gets 2^f = min(S, pow2(2c))
 //     sub-cells while the G level-2 words last.
+// LDS a single work-group may use on gfx950 (160 KiB per CU), less the
+// kernels' static LDS.
+constexpr int64_t kMaxLds = 160 * 1024 - 2048;
+
 // Cells are at most 2^12 keys wide (guide_log2 >= kMinGuideLog2).
 constexpr int kMinGuideLog2 = 12;
 constexpr int kMaxGuideLog2 = 14;

@@ -43,7 +43,8 @@ class Instance(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [("threads", C.c_int32), ("guide_shift", C.c_int32), ("chunk_events", C.c_int64),
                 ("scan_mode", C.c_int32), ("persistent", C.c_int32), ("table_format", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("reserved", C.c_int32), ("bucket_chunk_log2", C.c_int32 * 16),
+                ("bucket_guide_shift", C.c_int32 * 16)]
 
 
 class Layout(C.Structure):
@@ -53,7 +54,8 @@ class Layout(C.Structure):
         "mask_words", "partial_doubles")] + [(n, C.c_int32) for n in (
             "dtype", "threads", "scan_mode", "persistent", "grid_forward", "grid_backward",
             "global_instances", "range_instances", "cell_instances", "reserved")] + \
-        [("backward_plain", C.c_int64), ("cell_chunks", C.c_int64), ("backward_cell", C.c_int64)]
+        [(n, C.c_int64) for n in ("backward_plain", "cell_chunks", "backward_cell",
+                                  "forward_items", "lds_persistent_forward")]
 
 
 def make_instances(nodes, events):
@@ -87,6 +89,37 @@ def test_batched_layout_sizes_and_tiles(lib):
     tun2 = Tuning(512, 1, 4096, 0, 0)
     assert lib.loits_hip_batched_encode(inst, 3, C.byref(tun2), C.byref(lay), blob) == 1
     assert "does not match" in last_error(lib)
+
+
+def test_batched_persistent_queue_and_buckets(lib):
+    # 12000-node float32 curves keep global tables (built by a launch of their own)
+    inst = make_instances([12000, 512, 2048], [70000, 5000, 300000])
+    tun = Tuning(1024, 2, 1 << 16, 0, 1)
+    lay = Layout()
+    assert lib.loits_hip_batched_layout(inst, 3, 1, C.byref(tun), C.byref(lay)) == 0, last_error(lib)
+    assert lay.persistent == 1 and lay.global_instances == 1
+    assert lay.forward_items == lay.n_chunks
+    assert lay.lds_persistent_forward == lay.lds_forward
+    blob = (C.c_uint8 * lay.desc_bytes)()
+    assert lib.loits_hip_batched_encode(inst, 3, C.byref(tun), C.byref(lay), blob) == 0
+    # per-bucket chunk sizes: the 2048-node curve (bucket 11) in chunks of 2^17, the
+    # 512-node curve (bucket 9) in one chunk of 2^13
+    base_chunks = lay.n_chunks
+    tun.bucket_chunk_log2[11] = 17
+    tun.bucket_chunk_log2[9] = 13
+    lay2 = Layout()
+    assert lib.loits_hip_batched_layout(inst, 3, 1, C.byref(tun), C.byref(lay2)) == 0
+    # 12000 nodes: ceil(70000 / 2^16) = 2; 512: 1; 2048: ceil(300000 / 2^17) = 3 (was 5)
+    assert base_chunks == 2 + 1 + 5 and lay2.n_chunks == 2 + 1 + 3
+    for field, val, msg in (("bucket_chunk_log2", 9, "bucket_chunk_log2"),
+                            ("bucket_guide_shift", 5, "bucket_guide_shift")):
+        t = Tuning(1024, 2, 1 << 16, 0, 0)
+        getattr(t, field)[3] = val
+        assert lib.loits_hip_batched_layout(inst, 3, 1, C.byref(t), C.byref(Layout())) == 1
+        assert msg in last_error(lib)
+    t = Tuning(1024, 2, 1 << 16, 0, 2)
+    assert lib.loits_hip_batched_layout(inst, 3, 1, C.byref(t), C.byref(Layout())) == 1
+    assert "persistent must be 0 or 1" in last_error(lib)
 
 
 @pytest.mark.parametrize("bad,msg", [

@@ -26,11 +26,14 @@ def _grad_err(got, ref, node_off, n_nodes):
     return worst
 
 
-@pytest.mark.parametrize("dtype,window,fmt", [(torch.float32, (0.05, 0.95), 0),
-                                              (torch.float32, (0.05, 0.95), 1),
-                                              (torch.float32, None, 0),
-                                              (torch.float64, (0.1, 0.9), 0)])
-def test_graph_replays_match_eager_and_oracle(oracle_port, hip_device, dtype, window, fmt):
+@pytest.mark.parametrize("dtype,window,fmt,persistent", [(torch.float32, (0.05, 0.95), 0, 0),
+                                                         (torch.float32, (0.05, 0.95), 1, 0),
+                                                         (torch.float32, None, 0, 0),
+                                                         (torch.float64, (0.1, 0.9), 0, 0),
+                                                         (torch.float32, (0.05, 0.95), 1, 1),
+                                                         (torch.float64, (0.1, 0.9), 0, 1)])
+def test_graph_replays_match_eager_and_oracle(oracle_port, hip_device, dtype, window, fmt,
+                                              persistent):
     from quantom_loits_amd.batched import BatchedLOITS, Tuning
     from quantom_loits_amd.workloads import ragged_family
 
@@ -42,7 +45,7 @@ def test_graph_replays_match_eager_and_oracle(oracle_port, hip_device, dtype, wi
     if window is not None:  # (no window, all accepted: the dense path, no pack)
         accept[2] = 0
     tuning = Tuning(threads=512, chunk_events=8192, scan_mode=1 if dtype == torch.float64 else 0,
-                    table_format=fmt)
+                    table_format=fmt, persistent=persistent)
     s = BatchedLOITS(w["n_nodes"], w["events"], base, np.zeros(n, np.uint64), accept,
                      w["node_off"], dtype=dtype, window=window, tuning=tuning)
     x = torch.from_numpy(w["x"]).to(dtype).cuda()
@@ -55,7 +58,7 @@ def test_graph_replays_match_eager_and_oracle(oracle_port, hip_device, dtype, wi
 
     replays = 4
     for r in range(replays):
-        print("graph replay", dtype, window, fmt, r, flush=True)
+        print("graph replay", dtype, window, fmt, persistent, r, flush=True)
         rr = np.random.default_rng(100 + r)
         dens = w["density"] * rr.uniform(0.5, 2.0, w["density"].shape)
         streams = rr.integers(0, 2 ** 33, n).astype(np.uint64)
