@@ -57,6 +57,9 @@ def parse():
     p.add_argument("--table-format", type=int, default=0)
     p.add_argument("--persistent", type=int, default=0, choices=(0, 1),
                    help="1: the batched forward and VJP as one persistent launch each")
+    p.add_argument("--ragged-mode", type=int, default=None, choices=(0, 1),
+                   help="with --ragged-only: time only per-kind (0) or persistent (1) launches "
+                        "instead of both and the autotuned plan")
     p.add_argument("--autotune", action="store_true",
                    help="pick threads / chunk / guide shift with the launch-shape autotuner")
     p.add_argument("--graph", action="store_true",
@@ -219,12 +222,15 @@ def extra_workloads(dev, steps, fp64_row):
     return out
 
 
-def ragged_workload(dev, steps, loop=True, persistent=0):
+def ragged_workload(dev, steps, loop=True, persistent=None):
     """4096 ragged curves (N log-uniform in [128, 16384], E in [1024, 2^20],
     window cut [0.05, 0.95]) as one captured graph (forward, pack, VJP; the
-    planner picks each curve's table format and chunks), against a per-curve
-    launch loop."""
+    planner picks each curve's table format and chunks), timed with per-kind
+    launches, with persistent launches and with the autotuner's per-bucket
+    choice (reported as the fused time), against a per-curve launch loop.
+    ``persistent`` (0 or 1) times that mode alone instead."""
     import torch
+    from quantom_loits_amd.autotune import Autotuner
     from quantom_loits_amd.batched import BatchedLOITS, Tuning
     from quantom_loits_amd.workloads import ragged_family
 
@@ -233,26 +239,40 @@ def ragged_workload(dev, steps, loop=True, persistent=0):
     xr = torch.from_numpy(w["x"]).to(dev)
     dr = torch.from_numpy(w["density"]).to(dev)
     base = np.concatenate([[0], np.cumsum(w["events"])[:-1]]).astype(np.uint64)
-    tun = Tuning(persistent=persistent)
-    sr = BatchedLOITS(w["n_nodes"], w["events"], base, None, None, w["node_off"],
-                      dtype=torch.float32, device=dev, window=(0.05, 0.95), tuning=tun)
-    g = sr.capture(xr, dr, SEED)
-    g.grad.normal_()
-    fused = time_ms(g.replay, max(3, steps // 4))
-    kept = int(g.kept.sum())
-    del g
+    window = (0.05, 0.95)
+    if persistent is None:
+        t0 = time.perf_counter()
+        auto = Autotuner().tuning_for(w["n_nodes"], w["events"], torch.float32, window=window)
+        tune_s = time.perf_counter() - t0
+        runs = (("per_kind", Tuning()), ("persistent", Tuning(persistent=1)), ("autotuned", auto))
+    else:
+        tune_s = None
+        runs = ((("persistent" if persistent else "per_kind"), Tuning(persistent=persistent)),)
+    modes = {}
+    for label, tun in runs:
+        sr = BatchedLOITS(w["n_nodes"], w["events"], base, None, None, w["node_off"],
+                          dtype=torch.float32, device=dev, window=window, tuning=tun)
+        g = sr.capture(xr, dr, SEED)
+        g.grad.normal_()
+        modes[label] = time_ms(g.replay, max(3, steps // 4))
+        kept = int(g.kept.sum())
+        layout = sr.layout
+        del g, sr
+    label, tun = runs[-1]
+    fused = modes[label]
     nodes = int(np.sum(w["n_nodes"]))
     fb, bb = algorithmic_bytes(kept, nodes)
     ev = int(w["events"].sum())
-    out = dict(curves=nn, events=ev, kept=kept, fused_graph_ms=fused,
-               fused_events_per_s=ev / (fused * 1e-3),
+    out = dict(curves=nn, events=ev, kept=kept, fused_graph_ms=fused, mode=label,
+               modes_ms=modes, fused_events_per_s=ev / (fused * 1e-3),
                algorithmic_gbs=(fb + bb) / (fused * 1e-3) / 1e9,
                tuning=dict(threads=tun.threads, chunk_events=tun.chunk_events,
-                           guide_shift=tun.guide_shift, persistent=tun.persistent),
-               cell_table_curves=sr.layout["cell_instances"],
-               global_table_curves=sr.layout["global_instances"],
-               range_vjp_curves=sr.layout["range_instances"])
-    del sr
+                           guide_shift=tun.guide_shift, persistent=tun.persistent,
+                           buckets={int(b): list(v) for b, v in tun.buckets.items()}),
+               autotune_s=tune_s,
+               cell_table_curves=layout["cell_instances"],
+               global_table_curves=layout["global_instances"],
+               range_vjp_curves=layout["range_instances"])
     if not loop:
         return out
     singles = []
@@ -460,8 +480,8 @@ def main():
         print(json.dumps({"dropin_two_axis": dropin_workloads(dev, args.steps, only)}))
         return
     if args.ragged_only:
-        print(json.dumps({"ragged_4096_window": ragged_workload(dev, args.steps, loop=False,
-                                                                persistent=args.persistent)}))
+        print(json.dumps({"ragged_4096_window": ragged_workload(
+            dev, args.steps, loop=False, persistent=args.ragged_mode)}))
         return
     x_np, d_np = uniform_family(B, N, seed=0, dtype=np.float32)
     x = torch.from_numpy(x_np.reshape(-1)).to(dev)

@@ -97,7 +97,7 @@ class BatchedLOITS:
             if tuning != "auto":
                 raise ValueError("tuning must be a Tuning, None or 'auto'")
             from .autotune import auto_tuning
-            tuning = auto_tuning(n_nodes, events, dtype)
+            tuning = auto_tuning(n_nodes, events, dtype, window=window)
         self.tuning = tuning or Tuning()
         self.window = window
         ext = load_extension()

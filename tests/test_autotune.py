@@ -58,3 +58,43 @@ def test_unplannable_bucket_raises(tmp_path):
                                candidates_fn=lambda scan: [Tuning(threads=256)])
     with pytest.raises(ValueError, match="no launch shape"):
         tuner.tuning_for([30000], [10], torch.float64)
+
+
+def test_ragged_batches_are_tuned_per_grid_size_bucket(tmp_path):
+    """Curves in several grid-size buckets: per-kind or persistent launches, then
+    each bucket's chunk size (and guide shift where its curves take guide tables)
+    by coordinate descent on the whole batch; the result is cached."""
+    calls = []
+    # cost model of the whole batch: persistent saves 10%; bucket 7 is fastest at
+    # 2^16, bucket 14 at 2^17 with guide shift 1, the others at the default 2^18
+    want = {7: (1 << 16, 2), 14: (1 << 17, 1)}
+
+    def timer(n_nodes, events, dtype, t, window=None):
+        calls.append((dict(t.buckets), t.persistent, window))
+        ms = 10.0 - t.persistent
+        for b in (7, 10, 12, 14):
+            ce, gs = t.buckets.get(b, (1 << 18, 2))
+            goal = want.get(b, (1 << 18, 2))
+            ms += abs(np.log2(ce) - np.log2(goal[0])) * 0.5 + abs(gs - goal[1]) * 0.2
+        return ms
+
+    nodes = np.array([100, 120, 900, 1000, 3000, 9000, 12000])
+    events = np.array([5000, 7000, 3000, 9000, 40000, 20000, 1000])
+    tuner = autotune.Autotuner(path=tmp_path / "t.json", timer=timer)
+    t = tuner.tuning_for(nodes, events, torch.float32, window=(0.05, 0.95))
+    # buckets: 7 (100, 120), 10 (900, 1000), 12 (3000), 14 (9000, 12000)
+    assert t.persistent == 1 and t.threads == 1024
+    assert t.buckets == {7: (1 << 16, 2), 14: (1 << 17, 1)}
+    n_guide = len(autotune.BUCKET_CHUNKS) * len(autotune.GUIDE_SHIFTS) - 1
+    n_cell = len(autotune.BUCKET_CHUNKS) - 1
+    assert len(calls) == 2 + 3 * n_cell + n_guide  # the defaults are not timed twice
+    assert all(c[2] == (0.05, 0.95) for c in calls)
+    cache = json.loads((tmp_path / "t.json").read_text())
+    (entry,) = cache.values()
+    assert entry["persistent"] == 1 and entry["buckets"] == {"7": [65536, 2], "14": [131072, 1]}
+    n_calls = len(calls)
+    again = autotune.Autotuner(path=tmp_path / "t.json", timer=timer)
+    t2 = again.tuning_for(nodes, events, torch.float32, window=(0.05, 0.95))
+    assert len(calls) == n_calls and t2.buckets == t.buckets and t2.persistent == 1
+    ce, shift = t.bucket_arrays()
+    assert ce[7] == 16 and ce[14] == 17 and shift[14] == 2 and ce[12] == 0 and shift[12] == 0
