@@ -478,7 +478,17 @@ __device__ void build_tables(const BatchedParams& P, int inst, const InstanceDes
 
   // 6. level-2 words (the counts are dead).  Each thread fills a contiguous
   //    run of words, walking the redirected cells in order; a sub-cell's
-  //    upper bound is the next sub-cell's lower bound.
+  //    upper bound is the next sub-cell's lower bound.  With global tables
+  //    the sorted thresholds are first staged into the counts' LDS (n <= G
+  //    words), so that the bisections read LDS instead of global memory.
+  const uint32_t* srt6 = L.srt;
+  if constexpr (GLOBAL) {
+    if (n <= G) {
+      lds_copy(hist, L.srt, pad4(n) * 4);
+      __syncthreads();
+      srt6 = hist;
+    }
+  }
   const int used = sh.used, n_assigned = sh.assigned;
   const int per_item = (used + nt - 1) / nt;
   int item = tid * per_item;
@@ -497,7 +507,7 @@ __device__ void build_tables(const BatchedParams& P, int inst, const InstanceDes
     int a = lo0, b = hi0;  // first position with srt >= key
     while (a < b) {
       const int mid = (a + b) >> 1;
-      if (L.srt[mid] < key) a = mid + 1; else b = mid;
+      if (srt6[mid] < key) a = mid + 1; else b = mid;
     }
     for (; item < item_end; ++item) {
       if (item == cell_end) {
@@ -513,9 +523,9 @@ __device__ void build_tables(const BatchedParams& P, int inst, const InstanceDes
       int c2 = a, d2 = hi0;  // first position with srt >= key_hi
       while (c2 < d2) {
         const int mid = (c2 + d2) >> 1;
-        if (L.srt[mid] < key_hi) c2 = mid + 1; else d2 = mid;
+        if (srt6[mid] < key_hi) c2 = mid + 1; else d2 = mid;
       }
-      L.guide[G + item] = guide_encode(a, c2, L.srt, key, nn);
+      L.guide[G + item] = guide_encode(a, c2, srt6, key, nn);
       a = c2;
       key = key_hi;
     }
