@@ -402,7 +402,9 @@ def cpu_child():
     from oracle.oracle import Oracle, available
     from quantom_loits_amd.workloads import reference_1d_inputs, uniform_family
 
-    model, phys, logical = cpu_description()  # before OpenMP pins this thread to one place
+    # the parent describes the CPUs: in this process numpy's OpenMP runtime has
+    # already bound the thread to one place (OMP_PROC_BIND), so this view is narrow
+    model, phys, logical = cpu_description()
     if available("openmp", portable=True):
         lib, kind = Oracle("openmp", portable=True), "reference"
     else:
@@ -445,7 +447,10 @@ def cpu_baseline():
                        capture_output=True, text=True, timeout=600)
     if r.returncode != 0:
         raise RuntimeError("cpu baseline child failed:\n" + r.stderr[-2000:])
-    return json.loads(r.stdout.strip().splitlines()[-1])
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    model, phys, logical = cpu_description()  # this (unpinned) process's CPUs
+    out.update(cpu_model=model, physical_cores=phys, logical_cpus=logical)
+    return out
 
 
 def main():
