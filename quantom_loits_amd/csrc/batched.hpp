@@ -39,7 +39,7 @@ struct BatchedParams {
   // scratch
   int32_t* flags;        // [0] = dense
   int64_t* seg_kept;     // [n_segments]
-  int64_t* chunk_kept;   // [n_chunks] kept events of each chunk
+  int64_t* chunk_kept;   // [n_chunks] kept + 1 of a chunk that dropped slots, else 0
   int64_t* chunk_row;    // [n_chunks] first packed row of each chunk
   int64_t* seg_row;      // [n_segments] first packed row within its chunk
   int32_t* seg_drop;     // [n_segments] first iteration with a dropped slot
@@ -48,8 +48,9 @@ struct BatchedParams {
   double* partials;      // per-chunk partial VJP sums
 };
 
-// Scratch flag words (P.flags, 16 int32): [0] dense, [1] finished chunks of the
-// forward, [4]/[5] next item / exited work-groups of the persistent forward,
+// Scratch flag words (P.flags, 16 int32): [0] dense, [2..3] the forward's 64-bit
+// chunk counter (finished | dropped-slot chunks << 32), [4]/[5] next item /
+// exited work-groups of the persistent forward,
 // [6]/[7] the same for the persistent VJP.  Every counter returns to zero.
 constexpr int kFlagFwdNext = 4, kFlagFwdExit = 5, kFlagVjpNext = 6, kFlagVjpExit = 7;
 

@@ -251,24 +251,26 @@ def test_headline_vjp_matches_central_differences_full_size(hip_device):
     assert worst <= 1e-5
 
 
-def test_ragged_window_graph_full_size(hip_device, oracle_port):
+@pytest.mark.parametrize("persistent", [0, 1])
+def test_ragged_window_graph_full_size(hip_device, oracle_port, persistent):
     """4096 curves, N log-uniform in [128, 16384], E log-uniform in
     [2^10, 2^20], x-window [0.05, 0.95], forward + pack + VJP replayed from one
-    captured HIP graph (the benchmark's ragged workload): kept counts of all
-    4096 curves and every kept value against the slot reference, 8 curves
-    against the oracle."""
-    from quantom_loits_amd.batched import BatchedLOITS
+    captured HIP graph (the benchmark's ragged workload), with per-kind and
+    with persistent launches (the autotuner's choice for this batch): kept
+    counts of all 4096 curves and every kept value against the slot
+    reference, 8 curves against the oracle."""
+    from quantom_loits_amd.batched import BatchedLOITS, Tuning
     from quantom_loits_amd.workloads import ragged_family
 
     w = ragged_family(4096, n_range=(128, 16384), e_range=(1024, 1 << 20), seed=5)
     n = len(w["n_nodes"])
     base = np.concatenate([[0], np.cumsum(w["events"])[:-1]]).astype(np.uint64)
     s = BatchedLOITS(w["n_nodes"], w["events"], base, None, None, w["node_off"],
-                     dtype=torch.float32, window=WINDOW)
+                     dtype=torch.float32, window=WINDOW, tuning=Tuning(persistent=persistent))
     xr, dr = _gpu(w["x"], torch.float32), _gpu(w["density"], torch.float32)
-    print("ragged graph: curves", n, "events", int(w["events"].sum()), "global-table curves",
-          s.layout["global_instances"], "bin-range curves", s.layout["range_instances"],
-          flush=True)
+    print("ragged graph: persistent", persistent, "curves", n, "events",
+          int(w["events"].sum()), "global-table curves", s.layout["global_instances"],
+          "bin-range curves", s.layout["range_instances"], flush=True)
     graph = s.capture(xr, dr, SEED)
     total = int(graph.kept.sum())
     gen = torch.Generator(device="cuda").manual_seed(2)
