@@ -1857,6 +1857,102 @@ __device__ void vjp_tail(const BatchedParams& P, const InstanceDesc& in, const d
   }
 }
 
+// A single-tile curve's share of one thread in the VJP tail (nodes t0 .. t0 + 3,
+// t0 = kTailPer * threadIdx.x), loaded into registers right after the VJP's
+// main loop so that the loads overlap the conversion and its barrier: nodes
+// and CDF t0 - 1 .. t0 + 4 and the densities t0 .. t0 + 3 (zero past the ends).
+template <typename Real>
+struct TailNodes {
+  double x[kTailPer + 2], c[kTailPer + 2], s[kTailPer];
+
+  __device__ __forceinline__ void load(const BatchedParams& P, const InstanceDesc& in,
+                                       const double* cdf) {
+    const int64_t n = in.n_nodes, t0 = int64_t(threadIdx.x) * kTailPer;
+    const Real* xg = static_cast<const Real*>(P.x) + in.node_off;
+    const Real* sg = static_cast<const Real*>(P.density) + in.node_off;
+#pragma unroll
+    for (int q = 0; q < kTailPer + 2; ++q) {
+      const int64_t t = t0 - 1 + q;
+      const bool ok = t >= 0 && t < n;
+      x[q] = ok ? double(xg[t]) : 0.0;
+      c[q] = ok ? cdf[t] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < kTailPer; ++q) s[q] = t0 + q < n ? double(sg[t0 + q]) : 0.0;
+  }
+};
+
+// vjp_tail for a curve of at most kTailPer * blockDim.x nodes from the values
+// of TailNodes: the same operations in the same order (so the same bits), with
+// each bin's 1 / (C_{j+1} - C_j + eps) computed once and the rho adjoints kept
+// in registers.
+template <typename Real>
+__device__ void vjp_tail_regs(const BatchedParams& P, const InstanceDesc& in,
+                              const TailNodes<Real>& tn, double inv_n, const double* h0,
+                              const double* h1, Real* grad_density, double* scratch) {
+  const int64_t n = in.n_nodes;
+  const double acc = in.accept ? 1.0 : 0.0;
+  const int64_t t0 = int64_t(threadIdx.x) * kTailPer;
+  double inv_d[kTailPer + 1];  // bins t0 - 1 .. t0 + kTailPer - 1
+#pragma unroll
+  for (int q = 0; q <= kTailPer; ++q) {
+    const int64_t j = t0 - 1 + q;
+    inv_d[q] = j >= 0 && j + 1 < n ? 1.0 / (tn.c[q + 1] - tn.c[q] + kEpsilon) : 0.0;
+  }
+  double gc[kTailPer];
+#pragma unroll
+  for (int k = 0; k < kTailPer; ++k) {
+    const int64_t t = t0 + k;
+    double g = 0.0;
+    if (t < n) {
+      if (t + 1 < n) {
+        const double db = tn.x[k + 2] - tn.x[k + 1];
+        g += db * (inv_d[k + 1] * inv_d[k + 1] * h1[t] - inv_d[k + 1] * h0[t]);
+      }
+      if (t >= 1) {
+        const double db = tn.x[k + 1] - tn.x[k];
+        g -= db * inv_d[k] * inv_d[k] * h1[t - 1];
+      }
+    }
+    gc[k] = g;
+  }
+  double seg = 0.0;
+#pragma unroll
+  for (int k = kTailPer - 1; k >= 0; --k) {
+    seg += gc[k];
+    gc[k] = seg;
+  }
+  double total;
+  const double excl = block_exclusive_scan(seg, scratch, &total);
+  const double carry = 0.0;
+  const double after = carry + (total - excl - seg);
+  double rho[kTailPer], local_norm = 0.0;
+#pragma unroll
+  for (int k = 0; k < kTailPer; ++k) {
+    const int64_t t = t0 + k;
+    rho[k] = 0.0;
+    if (t >= n) continue;
+    const double S_t = gc[k] + after;
+    const double S_next = k + 1 < kTailPer ? gc[k + 1] + after : after;
+    double g = 0.0;
+    if (t >= 1) g += 0.5 * (tn.x[k + 1] - tn.x[k]) * acc * S_t;
+    if (t + 1 < n) g += 0.5 * (tn.x[k + 2] - tn.x[k + 1]) * acc * S_next;
+    local_norm += g * tn.s[k];
+    rho[k] = g;
+  }
+  const double grad_norm = -block_sum(local_norm, scratch) * inv_n * inv_n;
+  Real* out = grad_density + in.node_off;
+#pragma unroll
+  for (int k = 0; k < kTailPer; ++k) {
+    const int64_t t = t0 + k;
+    if (t >= n) continue;
+    double wgt = 0.0;
+    if (t >= 1) wgt += 0.5 * (tn.x[k + 1] - tn.x[k]);
+    if (t + 1 < n) wgt += 0.5 * (tn.x[k + 2] - tn.x[k + 1]);
+    out[t] = static_cast<Real>(rho[k] * inv_n + grad_norm * wgt);
+  }
+}
+
 // Per-bin data of the fixed-point VJP, {alpha_j, beta_j}: t / (tau_j d_j) =
 // float(k - T_j) * alpha_j + beta_j lies in [0, 1) for every key of bin j of a
 // monotone curve (tau_j = 1 except for the last bin, which also takes the keys
@@ -2544,6 +2640,11 @@ __device__ void fast_vjp_body(const BatchedParams& P, const float* grad, float* 
       }
     }
   }
+  // a single-chunk curve of at most kTailPer * nt nodes: the tail's nodes,
+  // densities and CDF values load now, under the barrier and the conversion
+  const bool regs_tail = in.n_chunks == 1 && int64_t(n) <= int64_t(kTailPer) * nt;
+  TailNodes<float> tn;
+  if (regs_tail) tn.load(P, in, cdf);
   __syncthreads();
 
   // integers (and float64 sums) -> h0 = sum g, h1 = sum g (u - C_j), in place
@@ -2552,12 +2653,11 @@ __device__ void fast_vjp_body(const BatchedParams& P, const float* grad, float* 
   {
     const double inv_scale = __builtin_ldexp(1.0, -scale_log2);
     const bool any_outl = sh_outl != 0;
-    for (int64_t i = tid; i < nb; i += nt) {
+    auto convert = [&](int64_t i, double C) {
       double v0 = 0.0, v1 = 0.0;
       if (i < n - 1) {
         const uint64_t A0 = a0[i], A1 = a1[i];
         const uint32_t T = own[i];
-        const double C = cdf[i];
         const int64_t d = static_cast<int64_t>(A1 - uint64_t(T) * A0);  // sum q (k - T_j)
         v0 = double(static_cast<int64_t>(A0)) * inv_scale + (any_outl ? load_l2(outl + i) : 0.0);
         v1 = (double(d) * inv_scale + (any_outl ? load_l2(outl + nb + i) : 0.0)) * 0x1.0p-24 +
@@ -2565,9 +2665,21 @@ __device__ void fast_vjp_body(const BatchedParams& P, const float* grad, float* 
       }
       h0[i] = v0;
       h1[i] = v1;
+    };
+    if (regs_tail) {  // this thread's own bins (nb <= kTailPer * nt), C from registers
+      const int64_t t0 = int64_t(tid) * kTailPer;
+#pragma unroll
+      for (int k = 0; k < kTailPer; ++k)
+        if (t0 + k < nb) convert(t0 + k, tn.c[k + 1]);
+    } else {
+      for (int64_t i = tid; i < nb; i += nt) convert(i, i < n - 1 ? cdf[i] : 0.0);
     }
   }
   __syncthreads();
+  if (regs_tail) {
+    vjp_tail_regs<float>(P, in, tn, hdr->inv_n, h0, h1, grad_density, scratch);
+    return;
+  }
   if (in.n_chunks == 1) {
     vjp_tail<float>(P, in, cdf, hdr->inv_n, h0, h1, grad_density, scratch);
     return;
