@@ -341,7 +341,8 @@ int loits_hip_batched_backward(const loits_hip_batched_args* args, const void* g
  * The backward writes d(sum_ic ghist[i][c] hist[i][c]) / d v_r to grad_values.
  * row_start is a device array of n_instances + 1 ascending offsets; bins <= 128.
  * NaN values take no part (no histogram weight, gradient 0); values outside
- * [lo, hi] by more than three bins reach no bin.
+ * [lo, hi] by more than three bins reach no bin; rows before row_start[0] or
+ * from row_start[n_instances] on take no part either (gradient 0).
  * Float atomics: sums may differ from run to run at float rounding. */
 int loits_hip_soft_histogram(const float* values, const int64_t* row_start, int64_t n_instances,
                              int64_t n_rows, float lo, float hi, int bins, float width,

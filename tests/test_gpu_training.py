@@ -80,20 +80,23 @@ def test_torch_comparator_step_runs(hip_device):
     assert 0 < loits_ms < step_ms
 
 
-def test_hip_soft_histogram_matches_torch_layer(hip_device):
+@pytest.mark.parametrize("width", [0.5, 0.1])
+def test_hip_soft_histogram_matches_torch_layer(hip_device, width):
     """The HIP soft-binning kernels against the torch formulation: values,
-    gradients, an empty instance, instances spanning work-groups, and values
-    outside the window."""
-    from quantom_loits_amd.training import soft_histogram, soft_histogram_hip
+    gradients, an empty instance, instances spanning work-groups, values
+    outside the window; width 0.5 bins takes the factored taps (three
+    exponentials per row), width 0.1 the seven exponentials."""
+    from quantom_loits_amd.training import BINS, WINDOW, soft_histogram, soft_histogram_hip
 
-    print("soft histogram", flush=True)
+    print("soft histogram width", width, flush=True)
     g = torch.Generator(device="cuda").manual_seed(5)
-    kept = torch.tensor([30000, 0, 5, 20000, 1, 17000], device="cuda")
+    kept = torch.tensor([30000, 0, 5, 20000, 1, 17000, 9000], device="cuda")
     v = torch.rand(int(kept.sum()), device="cuda", generator=g) * 1.1 - 0.05
+    v[-9000:] = 0.5 + 0.01 * torch.randn(9000, device="cuda", generator=g)  # one crowded bin
     v1 = v.clone().requires_grad_(True)
     v2 = v.clone().requires_grad_(True)
-    h1 = soft_histogram_hip(v1, kept)
-    h2 = soft_histogram(v2, kept)
+    h1 = soft_histogram_hip(v1, kept, WINDOW[0], WINDOW[1], BINS, width)
+    h2 = soft_histogram(v2, kept, WINDOW[0], WINDOW[1], BINS, width)
     assert torch.allclose(h1, h2, rtol=1e-4, atol=1e-6)
     w = torch.randn(h1.shape, device="cuda", generator=g)
     (h1 * w).sum().backward()
