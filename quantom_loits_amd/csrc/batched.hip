@@ -344,11 +344,17 @@ __device__ void build_tables(const BatchedParams& P, int inst, const InstanceDes
       if (!__builtin_isfinite(m)) bits &= ~2u;
       if constexpr (sizeof(Real) == 4) {
         const double r = double(thr) * 0x1.0p-24 - c;
-        const CoefF32View cv = coef_f32_view(L.coef, pad4(n));
-        cv.ma[t] = make_float2(static_cast<float>(m * 0x1.0p-24),
-                               static_cast<float>(__builtin_fma(m, r, xt)));
-        cv.thr[t] = thr;
-        cv.r[t] = static_cast<float>(r);
+        const float M = static_cast<float>(m * 0x1.0p-24);
+        const float A = static_cast<float>(__builtin_fma(m, r, xt));
+        if constexpr (GLOBAL) {  // one 16-byte record per bin (CoefF32)
+          reinterpret_cast<float4*>(L.coef)[t] =
+              make_float4(M, A, __uint_as_float(thr), static_cast<float>(r));
+        } else {
+          const CoefF32View cv = coef_f32_view(L.coef, pad4(n));
+          cv.ma[t] = make_float2(M, A);
+          cv.thr[t] = thr;
+          cv.r[t] = static_cast<float>(r);
+        }
       } else {
         L.coef[t] = CoefF64{c, m, xt, thr, 0u};
       }
@@ -1014,7 +1020,7 @@ __device__ __noinline__ double interp_reference(const double* cdf, const Real* x
 // differ from the fp64 decision (|v| < 1e-30 or > 1e30, non-finite, or within
 // 2e-6 |v| of a window edge) is re-evaluated in fp64 from the global tables.
 // tarr: the bins' own thresholds T_j (the sorted array when it is the same).
-template <typename Real, bool WINDOW, int NK>
+template <typename Real, bool WINDOW, int NK, bool AOS = false>
 __device__ __forceinline__ int slot_values(const uint32_t (&k)[NK], const int (&j)[NK],
                                            const typename CoefOf<Real>::type* coef,
                                            const uint32_t* tarr, const double* cdf_g,
@@ -1022,13 +1028,23 @@ __device__ __forceinline__ int slot_values(const uint32_t (&k)[NK], const int (&
                                            bool safe_curve, Real (&v)[NK]) {
   int keep = 0;
   if constexpr (sizeof(Real) == 4) {
-    const float2* ma = reinterpret_cast<const float2*>(coef);
     float2 cm[NK];
     uint32_t ct[NK];
+    if constexpr (AOS) {
+      const float4* rec = reinterpret_cast<const float4*>(coef);
 #pragma unroll
-    for (int e = 0; e < NK; ++e) {
-      cm[e] = ma[j[e]];
-      ct[e] = tarr[j[e]];
+      for (int e = 0; e < NK; ++e) {
+        const float4 q = rec[j[e]];
+        cm[e] = make_float2(q.x, q.y);
+        ct[e] = __float_as_uint(q.z);
+      }
+    } else {
+      const float2* ma = reinterpret_cast<const float2*>(coef);
+#pragma unroll
+      for (int e = 0; e < NK; ++e) {
+        cm[e] = ma[j[e]];
+        ct[e] = tarr[j[e]];
+      }
     }
     if (!WINDOW && safe_curve) {  // every value of this curve is kept (kSafe)
 #pragma unroll
@@ -1215,7 +1231,9 @@ __device__ void finalize_if_last(const BatchedParams& P, int c) {
 // each call's address spaces known to the compiler).
 // Interval search and values of a trip's keys over the two-level guide tables
 // (find_bins, slot_values); returns the keep bits.
-template <typename Real, bool WINDOW>
+// AOS: float32 per-bin data as one {M, A, T, r} record per bin (the layout of
+// global tables, one 16-byte read per key from L2) instead of the arrays.
+template <typename Real, bool WINDOW, bool AOS = false>
 struct GuideLookup {
   static constexpr int kBinBias = 0;  // returns j
   const uint32_t* guide;
@@ -1231,7 +1249,7 @@ struct GuideLookup {
   __device__ __forceinline__ int operator()(const uint32_t (&k)[NK], int (&j)[NK], Real (&v)[NK],
                                             const BatchedParams& P) const {
     find_bins<NK>(k, guide, srt, shift, n, j);
-    return slot_values<Real, WINDOW, NK>(k, j, coef, thr, cdf, x, P, safe, v);
+    return slot_values<Real, WINDOW, NK, AOS>(k, j, coef, thr, cdf, x, P, safe, v);
   }
 };
 
@@ -1560,12 +1578,10 @@ __device__ __forceinline__ void guide_chunk(const BatchedParams& P, int c, uint8
     lds_copy(lds, T.guide, guide_bytes(T, in));
     if (threadIdx.x == 0) sh.flags = T.hdr->flags;
     __syncthreads();
-    const uint32_t* thr = (sh.flags & kMonotone) != 0 || sizeof(Real) != 4
-                              ? T.srt
-                              : coef_f32_view(T.coef, nb).thr;
-    const GuideLookup<Real, WINDOW> lk{reinterpret_cast<const uint32_t*>(lds), T.srt, T.coef, thr,
-                                       T.cdf, x_g, 24 - in.guide_log2, int(in.n_nodes),
-                                       (sh.flags & kSafe) != 0};
+    // float32 global tables hold each bin's own threshold in its record
+    const GuideLookup<Real, WINDOW, sizeof(Real) == 4> lk{
+        reinterpret_cast<const uint32_t*>(lds), T.srt, T.coef, T.srt, T.cdf, x_g,
+        24 - in.guide_log2, int(in.n_nodes), (sh.flags & kSafe) != 0};
     for (int sgi = next_segment(sh_next); sgi < P.segs; sgi = next_segment(sh_next))
       sample_segment<Real, WINDOW>(P, c, ch, in, sgi, lk, sh_kept);
   } else {
@@ -2293,10 +2309,20 @@ __device__ __forceinline__ void vjp_body(const BatchedParams& P, const Real* gra
           for (int e = 0; e < 4; ++e) {
             const int je = j[4 * h + e];
             if constexpr (sizeof(Real) == 4) {
-              const CoefF32View cv = coef_f32_view(const_cast<uint8_t*>(s_bin), nb);
-              t[e] = double(static_cast<float>(static_cast<int32_t>(k[4 * h + e] - cv.thr[je])) *
+              uint32_t tj;
+              float rj;
+              if constexpr (GLOBAL) {  // global tables: {M, A, T, r} records
+                const float4 q = reinterpret_cast<const float4*>(s_bin)[je];
+                tj = __float_as_uint(q.z);
+                rj = q.w;
+              } else {
+                const CoefF32View cv = coef_f32_view(const_cast<uint8_t*>(s_bin), nb);
+                tj = cv.thr[je];
+                rj = cv.r[je];
+              }
+              t[e] = double(static_cast<float>(static_cast<int32_t>(k[4 * h + e] - tj)) *
                                 0x1.0p-24f +
-                            cv.r[je]);
+                            rj);
             } else {
               t[e] = double(k[4 * h + e]) * 0x1.0p-24 - s_coef[je].c;
             }

@@ -347,11 +347,14 @@ constexpr uint32_t kNoPoint = 0x1000u;
 // fp32 path: value = A_j + M_j * (k - T_j) with M_j = m_j 2^-24 and
 // A_j = x_j + m_j (T_j 2^-24 - C_j), both rounded from fp64, which is
 // x_j + m_j (u - C_j) for u = k 2^-24; r_j = T_j 2^-24 - C_j rounded to float
-// gives u - C_j = (k - T_j) 2^-24 + r_j for the VJP.  Stored as a structure
-// of arrays over nb = pad4(n) bins (a random 8-byte LDS read costs what a
-// 4-byte one does, a 12-byte one three times that):
+// gives u - C_j = (k - T_j) 2^-24 + r_j for the VJP.  Tables built in LDS
+// store a structure of arrays over nb = pad4(n) bins (a random 8-byte LDS read
+// costs what a 4-byte one does, a 12-byte one three times that):
 //   float2 ma[nb] = {M_j, A_j} | uint32 thr[nb] = T_j | float r[nb] = r_j
-struct alignas(16) CoefF32 {  // one bin's share of the arrays (sizing only)
+// Tables built in global memory (read from L2 while sampling) store one
+// 16-byte record {M_j, A_j, T_j (bits), r_j} per bin instead, so that a key
+// costs one L2 access rather than two.
+struct alignas(16) CoefF32 {  // one bin's share (sizing only)
   uint32_t w[4];
 };
 struct CoefF32View {
