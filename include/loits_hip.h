@@ -215,10 +215,10 @@ int loits_hip_backward_bounded(const loits_hip_backward_args* args, const int64_
  * density at the same offsets; slot s (0 <= s < events) draws the uniform of
  * Philox counter p = counter_base + s on stream `stream` (word p % 4 of block
  * p / 4), exactly as the reference's slot p does (cpp/bindings.cpp:79-92).
- * Capacity row out_off + s belongs to slot s.  Each wave of a work-group owns
- * a contiguous run of slots (a segment) and writes the segment's kept events,
- * in slot order, to the front of the segment's capacity rows; when every slot
- * is kept the capacity layout is already the packed output. */
+ * Capacity row out_off + s belongs to slot s: the forward writes each slot's
+ * value there, kept or not, with a keep mask; loits_hip_batched_pack gathers
+ * the kept rows.  When every slot is kept the capacity layout is already the
+ * packed output. */
 typedef struct loits_hip_instance {
   int64_t node_off;
   int64_t n_nodes;      /* >= 2 */
@@ -317,7 +317,7 @@ typedef struct loits_hip_batched_args {
  * per-instance stable compaction into the capacity layout.  No host sync. */
 int loits_hip_batched_forward(const loits_hip_batched_args* args, void* stream);
 
-/* Gather the segments of the capacity layout into a contiguous [total] array
+/* Gather the kept rows of the capacity layout into a contiguous [total] array
  * (stable order); does nothing when every slot was kept (the capacity layout
  * is then already packed).
  * `packed` has capacity rows; only the first *total are written. */

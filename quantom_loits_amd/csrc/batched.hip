@@ -1384,7 +1384,15 @@ __device__ __forceinline__ void sample_segment(const BatchedParams& P, int c, co
   int64_t running = 0;
   int first_drop = int(sg.iters);  // iterations before this one kept every slot
 
-  // Keep bits, rows and stores of one iteration (four slots per lane).
+  // One iteration (four slots per lane): the running kept count; the keep
+  // masks from the segment's first iteration that dropped a live slot on; and
+  // every live slot's value (and bin) at its own capacity row 4 i + e - off0,
+  // kept or not, so that no row depends on a scan (pack_kernel gathers the
+  // kept rows).  Whole blocks go out as one 16-byte store where the rows are
+  // aligned.
+  const bool vec_rows = ((base_row - int64_t(off0)) & 3) == 0;
+  Real* out0 = out - off0;  // slot position 4 i + e of the segment
+  int32_t* bins0 = bins_out ? bins_out - off0 : nullptr;
   auto emit = [&](int it, uint32_t i0, const Real* v, const int* j, int keep) {
     const uint32_t i = i0 + uint32_t(lane);
     const bool full_iter = (i0 > 0 || off0 == 0) && 4 * (i0 + 64) <= span;
@@ -1398,45 +1406,40 @@ __device__ __forceinline__ void sample_segment(const BatchedParams& P, int c, co
       }
     }
     const int keepbits = keep & live;
-    int64_t row;
-    int total;
     if (first_drop == int(sg.iters) && ballot(keepbits != live) == 0ull) {
-      // every slot so far kept: slot 4i + e is row 4i + e - off0
-      const int32_t r0 = int32_t(4 * i) - int32_t(off0);
-      row = r0 < 0 ? 0 : r0;
       const uint32_t a = 4 * i0 > off0 ? 4 * i0 : off0;
       const uint32_t b = 4 * (i0 + 64) < span ? 4 * (i0 + 64) : span;
-      total = b > a ? int(b - a) : 0;
+      running += b > a ? int64_t(b - a) : 0;
     } else {
       if (first_drop == int(sg.iters)) first_drop = it;
+      int total = 0;
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         const uint64_t bb = ballot((keepbits >> e) & 1);
         if (lane == e) mask[int64_t(it) * 4 + e] = bb;
+        total += popc64(bb);
       }
-      row = running + wave_scan_counts(__popc(keepbits), &total);
+      running += total;
     }
-    if (keepbits == 0xF && ((base_row + row) & 3) == 0) {
+    if (vec_rows && live == 0xF) {
       if constexpr (sizeof(Real) == 4) {
-        store_stream4(out + row, v[0], v[1], v[2], v[3]);
+        store_stream4(out0 + 4 * i, v[0], v[1], v[2], v[3]);
       } else {
-        reinterpret_cast<double2*>(out + row)[0] = make_double2(v[0], v[1]);
-        reinterpret_cast<double2*>(out + row)[1] = make_double2(v[2], v[3]);
+        reinterpret_cast<double2*>(out0 + 4 * i)[0] = make_double2(v[0], v[1]);
+        reinterpret_cast<double2*>(out0 + 4 * i)[1] = make_double2(v[2], v[3]);
       }
-      if (bins_out)
-        *reinterpret_cast<int4*>(bins_out + row) =
+      if (bins0)
+        *reinterpret_cast<int4*>(bins0 + 4 * i) =
             make_int4(j[0] - kBias, j[1] - kBias, j[2] - kBias, j[3] - kBias);
     } else {
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        if ((keepbits >> e) & 1) {
-          out[row] = v[e];
-          if (bins_out) bins_out[row] = j[e] - kBias;
-          ++row;
+        if ((live >> e) & 1) {
+          out0[4 * i + e] = v[e];
+          if (bins0) bins0[4 * i + e] = j[e] - kBias;
         }
       }
     }
-    running += total;
   };
 
   // A curve whose every value is kept (kSafe, no window): slot 4 i + e of the
@@ -1744,49 +1747,91 @@ __global__ void __launch_bounds__(1024) forward_persistent_kernel(BatchedParams 
   leave_queue(P, kFlagFwdNext, kFlagFwdExit);
 }
 
-// Pack: one work-group per chunk, segments round-robin over its waves; a no-op when dense.
+// Pack: one work-group per chunk, segments round-robin over its waves; a no-op
+// when dense.  The forward left slot 4 i + e of a segment at capacity row
+// base + 4 i + e - off0 (kept or not): the rows before the segment's first
+// drop are copied straight, the rest are gathered by their keep masks.
+constexpr int kPackTrip = 2;  // mask iterations per wave with loads in flight
+
+// One segment's rows of one array (values or bins) into `out` from row dst.
+template <typename T>
+__device__ __forceinline__ void pack_segment(const T* src_rows, T* out, int64_t dst,
+                                             int64_t straight, const uint64_t* mask,
+                                             int first_drop, int iters, int64_t off0) {
+  const int lane = lane_id();
+  for (int64_t r0 = 0; r0 < straight; r0 += 8 * kWave) {  // eight rows per lane in flight
+    T v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int64_t r = r0 + u * kWave + lane;
+      v[u] = r < straight ? src_rows[r] : T(0);
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int64_t r = r0 + u * kWave + lane;
+      if (r < straight) out[dst + r] = v[u];
+    }
+  }
+  if (first_drop >= iters) return;
+  // From the first drop on, four positions p = 64 m + lane of each
+  // iteration's 256 slots per lane: slot p of the forward's wave was lane
+  // p / 4, word p % 4 of the iteration's keep masks.  Loads and stores are
+  // coalesced: the kept positions of one m take consecutive rows, the
+  // ballot's lower bits counting those before the lane.
+  const T* pos0 = src_rows - off0;  // slot position 4 i + e of the segment
+  int64_t running = dst + straight;
+  const int sel = lane & 3, quad = lane >> 2;
+  for (int it0 = first_drop; it0 < iters; it0 += kPackTrip) {
+    T v[4 * kPackTrip];
+    uint64_t bb[4 * kPackTrip];
+#pragma unroll
+    for (int h = 0; h < kPackTrip; ++h) {
+      const bool in_seg = it0 + h < iters;
+      const int it = in_seg ? it0 + h : it0;
+      const uint64_t* mw = mask + int64_t(it) * 4;
+      const uint64_t word = sel == 0 ? mw[0] : (sel == 1 ? mw[1] : (sel == 2 ? mw[2] : mw[3]));
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        const bool k = in_seg && ((word >> (16 * m + quad)) & 1ull);
+        v[4 * h + m] = k ? pos0[int64_t(it) * 256 + 64 * m + lane] : T(0);
+        bb[4 * h + m] = ballot(k);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 4 * kPackTrip; ++q) {
+      if ((bb[q] >> lane) & 1ull) out[running + popc64(bb[q] & lanemask_lt())] = v[q];
+      running += popc64(bb[q]);
+    }
+  }
+}
+
+// Eight waves per SIMD for float32 (two work-groups per CU), four for float64.
 template <typename Real>
-__global__ void __launch_bounds__(1024) pack_kernel(BatchedParams P, Real* packed,
-                                                    int32_t* packed_bins) {
+__global__ void __launch_bounds__(1024, sizeof(Real) == 4 ? 8 : 4)
+    pack_kernel(BatchedParams P, Real* packed, int32_t* packed_bins) {
   if (P.flags[0]) return;
   const ChunkDesc ch = P.chunks[blockIdx.x];
   const InstanceDesc in = P.inst[ch.instance];
-  const int nw = blockDim.x >> 6;
-  for (int w = threadIdx.x >> 6; w < P.segs; w += nw) {
+  const int nwv = blockDim.x >> 6;
+  const int64_t per_wave = int64_t(blocks_per_wave(uint64_t(chunk_blocks(in, ch)), P.segs) / 64);
+  for (int w = threadIdx.x >> 6; w < P.segs; w += nwv) {
     const Segment sg = segment_of(in.counter_base, ch.slot_lo, ch.slot_hi, w, P.segs);
     const int64_t seg = int64_t(blockIdx.x) * P.segs + w;
-    const int64_t cnt = P.seg_kept[seg];
     const int64_t src = in.out_off + int64_t(sg.p_lo - in.counter_base);
     const int64_t dst = P.chunk_row[blockIdx.x] + P.seg_row[seg];
-    const Real* ev = static_cast<const Real*>(P.events);
-    const int lane = lane_id();
-    // eight rows per lane in flight: all loads before the stores
-    for (int64_t r0 = 0; r0 < cnt; r0 += 8 * kWave) {
-      Real v[8];
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const int64_t r = r0 + u * kWave + lane;
-        v[u] = r < cnt ? ev[src + r] : Real(0);
-      }
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const int64_t r = r0 + u * kWave + lane;
-        if (r < cnt) packed[dst + r] = v[u];
-      }
-      if (packed_bins && P.bins) {
-        int32_t b[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          const int64_t r = r0 + u * kWave + lane;
-          b[u] = r < cnt ? P.bins[src + r] : 0;
-        }
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          const int64_t r = r0 + u * kWave + lane;
-          if (r < cnt) packed_bins[dst + r] = b[u];
-        }
-      }
-    }
+    const int first_drop = P.seg_drop[seg];
+    const uint32_t off0 = uint32_t(sg.p_lo - 4 * sg.qa);
+    const uint32_t span = uint32_t(sg.p_hi - 4 * sg.qa);
+    // rows of the iterations before the first drop: every live slot was kept
+    const uint64_t lim = uint64_t(256) * uint64_t(first_drop);
+    const int64_t end = int64_t(lim < span ? lim : span);
+    const int64_t straight = end > int64_t(off0) ? end - int64_t(off0) : 0;
+    const uint64_t* mask = P.keepmask + ch.mask_off + int64_t(w) * per_wave * 4;
+    pack_segment<Real>(static_cast<const Real*>(P.events) + src, packed, dst, straight, mask,
+                       first_drop, int(sg.iters), off0);
+    if (packed_bins && P.bins)
+      pack_segment<int32_t>(P.bins + src, packed_bins, dst, straight, mask, first_drop,
+                            int(sg.iters), off0);
   }
 }
 
