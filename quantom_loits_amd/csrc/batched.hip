@@ -1101,6 +1101,49 @@ __device__ __forceinline__ int wave_scan_counts(int cnt, int* total) {
   return popc64(b0 & lt) + 2 * popc64(b1 & lt) + 4 * popc64(b2 & lt);
 }
 
+// Keep bits of a lane's four slots in iteration `it` of a segment of the VJP,
+// and the segment-relative packed row of its first kept slot (its other kept
+// slots follow in slot order).  Before the segment's first drop every live
+// slot was kept: slot 4 i + e is row 4 i + e - off0.  From it on the
+// iteration's keep masks count the kept slots of the lower lanes (mbcnt), and
+// `run` (wave-uniform) the rows of the iterations before; it must start at
+// the rows before the first drop.
+__device__ __forceinline__ int kept_slots(const uint64_t* mask, int it, int first_drop,
+                                          uint32_t i0, uint32_t nblk, uint32_t off0,
+                                          uint32_t span, int64_t& run, int64_t* row) {
+  const int lane = lane_id();
+  const uint32_t i = i0 + uint32_t(lane);
+  if (it < first_drop) {
+    int live = 0xF;
+    if (!((i0 > 0 || off0 == 0) && 4 * (i0 + 64) <= span)) {
+      live = 0;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const uint32_t r = 4 * i + e;
+        live |= (i < nblk && r >= off0 && r < span) ? (1 << e) : 0;
+      }
+    }
+    const int32_t r0 = int32_t(4 * i) - int32_t(off0);
+    *row = r0 < 0 ? 0 : r0;
+    return live;
+  }
+  const uint64_t* mw = mask + int64_t(it) * 4;
+  const uint64_t m0 = mw[0], m1 = mw[1], m2 = mw[2], m3 = mw[3];
+  const int kb = int((m0 >> lane) & 1ull) | (int((m1 >> lane) & 1ull) << 1) |
+                 (int((m2 >> lane) & 1ull) << 2) | (int((m3 >> lane) & 1ull) << 3);
+  uint32_t below = __builtin_amdgcn_mbcnt_lo(uint32_t(m0), 0u);
+  below = __builtin_amdgcn_mbcnt_hi(uint32_t(m0 >> 32), below);
+  below = __builtin_amdgcn_mbcnt_lo(uint32_t(m1), below);
+  below = __builtin_amdgcn_mbcnt_hi(uint32_t(m1 >> 32), below);
+  below = __builtin_amdgcn_mbcnt_lo(uint32_t(m2), below);
+  below = __builtin_amdgcn_mbcnt_hi(uint32_t(m2 >> 32), below);
+  below = __builtin_amdgcn_mbcnt_lo(uint32_t(m3), below);
+  below = __builtin_amdgcn_mbcnt_hi(uint32_t(m3 >> 32), below);
+  *row = run + int64_t(below);
+  run += popc64(m0) + popc64(m1) + popc64(m2) + popc64(m3);
+  return kb;
+}
+
 // Keep-mask words of a chunk: wave w's iteration `it` owns words
 // [(w * per_wave + it) * 4, +4), one 64-bit ballot per slot position e.
 __device__ __forceinline__ int64_t chunk_blocks(const InstanceDesc& in, const ChunkDesc& ch) {
@@ -2267,29 +2310,8 @@ __device__ __forceinline__ void vjp_body(const BatchedParams& P, const Real* gra
 
     // keep bits and first packed row of a lane's four slots in iteration `it`
     auto rows_of = [&](int it, uint32_t i0, int& keepbits, int64_t& row) {
-      const uint32_t i = i0 + uint32_t(lane);
-      const bool full_iter = (i0 > 0 || off0 == 0) && 4 * (i0 + 64) <= span;
-      keepbits = 0xF;
-      if (it < first_drop) {  // every live slot of these iterations was kept
-        if (!full_iter) {
-          keepbits = 0;
-  #pragma unroll
-          for (int e = 0; e < 4; ++e) {
-            const uint32_t r = 4 * i + e;
-            keepbits |= (i < nblk && r >= off0 && r < span) ? (1 << e) : 0;
-          }
-        }
-        const int32_t r0 = int32_t(4 * i) - int32_t(off0);
-        row = row0 + (r0 < 0 ? 0 : r0);
-      } else {
-        const uint64_t* mw = mask + int64_t(it) * 4;
-        keepbits = 0;
-  #pragma unroll
-        for (int e = 0; e < 4; ++e) keepbits |= int((mw[e] >> lane) & 1ull) << e;
-        int total;
-        row = row0 + running + wave_scan_counts(__popc(keepbits), &total);
-        running += total;
-      }
+      keepbits = kept_slots(mask, it, first_drop, i0, nblk, off0, span, running, &row);
+      row += row0;
     };
 
     // kTrip iterations per trip (independent Philox / search chains).
@@ -2555,32 +2577,6 @@ __device__ void fast_vjp_body(const BatchedParams& P, const float* grad, float* 
     int64_t running = int64_t(4 * 64 * int64_t(first_drop)) - int64_t(off0);
     running = running < 0 ? 0 : running;
 
-    auto rows_of = [&](int it, uint32_t i0, int& keepbits, int64_t& row) {
-      const uint32_t i = i0 + uint32_t(lane);
-      const bool full_iter = (i0 > 0 || off0 == 0) && 4 * (i0 + 64) <= span;
-      keepbits = 0xF;
-      if (it < first_drop) {
-        if (!full_iter) {
-          keepbits = 0;
-#pragma unroll
-          for (int e = 0; e < 4; ++e) {
-            const uint32_t r = 4 * i + e;
-            keepbits |= (i < nblk && r >= off0 && r < span) ? (1 << e) : 0;
-          }
-        }
-        const int32_t r0 = int32_t(4 * i) - int32_t(off0);
-        row = row0 + (r0 < 0 ? 0 : r0);
-      } else {
-        const uint64_t* mw = mask + int64_t(it) * 4;
-        keepbits = 0;
-#pragma unroll
-        for (int e = 0; e < 4; ++e) keepbits |= int((mw[e] >> lane) & 1ull) << e;
-        int total;
-        row = row0 + running + wave_scan_counts(__popc(keepbits), &total);
-        running += total;
-      }
-    };
-
     // jj = j + 1 (cell_bins): the arrays are addressed from one entry below
     auto add = [&](int jj, uint32_t k, float gv) {
       const float gs = gv * scale;
@@ -2661,6 +2657,9 @@ __device__ void fast_vjp_body(const BatchedParams& P, const float* grad, float* 
       continue;
     }
 
+    // Cotangents of the kept slots (0 for the others, whose adds are then
+    // zero): no branch on the keep bits around the histogram updates.
+    const float* gseg = grad + row0;
     int it = 0;
     for (uint32_t i0 = 0; i0 < nblk; i0 += 64 * kTrip, it += kTrip) {
       uint32_t k[4 * kTrip];
@@ -2673,42 +2672,31 @@ __device__ void fast_vjp_body(const BatchedParams& P, const float* grad, float* 
         k[4 * h + 2] = wd.w2 >> 8;
         k[4 * h + 3] = wd.w3 >> 8;
       }
-      int kb[kTrip];
-      int64_t rw[kTrip];
-#pragma unroll
-      for (int h = 0; h < kTrip; ++h) {
-        kb[h] = 0;
-        rw[h] = 0;
-        if (h == 0 || i0 + 64 * h < nblk) rows_of(it + h, i0 + 64 * h, kb[h], rw[h]);
-      }
-      int j[4 * kTrip];
-      cell_bins<4 * kTrip>(k, cells, l2, srt, s, j);
       float gv[4 * kTrip];
 #pragma unroll
       for (int h = 0; h < kTrip; ++h) {
-        if (kb[h] == 0xF && (rw[h] & 3) == 0) {
-          const float4 g4 = load_stream4(grad + rw[h]);
+        int kb = 0;
+        int64_t rw = 0;
+        if (h == 0 || i0 + 64 * h < nblk)
+          kb = kept_slots(mask, it + h, first_drop, i0 + 64 * h, nblk, off0, span, running, &rw);
+        if (kb == 0xF && ((row0 + rw) & 3) == 0) {
+          const float4 g4 = load_stream4(gseg + rw);
           gv[4 * h] = g4.x;
           gv[4 * h + 1] = g4.y;
           gv[4 * h + 2] = g4.z;
           gv[4 * h + 3] = g4.w;
         } else {
-          int64_t r = rw[h];
 #pragma unroll
-          for (int e = 0; e < 4; ++e) gv[4 * h + e] = ((kb[h] >> e) & 1) ? grad[r++] : 0.0f;
+          for (int e = 0; e < 4; ++e) {
+            gv[4 * h + e] = ((kb >> e) & 1) ? gseg[rw] : 0.0f;
+            rw += (kb >> e) & 1;
+          }
         }
       }
+      int j[4 * kTrip];
+      cell_bins<4 * kTrip>(k, cells, l2, srt, s, j);
 #pragma unroll
-      for (int h = 0; h < kTrip; ++h) {
-        if (kb[h] == 0xF) {
-#pragma unroll
-          for (int e = 0; e < 4; ++e) add(j[4 * h + e], k[4 * h + e], gv[4 * h + e]);
-        } else {
-#pragma unroll
-          for (int e = 0; e < 4; ++e)
-            if ((kb[h] >> e) & 1) add(j[4 * h + e], k[4 * h + e], gv[4 * h + e]);
-        }
-      }
+      for (int e = 0; e < 4 * kTrip; ++e) add(j[e], k[e], gv[e]);
     }
   }
   // a single-chunk curve of at most kTailPer * nt nodes: the tail's nodes,
@@ -2950,29 +2938,8 @@ __device__ void vjp_range_body(const BatchedParams& P, const Real* grad, Real* g
     running = running < 0 ? 0 : running;
 
     auto rows_of = [&](int it, uint32_t i0, int& keepbits, int64_t& row) {
-      const uint32_t i = i0 + uint32_t(lane);
-      const bool full_iter = (i0 > 0 || off0 == 0) && 4 * (i0 + 64) <= span_s;
-      keepbits = 0xF;
-      if (it < first_drop) {
-        if (!full_iter) {
-          keepbits = 0;
-  #pragma unroll
-          for (int e = 0; e < 4; ++e) {
-            const uint32_t r = 4 * i + e;
-            keepbits |= (i < nblk && r >= off0 && r < span_s) ? (1 << e) : 0;
-          }
-        }
-        const int32_t r0 = int32_t(4 * i) - int32_t(off0);
-        row = row0 + (r0 < 0 ? 0 : r0);
-      } else {
-        const uint64_t* mw = mask + int64_t(it) * 4;
-        keepbits = 0;
-  #pragma unroll
-        for (int e = 0; e < 4; ++e) keepbits |= int((mw[e] >> lane) & 1ull) << e;
-        int total;
-        row = row0 + running + wave_scan_counts(__popc(keepbits), &total);
-        running += total;
-      }
+      keepbits = kept_slots(mask, it, first_drop, i0, nblk, off0, span_s, running, &row);
+      row += row0;
     };
 
     // Per trip, 4 * kTrip keys per lane go through each step together (their
