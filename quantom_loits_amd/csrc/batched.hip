@@ -317,8 +317,39 @@ __device__ void build_tables(const BatchedParams& P, int inst, const InstanceDes
     sh.used = 0;
   }
 
+  // 0. global tables: nodes and densities staged in LDS when they fit (the
+  //    crowded-cell list and the counts take that LDS from step 4 on)
+  const Real* xs = x;
+  const Real* ds = s;
+  if constexpr (GLOBAL) {
+    if (global_build_stage_bytes(n, sizeof(Real)) > 0) {
+      Real* sx = reinterpret_cast<Real*>(lds);
+      Real* sd = sx + pad4(n);
+      for (int64_t t0 = tid; t0 < n; t0 += 4 * nt) {  // four nodes per thread in flight
+        Real a[4], b[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int64_t t = t0 + int64_t(u) * nt;
+          a[u] = t < n ? x[t] : Real(0);
+          b[u] = t < n ? s[t] : Real(0);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int64_t t = t0 + int64_t(u) * nt;
+          if (t < n) {
+            sx[t] = a[u];
+            sd[t] = b[u];
+          }
+        }
+      }
+      __syncthreads();
+      xs = sx;
+      ds = sd;
+    }
+  }
+
   // 1. norm and CDF
-  build_norm_cdf<Real, SEQUENTIAL>(x, s, n, acc, L.cdf, sh);
+  build_norm_cdf<Real, SEQUENTIAL>(xs, ds, n, acc, L.cdf, sh);
 
   // 2. thresholds, per-bin interpolation data, curve properties
   const int64_t pw = pow2_ceil(n);
@@ -332,7 +363,7 @@ __device__ void build_tables(const BatchedParams& P, int inst, const InstanceDes
     const uint32_t thr = threshold_of(c);
     L.srt[t] = thr;
     if (t > 0 && thr < threshold_of(L.cdf[t - 1])) bits &= ~1u;
-    const double xt = double(x[t]);
+    const double xt = double(xs[t]);
     const double ax = __builtin_fabs(xt);
     if (!(__builtin_isfinite(c) && ax >= 1e-30 && ax <= 1e29)) bits &= ~2u;
     if (!(xt > 0.0)) bits &= ~4u;
@@ -340,7 +371,7 @@ __device__ void build_tables(const BatchedParams& P, int inst, const InstanceDes
     if (t + 1 < n) {
       const double c1 = L.cdf[t + 1];
       const double inv_d = 1.0 / (c1 - c + kEpsilon);              // cpp/loits_core.cpp:193
-      const double m = (double(x[t + 1]) - xt) * inv_d;            // :194
+      const double m = (double(xs[t + 1]) - xt) * inv_d;            // :194
       if (!__builtin_isfinite(m)) bits &= ~2u;
       if constexpr (sizeof(Real) == 4) {
         const double r = double(thr) * 0x1.0p-24 - c;

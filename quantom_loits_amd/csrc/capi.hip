@@ -221,7 +221,8 @@ int make_plan(const loits_hip_instance* instances, int64_t n, int dtype,
       if (d.global_tables) {
         ++n_global;
         lds_f = std::max(lds_f, guide_bytes);
-        lds_t = std::max(lds_t, lds_global_build_bytes(d.guide_log2));
+        lds_t = std::max(lds_t, lds_global_build_bytes(s.n_nodes, d.guide_log2,
+                                                        dtype == LOITS_HIP_F32 ? 4 : 8));
       } else {
         lds_f = std::max(lds_f, build_bytes(d.guide_log2));
         lds_t = std::max(lds_t, build_bytes(d.guide_log2));

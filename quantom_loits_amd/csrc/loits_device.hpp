@@ -450,9 +450,19 @@ __host__ __device__ __forceinline__ LdsLayout lds_build_layout(int64_t n, int gu
 // Instances whose LDS image does not fit keep srt, coef and cdf in the global
 // blob ("global tables"); a work-group then stages only the guide in LDS.
 // Standalone build: crowded-cell list (G / 4 uint2) | cell counts (G uint32).
-__host__ __device__ __forceinline__ int64_t lds_global_build_bytes(int guide_log2) {
+// A global-table build first stages the curve's nodes and densities in LDS
+// (read from there by the norm, CDF and threshold steps) when both fit.
+__host__ __device__ __forceinline__ int64_t global_build_stage_bytes(int64_t n, int64_t real_size) {
+  const int64_t b = 2 * pad4(n) * real_size;
+  return b <= kMaxLds ? b : 0;
+}
+
+__host__ __device__ __forceinline__ int64_t lds_global_build_bytes(int64_t n, int guide_log2,
+                                                                   int64_t real_size) {
   const int64_t G = int64_t(1) << guide_log2;
-  return (G / 4) * 8 + G * 4;
+  const int64_t lists = (G / 4) * 8 + G * 4;
+  const int64_t stage = global_build_stage_bytes(n, real_size);
+  return lists > stage ? lists : stage;
 }
 
 // Bin-range VJP (curves whose VJP tables and histograms do not fit LDS): the
