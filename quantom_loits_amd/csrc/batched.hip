@@ -1132,6 +1132,17 @@ __device__ __forceinline__ int wave_scan_counts(int cnt, int* total) {
   return popc64(b0 & lt) + 2 * popc64(b1 & lt) + 4 * popc64(b2 & lt);
 }
 
+// A value every lane of the wave holds, in scalar registers (the segment
+// words below are loaded per lane; without this the loops that depend on them
+// count as divergent and keep their mask words and counters in vector registers).
+__device__ __forceinline__ int uniform_i32(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ uint64_t uniform_u64(uint64_t v) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane(uint32_t(v));
+  const uint32_t hi = __builtin_amdgcn_readfirstlane(uint32_t(v >> 32));
+  return (uint64_t(hi) << 32) | lo;
+}
+__device__ __forceinline__ int64_t uniform_i64(int64_t v) { return int64_t(uniform_u64(uint64_t(v))); }
+
 // Keep bits of a lane's four slots in iteration `it` of a segment of the VJP,
 // and the segment-relative packed row of its first kept slot (its other kept
 // slots follow in slot order).  Before the segment's first drop every live
@@ -1159,7 +1170,8 @@ __device__ __forceinline__ int kept_slots(const uint64_t* mask, int it, int firs
     return live;
   }
   const uint64_t* mw = mask + int64_t(it) * 4;
-  const uint64_t m0 = mw[0], m1 = mw[1], m2 = mw[2], m3 = mw[3];
+  const uint64_t m0 = uniform_u64(mw[0]), m1 = uniform_u64(mw[1]), m2 = uniform_u64(mw[2]),
+                 m3 = uniform_u64(mw[3]);
   const int kb = int((m0 >> lane) & 1ull) | (int((m1 >> lane) & 1ull) << 1) |
                  (int((m2 >> lane) & 1ull) << 2) | (int((m3 >> lane) & 1ull) << 3);
   uint32_t below = __builtin_amdgcn_mbcnt_lo(uint32_t(m0), 0u);
@@ -1892,8 +1904,8 @@ __global__ void __launch_bounds__(1024, sizeof(Real) == 4 ? 8 : 4)
     const Segment sg = segment_of(in.counter_base, ch.slot_lo, ch.slot_hi, w, P.segs);
     const int64_t seg = int64_t(blockIdx.x) * P.segs + w;
     const int64_t src = in.out_off + int64_t(sg.p_lo - in.counter_base);
-    const int64_t dst = P.chunk_row[blockIdx.x] + P.seg_row[seg];
-    const int first_drop = P.seg_drop[seg];
+    const int64_t dst = uniform_i64(P.chunk_row[blockIdx.x] + P.seg_row[seg]);
+    const int first_drop = uniform_i32(P.seg_drop[seg]);
     const uint32_t off0 = uint32_t(sg.p_lo - 4 * sg.qa);
     const uint32_t span = uint32_t(sg.p_hi - 4 * sg.qa);
     // rows of the iterations before the first drop: every live slot was kept
@@ -2311,8 +2323,8 @@ __device__ __forceinline__ void vjp_body(const BatchedParams& P, const Real* gra
   for (int w = next_segment(&sh_next); w < nw; w = next_segment(&sh_next)) {
     const Segment sg = segment_of(in.counter_base, ch.slot_lo, ch.slot_hi, w, nw);
     const int64_t seg = int64_t(c) * nw + w;
-    const int first_drop = P.seg_drop[seg];
-    const int64_t row0 = P.chunk_row[c] + P.seg_row[seg];
+    const int first_drop = uniform_i32(P.seg_drop[seg]);
+    const int64_t row0 = uniform_i64(P.chunk_row[c] + P.seg_row[seg]);
     const int64_t per_wave = int64_t(blocks_per_wave(uint64_t(chunk_blocks(in, ch)), nw) / 64);
     const uint64_t* mask = P.keepmask + ch.mask_off + int64_t(w) * per_wave * 4;
     const uint32_t nblk = uint32_t(sg.qb - sg.qa);
@@ -2598,8 +2610,8 @@ __device__ void fast_vjp_body(const BatchedParams& P, const float* grad, float* 
   for (int w = next_segment(&sh_next); w < nw; w = next_segment(&sh_next)) {
     const Segment sg = segment_of(in.counter_base, ch.slot_lo, ch.slot_hi, w, nw);
     const int64_t seg = int64_t(c) * nw + w;
-    const int first_drop = P.seg_drop[seg];
-    const int64_t row0 = P.chunk_row[c] + P.seg_row[seg];
+    const int first_drop = uniform_i32(P.seg_drop[seg]);
+    const int64_t row0 = uniform_i64(P.chunk_row[c] + P.seg_row[seg]);
     const int64_t per_wave = int64_t(blocks_per_wave(uint64_t(chunk_blocks(in, ch)), nw) / 64);
     const uint64_t* mask = P.keepmask + ch.mask_off + int64_t(w) * per_wave * 4;
     const uint32_t nblk = uint32_t(sg.qb - sg.qa);
@@ -2958,8 +2970,8 @@ __device__ void vjp_range_body(const BatchedParams& P, const Real* grad, Real* g
   for (int w = next_segment(&sh_next); w < nw; w = next_segment(&sh_next)) {
     const Segment sg = segment_of(in.counter_base, ch.slot_lo, ch.slot_hi, w, nw);
     const int64_t seg = int64_t(c) * nw + w;
-    const int first_drop = P.seg_drop[seg];
-    const int64_t row0 = P.chunk_row[c] + P.seg_row[seg];
+    const int first_drop = uniform_i32(P.seg_drop[seg]);
+    const int64_t row0 = uniform_i64(P.chunk_row[c] + P.seg_row[seg]);
     const int64_t per_wave = int64_t(blocks_per_wave(uint64_t(chunk_blocks(in, ch)), nw) / 64);
     const uint64_t* mask = P.keepmask + ch.mask_off + int64_t(w) * per_wave * 4;
     const uint32_t nblk = uint32_t(sg.qb - sg.qa);
