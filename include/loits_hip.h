@@ -270,8 +270,9 @@ typedef struct loits_hip_layout {
                                (too large for the forward's LDS build; built by a
                                pre-pass, slower) */
   int32_t range_instances;  /* instances whose VJP splits the bins into ranges of at
-                               most 4096 (tables and histograms too large for LDS);
-                               grid_backward counts one work item per chunk and range */
+                               most 7168 (float32) or 5120 (float64) bins (tables and
+                               histograms too large for LDS); grid_backward counts one
+                               work item per chunk and range */
   int32_t cell_instances;   /* instances on cell tables (table_format 0) */
   int32_t reserved;
   int64_t backward_plain;   /* work items of the instances without ranges (the cell-table

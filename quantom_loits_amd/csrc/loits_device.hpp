@@ -471,9 +471,9 @@ __host__ __device__ __forceinline__ int64_t lds_global_build_bytes(int64_t n, in
 // cell_bin[kRangeCells + 4] | thr[W + 4] | [float64: C_j per bin, 8 B] | two
 // 64-bit histograms[W].  float32 sums exact integers (no per-bin data), so its
 // ranges hold up to kRangeBinsF32 bins, float64's kRangeBins.
-constexpr int kRangeBins = 4096;
-constexpr int kRangeBinsF32 = 6144;
-constexpr int kRangeCellsLog2 = 13;
+constexpr int kRangeBins = 5120;
+constexpr int kRangeBinsF32 = 7168;
+constexpr int kRangeCellsLog2 = 12;
 constexpr int kRangeCells = 1 << kRangeCellsLog2;
 
 __host__ __device__ __forceinline__ int vjp_range_count(int64_t n, bool f32) {

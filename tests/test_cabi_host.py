@@ -218,7 +218,7 @@ def test_large_curves_use_global_tables(lib):
     assert lib.loits_hip_batched_layout(inst, 3, 1, C.byref(tun), C.byref(lay)) == 0
     assert lay.global_instances == 2
     assert lay.lds_forward <= 160 * 1024 and lay.lds_backward <= 160 * 1024
-    # float32 VJP bin ranges of at most 6144 bins: 6000 nodes -> 1, 16384 -> 3 work items
+    # float32 VJP bin ranges of at most 7168 bins: 6000 nodes -> 1, 16384 -> 3 work items
     assert lay.range_instances == 2 and lay.grid_backward == 1 + 1 + 3
     assert lay.backward_plain == 1
 

@@ -716,7 +716,7 @@ def _big_curves(kind, rng):
                                         ("nonmono", torch.float64)])
 def test_batched_vjp_bin_ranges_match_oracle(oracle_port, hip_device, kind, dtype):
     """Curves whose VJP tables and histograms exceed LDS: the VJP splits the bins
-    into ranges of at most 6144 (float32, exact integer sums) or 4096 (float64)
+    into ranges of at most 7168 (float32, exact integer sums) or 5120 (float64)
     bins, one work-group per chunk and range; a curve that is not monotone takes
     the global-table VJP instead."""
     from quantom_loits_amd.batched import BatchedLOITS, Tuning
