@@ -955,15 +955,17 @@ template <typename Real, bool SEQUENTIAL>
 __global__ void __launch_bounds__(1024) tables_kernel(BatchedParams P, int only_global) {
   extern __shared__ __align__(16) uint8_t lds[];
   __shared__ BuildShared sh;
-  const InstanceDesc in = P.inst[blockIdx.x];
+  // only_global: the global-table instances of global_list (most nodes first)
+  const int i = only_global ? P.global_list[blockIdx.x] : int(blockIdx.x);
+  const InstanceDesc in = P.inst[i];
   if (in.global_tables)
-    build_tables<Real, SEQUENTIAL, true>(P, blockIdx.x, in, lds, true, sh);
+    build_tables<Real, SEQUENTIAL, true>(P, i, in, lds, true, sh);
   else if (only_global)
     return;
   else if (sizeof(Real) == 4 && in.fast_log2 > 0)
-    build_fast<SEQUENTIAL>(P, blockIdx.x, in, lds, true, sh, nullptr);
+    build_fast<SEQUENTIAL>(P, i, in, lds, true, sh, nullptr);
   else
-    build_tables<Real, SEQUENTIAL, false>(P, blockIdx.x, in, lds, true, sh);
+    build_tables<Real, SEQUENTIAL, false>(P, i, in, lds, true, sh);
 }
 
 // ---------------------------------------------------------------------------
@@ -3320,7 +3322,9 @@ hipError_t launch_tables(const BatchedParams& P, int threads, bool sequential, i
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(kern, dim3(P.n_instances), dim3(threads), size_t(lds), stream, P,
+  const int64_t groups = only_global ? P.n_global : P.n_instances;
+  if (groups == 0) return hipSuccess;
+  hipLaunchKernelGGL(kern, dim3(unsigned(groups)), dim3(threads), size_t(lds), stream, P,
                      only_global ? 1 : 0);
   return hipGetLastError();
 }

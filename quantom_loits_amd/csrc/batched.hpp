@@ -17,7 +17,8 @@ struct BatchedParams {
   const int32_t* fwd_order;  // chunks of the cell-table instances first, then the others
   const int32_t* fwd_queue;  // persistent forward: chunks, largest first
   const int32_t* vjp_queue;  // persistent VJP items: indices into vjp_items
-  int64_t n_fwd_items;
+  const int32_t* global_list;  // global-table instances, most nodes first (their build launch)
+  int64_t n_fwd_items, n_global;
   int64_t n_instances, n_chunks, n_vjp_items, n_vjp_plain;
   int64_t n_cell_chunks, n_vjp_cell;
   const void* x;

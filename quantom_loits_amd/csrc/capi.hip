@@ -88,6 +88,7 @@ struct Plan {
   std::vector<int32_t> fwd_order;  // chunks of the cell-table instances first
   std::vector<int32_t> fwd_queue;  // persistent forward: chunks, largest first
   std::vector<int32_t> vjp_queue;  // persistent VJP: indices into items
+  std::vector<int32_t> global_list;  // global-table instances, most nodes first
   int64_t n_plain = 0;
   loits_hip_layout layout;
 };
@@ -142,6 +143,7 @@ int make_plan(const loits_hip_instance* instances, int64_t n, int dtype,
   plan->fwd_order.clear();
   plan->fwd_queue.clear();
   plan->vjp_queue.clear();
+  plan->global_list.clear();
   for (int64_t i = 0; i < n; ++i) {
     const loits_hip_instance& s = instances[i];
     if (s.n_nodes < 2 || s.n_nodes > 32767)
@@ -220,6 +222,7 @@ int make_plan(const loits_hip_instance* instances, int64_t n, int dtype,
       const int64_t guide_bytes = (int64_t(2) << d.guide_log2) * 4;
       if (d.global_tables) {
         ++n_global;
+        plan->global_list.push_back(int32_t(i));
         lds_f = std::max(lds_f, guide_bytes);
         lds_t = std::max(lds_t, lds_global_build_bytes(s.n_nodes, d.guide_log2,
                                                         dtype == LOITS_HIP_F32 ? 4 : 8));
@@ -321,12 +324,17 @@ int make_plan(const loits_hip_instance* instances, int64_t n, int dtype,
     });
   }
   L.forward_items = int64_t(plan->fwd_queue.size());
+  // the global-table builds (a launch of their own): largest first
+  std::stable_sort(plan->global_list.begin(), plan->global_list.end(), [&](int32_t a, int32_t b) {
+    return plan->inst[size_t(a)].n_nodes > plan->inst[size_t(b)].n_nodes;
+  });
   L.desc_bytes = align16(n * int64_t(sizeof(InstanceDesc))) +
                  align16(L.n_chunks * int64_t(sizeof(ChunkDesc))) +
                  align16(int64_t(plan->items.size()) * int64_t(sizeof(VjpItem))) +
                  align16(L.n_chunks * int64_t(sizeof(int32_t))) +
                  align16(L.forward_items * int64_t(sizeof(int32_t))) +
-                 int64_t(plan->vjp_queue.size()) * int64_t(sizeof(int32_t));
+                 align16(int64_t(plan->vjp_queue.size()) * int64_t(sizeof(int32_t))) +
+                 int64_t(plan->global_list.size()) * int64_t(sizeof(int32_t));
   L.table_bytes = std::max<int64_t>(table_off, 16);
   L.lds_forward = lds_f;
   L.lds_backward = lds_b;
@@ -370,8 +378,13 @@ BatchedParams params_of(const loits_hip_batched_args* a) {
   P.fwd_order = reinterpret_cast<const int32_t*>(desc + order_at);
   const int64_t fq_at = order_at + align16(L.n_chunks * int64_t(sizeof(int32_t)));
   P.fwd_queue = reinterpret_cast<const int32_t*>(desc + fq_at);
-  P.vjp_queue = reinterpret_cast<const int32_t*>(
-      desc + fq_at + align16(L.forward_items * int64_t(sizeof(int32_t))));
+  const int64_t vq_at = fq_at + align16(L.forward_items * int64_t(sizeof(int32_t)));
+  P.vjp_queue = reinterpret_cast<const int32_t*>(desc + vq_at);
+  // the VJP queue holds every item in persistent mode, none otherwise
+  const int64_t vjp_queue_len = L.persistent ? int64_t(L.grid_backward) : 0;
+  P.global_list = reinterpret_cast<const int32_t*>(
+      desc + vq_at + align16(vjp_queue_len * int64_t(sizeof(int32_t))));
+  P.n_global = L.global_instances;
   P.n_fwd_items = L.forward_items;
   P.n_instances = L.n_instances;
   P.n_chunks = L.n_chunks;
@@ -517,9 +530,12 @@ int loits_hip_batched_encode(const loits_hip_instance* instances, int64_t n,
     const int64_t fq_at = order_at + align16(int64_t(plan.fwd_order.size() * sizeof(int32_t)));
     if (!plan.fwd_queue.empty())
       std::memcpy(out + fq_at, plan.fwd_queue.data(), plan.fwd_queue.size() * sizeof(int32_t));
+    const int64_t vq_at = fq_at + align16(int64_t(plan.fwd_queue.size() * sizeof(int32_t)));
     if (!plan.vjp_queue.empty())
-      std::memcpy(out + fq_at + align16(int64_t(plan.fwd_queue.size() * sizeof(int32_t))),
-                  plan.vjp_queue.data(), plan.vjp_queue.size() * sizeof(int32_t));
+      std::memcpy(out + vq_at, plan.vjp_queue.data(), plan.vjp_queue.size() * sizeof(int32_t));
+    if (!plan.global_list.empty())
+      std::memcpy(out + vq_at + align16(int64_t(plan.vjp_queue.size() * sizeof(int32_t))),
+                  plan.global_list.data(), plan.global_list.size() * sizeof(int32_t));
     return LOITS_HIP_OK;
   } catch (const std::exception& e) {
     return fail(LOITS_HIP_ERUNTIME, std::string("batched_encode: ") + e.what());
