@@ -1422,17 +1422,17 @@ struct CellLookup {
           b[e].x, static_cast<float>(static_cast<int32_t>(k[e] - __float_as_uint(b[e].z))),
           b[e].y);
     if (WINDOW && classed) {  // the bin's class decides, except in straddling bins
-      int check = 0;
+      uint32_t least = 0xFFFFFFFFu;  // a zero class marks a straddling bin
 #pragma unroll
       for (int e = 0; e < NK; ++e) {
         const uint32_t cls = __float_as_uint(b[e].w);
         keep |= int(cls & 1u) << e;
-        check |= (cls == 0u ? 1 : 0) << e;
+        least = cls < least ? cls : least;
       }
-      if (ballot(check != 0) != 0ull) {
+      if (ballot(least == 0u) != 0ull) {
 #pragma unroll
         for (int e = 0; e < NK; ++e)
-          if ((check >> e) & 1) decide(e);
+          if (__float_as_uint(b[e].w) == 0u) decide(e);
       }
     } else {
 #pragma unroll
